@@ -1,0 +1,265 @@
+"""ctypes binding of the CPU oracle (oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  The product package never imports it.
+All arrays use the reference layouts (AoS q with 10 vars per node, etc.).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+
+class OMesh(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nbnode", C.c_int), ("nedge", C.c_int),
+                ("edges", ip), ("edge_a", dp), ("bedges", ip), ("bedge_a", dp), ("bfactag", ip),
+                ("xyz", dp), ("vol", dp), ("nfactag", C.c_int), ("bc_type", ip), ("qref", dp)]
+
+
+class OParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("sorder", C.c_int), ("nfirst", C.c_int), ("chi", C.c_double),
+                ("limiter", C.c_int), ("venkat_K", C.c_double), ("limiter_refresh", C.c_int),
+                ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double),
+                ("cfl_ramp", C.c_int), ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int),
+                ("clip_jacobi", C.c_int)]
+
+
+class OCrs(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nblocks", C.c_int), ("ia", ip), ("ja", ip), ("iau", ip), ("pv", ip),
+                ("M", dp), ("e_lr", ip), ("e_rl", ip)]
+
+
+class OSolver(C.Structure):
+    _fields_ = [("m", C.POINTER(OMesh)), ("p", OParams), ("q", dp), ("grad", dp), ("l", dp), ("sw", dp),
+                ("b", dp), ("x", dp), ("dt", dp), ("A", OCrs), ("iter", C.c_int), ("cfl", C.c_double),
+                ("order", ip), ("t_jac", C.c_double), ("t_grad", C.c_double), ("t_res", C.c_double),
+                ("t_lin", C.c_double), ("t_upd", C.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            from . import build as _b
+            _b.build()
+        L = C.CDLL(path)
+        L.o_aux.argtypes = [dp, C.c_double]
+        L.o_roe_flux.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_int, dp]
+        L.o_roe_flux.restype = C.c_int
+        L.o_bc_state.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double]
+        L.o_update_bcs.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp]
+        L.o_lsq_moments.argtypes = [C.POINTER(OMesh), dp]
+        L.o_gradient.argtypes = [C.POINTER(OMesh), dp, dp, dp]
+        L.o_limiter.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, dp, dp]
+        L.o_residual.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), C.c_int, dp, dp, dp, dp, dp]
+        L.o_timestep.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, C.c_double, dp]
+        L.o_timestep.restype = C.c_double
+        L.o_cfl.argtypes = [C.POINTER(OParams), C.c_int]
+        L.o_cfl.restype = C.c_double
+        L.o_crs_init.argtypes = [C.POINTER(OMesh), C.POINTER(OCrs)]
+        L.o_crs_free.argtypes = [C.POINTER(OCrs)]
+        L.o_jacobian.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, dp, C.POINTER(OCrs)]
+        L.o_prepare_sgs.argtypes = [C.POINTER(OCrs)]
+        L.o_sgs.argtypes = [C.POINTER(OCrs), C.c_int, dp, dp, ip]
+        L.o_sgs.restype = C.c_double
+        L.o_lu.argtypes = [dp, ip, C.c_int]
+        L.o_lu.restype = C.c_int
+        L.o_lu_solve.argtypes = [dp, dp, ip, dp, C.c_int]
+        L.o_apply_dq.argtypes = [dp, dp, C.c_double]
+        L.o_solver_init.argtypes = [C.POINTER(OSolver), C.POINTER(OMesh), C.POINTER(OParams), dp]
+        L.o_solver_step.argtypes = [C.POINTER(OSolver), dp, ip]
+        L.o_solver_step.restype = C.c_double
+        L.o_solver_free.argtypes = [C.POINTER(OSolver)]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Problem:
+    """Mesh + BCs + parameters in oracle form (keeps the numpy buffers alive)."""
+
+    def __init__(self, mesh, bc_type, qref, params, clip_jacobi=False):
+        self.keep = dict(edges=_i32(mesh.edges), edge_a=_f64(mesh.edge_a), bedges=_i32(mesh.bedges),
+                         bedge_a=_f64(mesh.bedge_a), bfactag=_i32(mesh.bfactag), xyz=_f64(mesh.xyz),
+                         vol=_f64(mesh.vol), bc_type=_i32(bc_type), qref=_f64(qref))
+        k = self.keep
+        self.nnode, self.nbnode, self.nedge = mesh.nnode, mesh.nbedge, mesh.nedge
+        self.m = OMesh(mesh.nnode, mesh.nbedge, mesh.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
+                       _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), len(k["bc_type"]),
+                       _i(k["bc_type"]), _d(k["qref"]))
+        self.params = params
+        self.p = OParams(params.gamma, params.sorder, params.nfirst, params.chi, params.limiter, params.venkat_K,
+                         params.limiter_refresh, params.limiter_freeze, params.cfl, params.cfl_start,
+                         params.cfl_ramp, params.nsgs, params.symmetry2d, params.jac_freq, int(clip_jacobi))
+
+    @classmethod
+    def from_case(cls, case, clip_jacobi=False):
+        return cls(case.mesh, case.bc_type, case.qref, case.params, clip_jacobi)
+
+    # --- single phases (arrays in reference AoS layout) ---
+    def update_bcs(self, q):
+        q = _f64(q).copy()
+        lib().o_update_bcs(C.byref(self.m), C.byref(self.p), _d(q))
+        return q
+
+    def lsq_moments(self):
+        sw = np.zeros((self.nnode, 6))
+        lib().o_lsq_moments(C.byref(self.m), _d(sw))
+        return sw
+
+    def gradient(self, q, sw=None):
+        if sw is None:
+            sw = self.lsq_moments()
+        g = np.zeros((self.nnode, 27))
+        lib().o_gradient(C.byref(self.m), _d(_f64(q)), _d(_f64(sw)), _d(g))
+        return g
+
+    def limiter(self, q, grad):
+        l = np.ones((self.nnode, 5))
+        lib().o_limiter(C.byref(self.m), C.byref(self.p), _d(_f64(q)), _d(_f64(grad)), _d(l))
+        return l
+
+    def residual(self, q, grad=None, l=None, it=1):
+        if grad is None:
+            grad = np.zeros((self.nnode, 27))
+        if l is None:
+            l = np.ones((self.nnode, 5))
+        b = np.zeros((self.nnode, 5))
+        norms = np.zeros(6)
+        lib().o_residual(C.byref(self.m), C.byref(self.p), it, _d(_f64(q)), _d(_f64(grad)), _d(_f64(l)), _d(b),
+                         _d(norms))
+        return b, norms
+
+    def timestep(self, q, cfl):
+        dt = np.zeros(self.nnode)
+        dtmin = lib().o_timestep(C.byref(self.m), C.byref(self.p), _d(_f64(q)), cfl, _d(dt))
+        return dt, dtmin
+
+    def jacobian(self, q, dt):
+        """Returns (ia, ja, iau, M[nblocks,5,5], q_after) -- q_after holds the
+        phantom states the boundary Jacobian re-evaluated."""
+        A = OCrs()
+        lib().o_crs_init(C.byref(self.m), C.byref(A))
+        q = _f64(q).copy()
+        lib().o_jacobian(C.byref(self.m), C.byref(self.p), _d(q), _d(_f64(dt)), C.byref(A))
+        nb = A.nblocks
+        ia = np.ctypeslib.as_array(A.ia, (self.nnode + 1,)).copy()
+        ja = np.ctypeslib.as_array(A.ja, (nb,)).copy()
+        iau = np.ctypeslib.as_array(A.iau, (self.nnode,)).copy()
+        Mm = np.ctypeslib.as_array(A.M, (nb * 25,)).copy().reshape(nb, 5, 5)
+        lib().o_crs_free(C.byref(A))
+        return ia, ja, iau, Mm, q
+
+    def sgs(self, q, dt, b, nsgs, order=None):
+        """Jacobian + PrepareSGS + SGS from x0 = 0; returns x."""
+        A = OCrs()
+        lib().o_crs_init(C.byref(self.m), C.byref(A))
+        qc = _f64(q).copy()
+        lib().o_jacobian(C.byref(self.m), C.byref(self.p), _d(qc), _d(_f64(dt)), C.byref(A))
+        lib().o_prepare_sgs(C.byref(A))
+        x = np.zeros((self.nnode, 5))
+        o = None if order is None else _i32(order)
+        lib().o_sgs(C.byref(A), nsgs, _d(_f64(b)), _d(x), None if o is None else _i(o))
+        lib().o_crs_free(C.byref(A))
+        return x
+
+
+def sgs_on_matrix(ia, ja, iau, M, b, nsgs, order=None):
+    """PrepareSGS + SGS (x0 = 0) on a given block-CRS matrix (reference layout)."""
+    ia, ja, iau = _i32(ia), _i32(ja), _i32(iau)
+    Mc = _f64(M).copy().reshape(-1)
+    nn = len(ia) - 1
+    pv = np.zeros(5 * nn, dtype=np.int32)
+    A = OCrs(nn, len(ja), _i(ia), _i(ja), _i(iau), _i(pv), _d(Mc), None, None)
+    lib().o_prepare_sgs(C.byref(A))
+    x = np.zeros((nn, 5))
+    o = None if order is None else _i32(order)
+    lib().o_sgs(C.byref(A), nsgs, _d(_f64(b)), _d(x), None if o is None else _i(o))
+    return x
+
+
+def roe_flux(QL, QR, avec, gamma=1.4, symmetry2d=-1):
+    f = np.zeros(5)
+    lib().o_roe_flux(_d(_f64(QL)), _d(_f64(QR)), _d(_f64(avec)), 0.0, gamma, symmetry2d, _d(f))
+    return f
+
+
+def lu(a):
+    a = _f64(a).copy()
+    p = np.zeros(a.shape[0], dtype=np.int32)
+    bad = lib().o_lu(_d(a), _i(p), a.shape[0])
+    return a, p, bad
+
+
+def lu_solve(a, p, b):
+    b = _f64(b).copy()
+    x = np.zeros_like(b)
+    lib().o_lu_solve(_d(_f64(a)), _d(b), _i(_i32(p)), _d(x), len(b))
+    return b
+
+
+class Solver:
+    """The oracle's steady solve loop (PreTimeAdvance + NewtonIterate)."""
+
+    def __init__(self, problem: Problem, q0, order=None):
+        self.pb = problem
+        self.s = OSolver()
+        self._q0 = _f64(q0).copy()
+        self._order = None if order is None else _i32(order)
+        lib().o_solver_init(C.byref(self.s), C.byref(problem.m), C.byref(problem.p), _d(self._q0))
+        if self._order is not None:
+            self.s.order = _i(self._order)
+
+    def step(self):
+        norms = np.zeros(6)
+        nz = C.c_int(0)
+        r = lib().o_solver_step(C.byref(self.s), _d(norms), C.byref(nz))
+        return r, norms, nz.value
+
+    @property
+    def q(self):
+        n = self.pb.nnode + self.pb.nbnode
+        return np.ctypeslib.as_array(self.s.q, (n * 10,)).copy().reshape(n, 10)
+
+    @property
+    def iter(self):
+        return self.s.iter
+
+    def timers(self):
+        s = self.s
+        return dict(jacobian=s.t_jac, gradient=s.t_grad, residual=s.t_res, linear=s.t_lin, update=s.t_upd)
+
+    def close(self):
+        if self.s is not None:
+            lib().o_solver_free(C.byref(self.s))
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,120 @@
+/*
+ * CPU oracle (test infrastructure only -- never linked into the product).
+ *
+ * A clean-room FP64 restatement of the reference ucs.x implicit edge-loop
+ * hot path in the reference's own layouts: AoS state q[node*10 + k]
+ * (5 conservative + 5 auxiliary T,P,u,v,w), gradients
+ * qgrad[node*27 + term*3 + dir], limiter l[node*5+k], residual b[node*5+k],
+ * block-CRS M[blk*25 + r*5 + c] with the diagonal first in each row, and the
+ * reference's edge-ordered scatter.  Each function cites the reference
+ * file:line it follows.  It is the parity checker for the HIP path and the
+ * CPU baseline timed by bench.py.  It must not be used by the product path.
+ */
+#ifndef PCFD_ORACLE_H
+#define PCFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O_NEQN 5
+#define O_NVARS 10
+#define O_NTERMS 9
+
+/* BC type ids: ucs/bc_defines.h:4-30 */
+enum {
+  O_BC_PARALLEL = 0, O_BC_DIRICHLET = 1, O_BC_NEUMANN = 2, O_BC_IMPERMEABLE = 3, O_BC_NOSLIP = 4,
+  O_BC_FARFIELD_VISCOUS = 5, O_BC_FARFIELD = 6, O_BC_SONIC_INFLOW = 7, O_BC_SONIC_OUTFLOW = 8,
+  O_BC_SYMMETRY = 9
+};
+
+typedef struct {
+  int nnode, nbnode, nedge;
+  const int* edges;      /* [nedge*2] */
+  const double* edge_a;  /* [nedge*4] */
+  const int* bedges;     /* [nbnode*2] (node, phantom) */
+  const double* bedge_a; /* [nbnode*4] */
+  const int* bfactag;    /* [nbnode] */
+  const double* xyz;     /* [nnode*3] */
+  const double* vol;     /* [nnode] */
+  int nfactag;
+  const int* bc_type;    /* [nfactag] */
+  const double* qref;    /* [nfactag*10] reference state (with aux vars) */
+} o_mesh;
+
+typedef struct {
+  double gamma;
+  int sorder, nfirst;           /* spatialOrder, firstOrderSteps */
+  double chi;
+  int limiter;                  /* 0 off, 1 Barth, 2 Venkat, 3 modified Venkat */
+  double venkat_K;              /* K of the modified Venkat limiter (reference: 1) */
+  int limiter_refresh, limiter_freeze;
+  double cfl, cfl_start;
+  int cfl_ramp;
+  int nsgs;                     /* 0 -> explicit */
+  int symmetry2d;
+  int jac_freq;
+  int clip_jacobi;              /* 0: reference sequential pressure clip; 1: all edges see the pre-clip limiter */
+} o_params;
+
+typedef struct {
+  int nnode, nblocks;
+  int* ia;   /* [nnode+1] */
+  int* ja;   /* [nblocks] */
+  int* iau;  /* [nnode] */
+  int* pv;   /* [nnode*5] */
+  double* M; /* [nblocks*25] */
+  int* e_lr; /* per edge: block index of (n0 row, n1 col) */
+  int* e_rl; /* per edge: block index of (n1 row, n0 col) */
+} o_crs;
+
+void o_aux(double* Q, double gamma);
+int o_roe_flux(const double* QL, const double* QR, const double* avec, double vdotn, double gamma,
+               int symmetry2d, double* flux);
+void o_bc_state(int bctype, double* QL, double* QR, const double* Qref, const double* avec, double gamma);
+
+void o_update_bcs(const o_mesh* m, const o_params* p, double* q);
+void o_lsq_moments(const o_mesh* m, double* sw);
+void o_gradient(const o_mesh* m, const double* q, const double* sw, double* grad);
+void o_limiter(const o_mesh* m, const o_params* p, const double* q, const double* grad, double* l);
+void o_residual(const o_mesh* m, const o_params* p, int iter, const double* q, const double* grad,
+                const double* l, double* b, double* norms /* [6] */);
+double o_timestep(const o_mesh* m, const o_params* p, const double* q, double cfl, double* dt);
+double o_cfl(const o_params* p, int iter);
+
+int o_crs_init(const o_mesh* m, o_crs* A);
+void o_crs_free(o_crs* A);
+void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt, o_crs* A);
+void o_prepare_sgs(o_crs* A);
+double o_sgs(const o_crs* A, int nsgs, const double* b, double* x, const int* order);
+int o_lu(double* a, int* piv, int n);
+void o_lu_solve(const double* a, double* b, const int* piv, double* x, int n);
+void o_apply_dq(const double* dq, double* Q, double gamma);
+
+/* Full solver state for the steady Newton / explicit loop. */
+typedef struct {
+  const o_mesh* m;
+  o_params p;
+  double* q;     /* (nnode+nbnode)*10 */
+  double* grad;  /* nnode*27 */
+  double* l;     /* nnode*5 */
+  double* sw;    /* nnode*6 */
+  double* b;     /* nnode*5 */
+  double* x;     /* nnode*5 */
+  double* dt;    /* nnode */
+  o_crs A;
+  int iter;
+  double cfl;
+  const int* order; /* SGS row order (NULL = natural) */
+  double t_jac, t_grad, t_res, t_lin, t_upd; /* phase timers (s) */
+} o_solver;
+
+int o_solver_init(o_solver* s, const o_mesh* m, const o_params* p, const double* q0);
+/* one outer iteration: PreTimeAdvance + NewtonIterate + iter++ ; returns residGlobal */
+double o_solver_step(o_solver* s, double* norms, int* nzeroed);
+void o_solver_free(o_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,70 @@
+"""Build the native libraries in-tree.
+
+* ``libpcfd_mesh.so`` -- host C++ mesh generator / median-dual metrics (g++).
+* ``libpcfd.so``      -- HIP kernels for gfx950 + the C ABI (hipcc).
+
+Both land next to this file so they travel with the source tree.  Building
+never needs a GPU (hipcc cross-compiles for gfx950).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+MESH_SRC = ["mesh.cpp"]
+HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_jac.hip", "kernels_sgs.hip",
+           "kernels_misc.hip", "solver.hip"]
+
+# gfx950 only.  Packed-FP32 instructions are disabled for all device code.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-fno-gpu-rdc",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:6]) + " ...")
+    return r
+
+
+def build_mesh() -> str:
+    out = os.path.join(HERE, "libpcfd_mesh.so")
+    srcs = [os.path.join(CSRC, s) for s in MESH_SRC]
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INCLUDE, *srcs, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_hip() -> str:
+    out = os.path.join(HERE, "libpcfd.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SRC]
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    _run([hipcc, *HIPCC_FLAGS, "-I", INCLUDE, *srcs,
+          os.path.join(CSRC, "mesh.cpp"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all() -> None:
+    build_mesh()
+    build_hip()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    if "mesh" in what:
+        print(build_mesh())
+    elif "hip" in what:
+        print(build_hip())
+    else:
+        build_all()

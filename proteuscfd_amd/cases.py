@@ -1,0 +1,142 @@
+"""Benchmark cases: meshes, boundary conditions, freestream states, parameters.
+
+Boundary-condition ids are the reference's BCTypes enum (ucs/bc_defines.h:4-30).
+The freestream is the reference's non-dimensional Q_inf
+(ucs/compressible.tcc:823-867): rho, rho*u, rho*v, rho*w, E = p/(g-1)+rho|u|^2/2
+with the auxiliary set T = g*p/rho, p, u, v, w (ucs/compressible.tcc:1229-1243).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import mesh as M
+
+# ucs/bc_defines.h:4-30
+BC_PARALLEL = 0
+BC_DIRICHLET = 1
+BC_NEUMANN = 2
+BC_IMPERMEABLE = 3
+BC_NOSLIP = 4
+BC_FARFIELD = 6
+BC_SONIC_INFLOW = 7
+BC_SONIC_OUTFLOW = 8
+BC_SYMMETRY = 9
+BC_ISOTHERMAL = 18
+
+NEQN = 5
+NVARS = 10
+
+
+def qinf(gamma: float, mach: float, rho: float = 1.0, p: float | None = None,
+         direction=(1.0, 0.0, 0.0)) -> np.ndarray:
+    """Freestream conservative + auxiliary state (10 values)."""
+    if p is None:
+        p = 1.0 / gamma
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    u, v, w = mach * d
+    E = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)
+    q = np.zeros(NVARS)
+    q[:5] = [rho, rho * u, rho * v, rho * w, E]
+    return aux(q, gamma)
+
+
+def aux(q: np.ndarray, gamma: float) -> np.ndarray:
+    """ComputeAuxiliaryVariables on a (..., 10) array (in place, returned)."""
+    q = np.asarray(q)
+    r = q[..., 0]
+    u, v, w = q[..., 1] / r, q[..., 2] / r, q[..., 3] / r
+    P = (gamma - 1.0) * (q[..., 4] - 0.5 * r * (u * u + v * v + w * w))
+    q[..., 6] = P
+    q[..., 5] = gamma * P / r
+    q[..., 7], q[..., 8], q[..., 9] = u, v, w
+    return q
+
+
+@dataclass
+class Params:
+    """Solver parameters (mirrors ucs/param.tcc keywords)."""
+    gamma: float = 1.4
+    sorder: int = 1               # spatialOrder
+    nfirst: int = 0               # firstOrderSteps
+    limiter: int = 0              # 0 off, 1 Barth, 2 Venkat, 3 modified Venkat
+    venkat_K: float = 1.0
+    limiter_refresh: int = 1
+    limiter_freeze: int = 999999
+    cfl: float = 5.0
+    cfl_start: float = 0.5
+    cfl_ramp: int = 0
+    nsgs: int = 5                 # numberSGS (0 = explicit)
+    symmetry2d: int = -1
+    jac_freq: int = 1
+    precision: int = 0            # 0 plain FP64, 1 FP32 Jacobian + SGS
+
+    @property
+    def chi(self) -> float:       # ucs/param.tcc:410-417
+        return 0.5 if self.sorder >= 3 else 0.0
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: "M.Mesh"
+    bc_type: np.ndarray           # [nfactag] int32
+    qref: np.ndarray              # [nfactag, 10]
+    qinf: np.ndarray              # [10]
+    params: Params
+    extra: dict = field(default_factory=dict)
+
+    def initial_state(self) -> np.ndarray:
+        """Uniform freestream on interior and phantom nodes (AoS, 10 vars)."""
+        n = self.mesh.nnode + self.mesh.nbedge
+        return np.tile(self.qinf, (n, 1)).copy()
+
+
+def wedge_case(nx=60, ny=30, nz=2, seed=1, jitter=0.05, mach=2.5, gamma=1.4, rcm=False,
+               params: Params | None = None, z_bc=BC_SYMMETRY) -> Case:
+    """Supersonic 15-degree wedge channel: sonic inflow (x=0), sonic outflow
+    (x=3), slip walls (y), symmetry (z)."""
+    m = M.Mesh.build(M.WEDGE, nx=nx, ny=ny, nz=nz, seed=seed, jitter=jitter, rcm=int(rcm))
+    q0 = qinf(gamma, mach)
+    bct = np.array([BC_PARALLEL, BC_SONIC_INFLOW, BC_SONIC_OUTFLOW, BC_IMPERMEABLE,
+                    BC_IMPERMEABLE, z_bc, z_bc], dtype=np.int32)
+    qref = np.tile(q0, (M.NFACTAG, 1))
+    if params is None:
+        params = Params(gamma=gamma, sorder=1, cfl=0.9, cfl_start=0.9, nsgs=0)
+    return Case("wedge", m, bct, qref, q0, params)
+
+
+def bump_case(n=100, seed=2, jitter=0.03, mach=0.5, gamma=1.4, rcm=True,
+              params: Params | None = None) -> Case:
+    """Subsonic bump in a unit cube: farfield on all sides, slip floor."""
+    m = M.Mesh.build(M.BUMP, nx=n, ny=n, nz=n, seed=seed, jitter=jitter, rcm=int(rcm))
+    q0 = qinf(gamma, mach)
+    bct = np.array([BC_PARALLEL, BC_FARFIELD, BC_FARFIELD, BC_FARFIELD, BC_FARFIELD,
+                    BC_IMPERMEABLE, BC_FARFIELD], dtype=np.int32)
+    qref = np.tile(q0, (M.NFACTAG, 1))
+    if params is None:
+        params = Params(gamma=gamma, sorder=2, limiter=3, venkat_K=1.0, cfl=50.0, cfl_start=1.0,
+                        cfl_ramp=50, nsgs=8)
+    return Case("bump", m, bct, qref, q0, params)
+
+
+def perturbed_state(case: Case, amp: float = 0.05, seed: int = 7) -> np.ndarray:
+    """Freestream plus smooth and random perturbations (for parity tests)."""
+    rng = np.random.default_rng(seed)
+    m = case.mesh
+    g = case.params.gamma
+    q = case.initial_state()
+    x = np.concatenate([m.xyz, np.zeros((m.nbedge, 3))], axis=0)
+    n = q.shape[0]
+    smooth = np.sin(2.1 * x[:, 0] + 0.3) * np.cos(1.7 * x[:, 1]) + 0.5 * np.sin(3.3 * x[:, 2])
+    rho = q[:, 0] * (1.0 + amp * smooth + 0.2 * amp * rng.uniform(-1, 1, n))
+    vel = q[:, 1:4] / q[:, :1] * (1.0 + amp * rng.uniform(-1, 1, (n, 3)))
+    vel[:, 1] += amp * 0.3 * smooth
+    p = q[:, 6] * (1.0 + amp * np.cos(1.3 * x[:, 0] - 0.4 * x[:, 2]) + 0.2 * amp * rng.uniform(-1, 1, n))
+    out = np.zeros_like(q)
+    out[:, 0] = rho
+    out[:, 1:4] = rho[:, None] * vel
+    out[:, 4] = p / (g - 1.0) + 0.5 * rho * (vel ** 2).sum(axis=1)
+    return aux(out, g)

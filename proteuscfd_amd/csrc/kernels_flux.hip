@@ -1,0 +1,551 @@
+// Edge-loop kernels of the residual path, written as node gathers.
+//
+// The reference loops over edges and scatters both edge contributions into
+// node arrays (ucs/driver.tcc:5-67, 274-306).  Here one thread owns one node
+// and visits its incident edges in ascending neighbour order -- the same
+// order in which the reference's edge loop (edges sorted by (n0, n1))
+// reaches that node -- evaluating every edge function with the canonical
+// (n0, n1) argument order.  Both ends of an edge therefore see bit-identical
+// edge values, sums come out in the reference's order, and no
+// floating-point atomics are needed.
+#include <stdexcept>
+
+#include "common.h"
+#include "physics.h"
+
+namespace pcfd {
+
+// ---------------------------------------------------------------------------
+// weighted LSQ moments: Kernel_LSQ_CoefficientsWeighted (ucs/gradient.tcc:460-503)
+__global__ void __launch_bounds__(kBlock) k_lsq_moments(int nnode, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                              const double* __restrict__ xyz, double* __restrict__ sw) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  double xi = xyz[i], yi = xyz[nnode + i], zi = xyz[2 * nnode + i];
+  for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+    int j = col[k];
+    double xj = xyz[j], yj = xyz[nnode + j], zj = xyz[2 * nnode + j];
+    double dx0, dx1, dx2;
+    if (i < j) { dx0 = xi - xj; dx1 = yi - yj; dx2 = zi - zj; }
+    else       { dx0 = xj - xi; dx1 = yj - yi; dx2 = zj - zi; }
+    double ds2 = 1.0 / (dx0 * dx0 + dx1 * dx1 + dx2 * dx2);
+    s[0] += dx0 * dx0 * ds2;
+    s[1] += dx0 * dx1 * ds2;
+    s[2] += dx0 * dx2 * ds2;
+    s[3] += dx1 * dx1 * ds2;
+    s[4] += dx1 * dx2 * ds2;
+    s[5] += dx2 * dx2 * ds2;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sw[k * nnode + i] = s[k];
+}
+
+void launch_lsq_moments(Ctx& c) {
+  hipLaunchKernelGGL(k_lsq_moments, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.d.row_ptr,
+                     c.d.col, c.d.xyz, c.d.sw);
+  PCFD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// UpdateBCs (ucs/bc.tcc:1399-1457): one thread per boundary node walks its
+// half-edges in half-edge order, so a Dirichlet/sonic-inflow overwrite of the
+// node state is seen by the node's later half-edges exactly as in the
+// reference's sequential loop.  Each phantom is written by its owner only.
+__global__ void __launch_bounds__(kBlock) k_update_bcs(int nbnd, int nnode, int ntot, int ntag, const int* __restrict__ bnd_nodes,
+                             const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                             const int* __restrict__ he_tag, const double* __restrict__ he_a, int nbnode,
+                             const int* __restrict__ bc_type, const double* __restrict__ qref, double gamma,
+                             double* __restrict__ q) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbnd) return;
+  int i = bnd_nodes[t];
+  double QL[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) QL[k] = q[k * ntot + i];
+  double Q0[5] = {QL[0], QL[1], QL[2], QL[3], QL[4]};
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    int h = he_id[p];
+    int tag = he_tag[h];
+    int ph = nnode + h;
+    double a[4] = {he_a[h], he_a[nbnode + h], he_a[2 * nbnode + h], he_a[3 * nbnode + h]};
+    double QR[5], Qr[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      QR[k] = q[k * ntot + ph];
+      Qr[k] = qref[k * ntag + tag];
+    }
+    double pref = qref[5 * ntag + tag];
+    const int type = bc_type[tag];
+    if (is_dirichlet(type)) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        QL[k] = Qr[k];
+        q[k * ntot + ph] = Qr[k];
+      }
+    } else if (bc_state(type, QL, QR, Qr, pref, a, gamma)) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) q[k * ntot + ph] = QR[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    if (QL[k] != Q0[k]) q[k * ntot + i] = QL[k];
+}
+
+void launch_update_bcs(Ctx& c) {
+  if (c.nbnd == 0) return;
+  hipLaunchKernelGGL(k_update_bcs, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.nnode, c.ntot,
+                     c.ntag, c.d.bnd_nodes, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
+                     c.d.qref, c.prm.gamma, c.d.q);
+  PCFD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// gradient terms of the compressible set: rho, rho u, rho v, rho w, E, T, u, v, w
+// (GetGradientsLocation, ucs/compressible.tcc:1008-1027)
+template <int NT>
+__device__ __forceinline__ void grad_terms(const double* Q, double gm1, double gamma, double* t) {
+  t[0] = Q[0]; t[1] = Q[1]; t[2] = Q[2]; t[3] = Q[3]; t[4] = Q[4];
+  if (NT > 5) {
+    t[5] = gamma * comp_pressure(Q, gm1) / Q[0];
+    t[6] = Q[1] / Q[0];
+    t[7] = Q[2] / Q[0];
+    t[8] = Q[3] / Q[0];
+  }
+}
+
+// Gradient::Compute, weighted LSQ (ucs/gradient.tcc:56-112, 250-316) and the
+// symmetry-plane projection Bkernel_Symmetry_Fix (:544-565).
+template <int NT>
+__global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                           const double* __restrict__ xyz, const double* __restrict__ sw,
+                           const double* __restrict__ q, const int* __restrict__ he_ptr,
+                           const int* __restrict__ he_id, const int* __restrict__ he_tag,
+                           const double* __restrict__ he_a, int nbnode, const int* __restrict__ bc_type,
+                           double gamma, double* __restrict__ grad) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const double gm1 = gamma - 1.0;
+  double g[NT * 3];
+#pragma unroll
+  for (int k = 0; k < NT * 3; ++k) g[k] = 0.0;
+  double s[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s[k] = sw[k * nnode + i];
+  double Qi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Qi[k] = q[k * ntot + i];
+  double ti[NT];
+  grad_terms<NT>(Qi, gm1, gamma, ti);
+  double xi = xyz[i], yi = xyz[nnode + i], zi = xyz[2 * nnode + i];
+  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+    int j = col[p];
+    double Qj[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Qj[k] = q[k * ntot + j];
+    double tj[NT];
+    grad_terms<NT>(Qj, gm1, gamma, tj);
+    double xj = xyz[j], yj = xyz[nnode + j], zj = xyz[2 * nnode + j];
+    const bool left = i < j;
+    double d[3];
+    if (left) { d[0] = xi - xj; d[1] = yi - yj; d[2] = zi - zj; }
+    else      { d[0] = xj - xi; d[1] = yj - yi; d[2] = zj - zi; }
+    double dx2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double weight = 1.0 / sqrt(dx2);
+    d[0] *= weight; d[1] *= weight; d[2] *= weight;
+    if (!left) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
+    double we[3];
+    lsq_coeff(s, d, we);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double dq = left ? weight * (tj[t] - ti[t]) : weight * (ti[t] - tj[t]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        if (left) g[3 * t + k] += -we[k] * dq;
+        else      g[3 * t + k] += +we[k] * dq;
+      }
+    }
+  }
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    int h = he_id[p];
+    if (bc_type[he_tag[h]] != BC_SYMMETRY) continue;
+    double a0 = he_a[h], a1 = he_a[nbnode + h], a2 = he_a[2 * nbnode + h];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double dt = g[3 * t] * a0 + g[3 * t + 1] * a1 + g[3 * t + 2] * a2;
+      g[3 * t] -= dt * a0;
+      g[3 * t + 1] -= dt * a1;
+      g[3 * t + 2] -= dt * a2;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = g[k];
+}
+
+void launch_gradient(Ctx& c, int nterms) {
+  if (nterms == 9) {
+    hipLaunchKernelGGL(k_gradient<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
+                       c.d.row_ptr, c.d.col, c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag,
+                       c.d.he_a, c.nbnode, c.d.bc_type, c.prm.gamma, c.d.grad);
+  } else {
+    hipLaunchKernelGGL(k_gradient<5>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
+                       c.d.row_ptr, c.d.col, c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag,
+                       c.d.he_a, c.nbnode, c.d.bc_type, c.prm.gamma, c.d.grad);
+  }
+  PCFD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Limiter::Compute (ucs/limiters.tcc:52-132): neighbour min/max with both
+// bounds starting at 0 (:59-63), then the per-edge-end limiter value, the
+// minimum over the node's edges.  Writes the pre-clip limiter to lim2.
+__global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int type, double K3, double chi,
+                          const int* __restrict__ row_ptr, const int* __restrict__ col,
+                          const double* __restrict__ xyz, const double* __restrict__ vol,
+                          const double* __restrict__ q, const double* __restrict__ grad,
+                          double* __restrict__ lim2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  double qi[5], qmin[5], qmax[5], phi[5], gi[15];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    qi[k] = q[k * ntot + i];
+    qmin[k] = 0.0;
+    qmax[k] = 0.0;
+    phi[k] = 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
+  const int p0 = row_ptr[i], p1 = row_ptr[i + 1];
+  for (int p = p0; p < p1; ++p) {
+    int j = col[p];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double v = q[k * ntot + j];
+      qmax[k] = dmax(qmax[k], v);
+      qmin[k] = dmin(qmin[k], v);
+    }
+  }
+  double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
+  const double ep2 = 6.0 * 3.141592653589793 * vol[i] * K3;
+  for (int p = p0; p < p1; ++p) {
+    int j = col[p];
+    const bool left = i < j;
+    double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
+    double dx[3];
+    // dx = 0.5 (xR - xL) for the left end, its negation for the right end
+    if (left) { dx[0] = 0.5 * (xj[0] - xi[0]); dx[1] = 0.5 * (xj[1] - xi[1]); dx[2] = 0.5 * (xj[2] - xi[2]); }
+    else { dx[0] = -(0.5 * (xi[0] - xj[0])); dx[1] = -(0.5 * (xi[1] - xj[1])); dx[2] = -(0.5 * (xi[2] - xj[2])); }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double qj = q[k * ntot + j];
+      double dQ = left ? (qj - qi[k]) : -(qi[k] - qj);
+      double Qh = extrap1(qi[k], dQ, gi[3 * k], gi[3 * k + 1], gi[3 * k + 2], dx, chi, 1.0);
+      double t;
+      if (type == 3) {
+        double DM = Qh - qi[k];
+        double DP;
+        if (left) DP = (Qh > qi[k]) ? qmax[k] - qi[k] : ((Qh < qi[k]) ? qmin[k] - qi[k] : 0.0);
+        else DP = (Qh > qi[k]) ? qmax[k] - qi[k] : qmin[k] - qi[k];
+        t = (DP * DP + ep2 + 2.0 * DM * DP) / (DP * DP + 2.0 * DM * DM + DM * DP + ep2);
+      } else {
+        t = 1.0;
+        if (Qh > qi[k]) t = (qmax[k] - qi[k]) / (Qh - qi[k]);
+        else if (Qh < qi[k]) t = (qmin[k] - qi[k]) / (Qh - qi[k]);
+        if (type == 2) {
+          t = (t * t + 2.0 * t) / (t * t + t + 2.0);
+        } else {
+          t = dmax(0.0, t);
+          t = dmin(1.0, t);
+        }
+      }
+      phi[k] = dmin(phi[k], t);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) lim2[k * nnode + i] = phi[k];
+}
+
+// Kernel_PressureClip (ucs/limiters.tcc:736-815) with every edge seeing the
+// pre-clip limiter (the reference's sequential edge loop lets an earlier
+// edge's zeroing change a later edge's extrapolation; the two agree whenever
+// no clip chain forms), then negative limiters -> 0 (:118-124).
+__global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, double chi, double gamma, const int* __restrict__ row_ptr,
+                                const int* __restrict__ col, const double* __restrict__ xyz,
+                                const double* __restrict__ q, const double* __restrict__ grad,
+                                const double* __restrict__ lim2, double* __restrict__ lim) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  bool zero = false;
+  double qi[5], li[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    qi[k] = q[k * ntot + i];
+    li[k] = lim2[k * nnode + i];
+  }
+  double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
+  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+    int j = col[p];
+    const bool left = i < j;
+    double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
+    const int L = left ? i : j, R = left ? j : i;
+    const double* xL = left ? xi : xj;
+    const double* xR = left ? xj : xi;
+    double dx[3] = {0.5 * (xR[0] - xL[0]), 0.5 * (xR[1] - xL[1]), 0.5 * (xR[2] - xL[2])};
+    double mdx[3] = {-dx[0], -dx[1], -dx[2]};
+    double QL[5], QR[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double qL = q[k * ntot + L], qR = q[k * ntot + R];
+      double dQ = qR - qL;
+      QL[k] = extrap1(qL, dQ, grad[(3 * k) * nnode + L], grad[(3 * k + 1) * nnode + L],
+                      grad[(3 * k + 2) * nnode + L], dx, chi, lim2[k * nnode + L]);
+      QR[k] = extrap1(qR, -dQ, grad[(3 * k) * nnode + R], grad[(3 * k + 1) * nnode + R],
+                      grad[(3 * k + 2) * nnode + R], mdx, chi, lim2[k * nnode + R]);
+    }
+    if (left && bad_state(QL, gamma)) zero = true;
+    if (!left && bad_state(QR, gamma)) zero = true;
+    double Qr[5];
+    roe_state(QL, QR, gamma, Qr);
+    if (bad_state(Qr, gamma)) zero = true;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double v = zero ? 0.0 : li[k];
+    if (v < 0.0) v = 0.0;
+    lim[k * nnode + i] = v;
+  }
+}
+
+void launch_limiter(Ctx& c) {
+  const pcfd_params& p = c.prm;
+  double K3 = p.venkat_K * p.venkat_K * p.venkat_K;
+  hipLaunchKernelGGL(k_limiter, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.limiter, K3,
+                     p.chi, c.d.row_ptr, c.d.col, c.d.xyz, c.d.vol, c.d.q, c.d.grad, c.d.lim2);
+  PCFD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
+                     p.gamma, c.d.row_ptr, c.d.col, c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
+  PCFD_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// SpatialResidual (ucs/residual.tcc:65-122): Kernel_Inviscid_Flux (:191-296,
+// b[L] -= F, b[R] += F) gathered per node, then Bkernel_Inviscid_Flux
+// (:298-387, b[L] -= F on the unextrapolated state vs the phantom).
+// Also writes per-block partial sums of b^2 (total + per equation).
+template <bool HO>
+__global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double gamma, double chi, int sym2d,
+                                                    const int* __restrict__ row_ptr, const int* __restrict__ col,
+                                                    const double* __restrict__ nrm, int nnz,
+                                                    const double* __restrict__ xyz, const double* __restrict__ q,
+                                                    const double* __restrict__ grad, const double* __restrict__ lim,
+                                                    const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                    const double* __restrict__ he_a, int nbnode,
+                                                    double* __restrict__ b, double* __restrict__ red) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double bi[5] = {0, 0, 0, 0, 0};
+  if (i < nnode) {
+    double qi[5], gi[15], li[5], xi[3];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) qi[k] = q[k * ntot + i];
+    if (HO) {
+#pragma unroll
+      for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) li[k] = lim[k * nnode + i];
+      xi[0] = xyz[i]; xi[1] = xyz[nnode + i]; xi[2] = xyz[2 * nnode + i];
+    }
+    for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+      int j = col[p];
+      const bool left = i < j;
+      double a[4] = {nrm[p], nrm[nnz + p], nrm[2 * nnz + p], nrm[3 * nnz + p]};
+      double qj[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) qj[k] = q[k * ntot + j];
+      double QL[5], QR[5];
+      if (HO) {
+        double gj[15], lj[5];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) gj[k] = grad[k * nnode + j];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) lj[k] = lim[k * nnode + j];
+        double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
+        const double* xL = left ? xi : xj;
+        const double* xR = left ? xj : xi;
+        const double* qL = left ? qi : qj;
+        const double* qR = left ? qj : qi;
+        const double* gL = left ? gi : gj;
+        const double* gR = left ? gj : gi;
+        const double* lL = left ? li : lj;
+        const double* lR = left ? lj : li;
+        double dx[3] = {0.5 * (xR[0] - xL[0]), 0.5 * (xR[1] - xL[1]), 0.5 * (xR[2] - xL[2])};
+        double mdx[3] = {-dx[0], -dx[1], -dx[2]};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          double dQ = qR[k] - qL[k];
+          QL[k] = extrap1(qL[k], dQ, gL[3 * k], gL[3 * k + 1], gL[3 * k + 2], dx, chi, lL[k]);
+          QR[k] = extrap1(qR[k], -dQ, gR[3 * k], gR[3 * k + 1], gR[3 * k + 2], mdx, chi, lR[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          QL[k] = left ? qi[k] : qj[k];
+          QR[k] = left ? qj[k] : qi[k];
+        }
+      }
+      double F[5];
+      roe_flux(QL, QR, a, gamma, sym2d, F);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) bi[k] += left ? -F[k] : F[k];
+    }
+    for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+      int h = he_id[p];
+      double a[4] = {he_a[h], he_a[nbnode + h], he_a[2 * nbnode + h], he_a[3 * nbnode + h]};
+      double QR[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) QR[k] = q[k * ntot + nnode + h];
+      double F[5];
+      roe_flux(qi, QR, a, gamma, sym2d, F);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) bi[k] += -F[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) b[k * nnode + i] = bi[k];
+  }
+  // block partial sums of b^2, fixed-shape tree -> deterministic
+  __shared__ double sh[6][kBlock];
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double v = bi[k] * bi[k];
+    sh[1 + k][threadIdx.x] = v;
+    tot += v;
+  }
+  sh[0][threadIdx.x] = tot;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k * gridDim.x + blockIdx.x] = sh[k][0];
+  }
+}
+
+// final reduction of the per-block partial sums (single block, fixed order)
+__global__ void __launch_bounds__(kBlock) k_reduce6(int nparts, const double* __restrict__ red, double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  for (int k = 0; k < 6; ++k) {
+    double s = 0.0;
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += red[k * nparts + p];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
+    __syncthreads();
+  }
+}
+
+void launch_residual(Ctx& c, int ho) {
+  const pcfd_params& p = c.prm;
+  int nb = nblk(c.nnode);
+  if (ho) {
+    hipLaunchKernelGGL(k_residual<true>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
+                       p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.xyz, c.d.q, c.d.grad, c.d.lim,
+                       c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.b, c.d.red);
+  } else {
+    hipLaunchKernelGGL(k_residual<false>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
+                       p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.xyz, c.d.q, c.d.grad, c.d.lim,
+                       c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.b, c.d.red);
+  }
+  PCFD_HIP(hipGetLastError());
+  c.nred = nb;
+}
+
+// residGlobal = sqrt(sum b^2)/(nnode*neqn), per-equation sqrt(sum b_k^2)/nnode
+// (ParallelL2Norm / StridedParallelL2Norm, ucs/parallel.h:120-182)
+void launch_norms(Ctx& c, double* host6) {
+  hipLaunchKernelGGL(k_reduce6, dim3(1), dim3(kBlock), 0, c.stream, c.nred, c.d.red, c.d.red_out);
+  PCFD_HIP(hipGetLastError());
+  if (host6) {
+    double s[6];
+    PCFD_HIP(hipMemcpyAsync(s, c.d.red_out, sizeof(s), hipMemcpyDeviceToHost, c.stream));
+    PCFD_HIP(hipStreamSynchronize(c.stream));
+    host6[0] = std::sqrt(s[0]) / (double)((long long)c.nnode * 5);
+    for (int k = 0; k < 5; ++k) host6[1 + k] = std::sqrt(s[1 + k]) / (double)c.nnode;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ComputeTimesteps, local time stepping (ucs/timestep.tcc:6-143)
+__global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double gamma, double cfl, const int* __restrict__ row_ptr,
+                           const int* __restrict__ col, const double* __restrict__ nrm, int nnz,
+                           const double* __restrict__ q, const double* __restrict__ vol,
+                           const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                           const double* __restrict__ he_a, int nbnode, double* __restrict__ dt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  double qi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) qi[k] = q[k * ntot + i];
+  double s = 0.0;
+  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+    int j = col[p];
+    double a[4] = {nrm[p], nrm[nnz + p], nrm[2 * nnz + p], nrm[3 * nnz + p]};
+    double t[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = i < j ? 0.5 * (qi[k] + q[k * ntot + j]) : 0.5 * (q[k * ntot + j] + qi[k]);
+    s += max_eig(t, a, gamma) * a[3];
+  }
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    int h = he_id[p];
+    double a[4] = {he_a[h], he_a[nbnode + h], he_a[2 * nbnode + h], he_a[3 * nbnode + h]};
+    double t[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) t[k] = 0.5 * (qi[k] + q[k * ntot + nnode + h]);
+    s += max_eig(t, a, gamma) * a[3];
+  }
+  dt[i] = cfl * (vol[i] / s);
+}
+
+__global__ void __launch_bounds__(kBlock) k_min_partial(int n, const double* __restrict__ v, double* __restrict__ red) {
+  __shared__ double sh[kBlock];
+  double m = 1.0e300;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = dmin(m, v[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = dmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red[blockIdx.x] = sh[0];
+}
+
+void launch_timestep(Ctx& c, double cfl) {
+  hipLaunchKernelGGL(k_timestep, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma,
+                     cfl, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.q, c.d.vol, c.d.he_ptr, c.d.he_id, c.d.he_a,
+                     c.nbnode, c.d.dt);
+  PCFD_HIP(hipGetLastError());
+}
+
+double launch_dtmin(Ctx& c) {
+  int nb = std::min(nblk(c.nnode), 1024);
+  hipLaunchKernelGGL(k_min_partial, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.d.dt, c.d.red);
+  PCFD_HIP(hipGetLastError());
+  std::vector<double> h(nb);
+  PCFD_HIP(hipMemcpyAsync(h.data(), c.d.red, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
+  PCFD_HIP(hipStreamSynchronize(c.stream));
+  double m = h[0];
+  for (double v : h) m = v < m ? v : m;
+  return m;
+}
+
+}  // namespace pcfd

@@ -1,0 +1,371 @@
+// Device physics for the compressible (perfect-gas) equation set.
+// Each function follows the reference formula named in its comment; the
+// operation order is kept so results agree with the reference to rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pcfd {
+
+// host+device so the same formulas can be exercised in CPU builds
+#define PCFD_DEV __host__ __device__ __forceinline__
+
+// ComputeAuxiliaryVariables pressure (ucs/compressible.tcc:1229-1236):
+// P = (g-1)(E - 0.5*rho*|u|^2) -- the value GetPressure() later returns.
+PCFD_DEV double aux_pressure(const double* Q, double gm1) {
+  double u = Q[1] / Q[0], v = Q[2] / Q[0], w = Q[3] / Q[0];
+  double V2 = u * u + v * v + w * w;
+  return gm1 * (Q[4] - 0.5 * Q[0] * V2);
+}
+
+// ComputePressure (ucs/compressible.tcc:1090-1101): P = (g-1)(E - rho*v2h)
+PCFD_DEV double comp_pressure(const double* Q, double gm1) {
+  double r = Q[0];
+  double u = Q[1] / r, v = Q[2] / r, w = Q[3] / r;
+  double v2h = 0.5 * (u * u + v * v + w * w);
+  return gm1 * (Q[4] - r * v2h);
+}
+
+PCFD_DEV double theta_of(const double* Q, const double* a) {  // GetTheta, static mesh
+  double u = Q[1] / Q[0], v = Q[2] / Q[0], w = Q[3] / Q[0];
+  return u * a[0] + v * a[1] + w * a[2] + 0.0;
+}
+
+// Flux: ucs/compressible.tcc:686-710 (vdotn = 0)
+PCFD_DEV void phys_flux(const double* Q, const double* a, double gm1, double* f) {
+  double rho = Q[0];
+  double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+  double v2h = 0.5 * (u * u + v * v + w * w);
+  double P = gm1 * (Q[4] - rho * v2h);
+  double ht = (Q[4] + P) / rho;
+  double rt = rho * (a[0] * u + a[1] * v + a[2] * w + 0.0);
+  f[0] = rt;
+  f[1] = (u * rt + P * a[0]);
+  f[2] = (v * rt + P * a[1]);
+  f[3] = (w * rt + P * a[2]);
+  f[4] = (ht * rt - 0.0 * P);
+}
+
+PCFD_DEV double dmax(double x, double y) { return x > y ? x : y; }
+PCFD_DEV double dmin(double x, double y) { return x < y ? x : y; }
+
+// Roe flux with the Harten-Hyman type-2 entropy fix, RoeVariables and the
+// explicit right/left eigenvector matrices applied as two 5x5 mat-vecs
+// (ucs/compressible.tcc:92-230, 533-684), NaN components zeroed
+// (ucs/eqnset.tcc:74-88).  F is the flux through a[3]*a[0..2] from L to R.
+PCFD_DEV void roe_flux(const double* QL, const double* QR, const double* a, double gamma, int sym2d,
+                       double* F) {
+  const double gm1 = gamma - 1.0;
+  // RoeVariables
+  double rhoL = QL[0], rhoR = QR[0];
+  double uL = QL[1] / QL[0], uR = QR[1] / QR[0];
+  double vL = QL[2] / QL[0], vR = QR[2] / QR[0];
+  double wL = QL[3] / QL[0], wR = QR[3] / QR[0];
+  double v2L = uL * uL + vL * vL + wL * wL;
+  double v2R = uR * uR + vR * vR + wR * wR;
+  double PL = gm1 * (QL[4] - 0.5 * rhoL * v2L);
+  double PR = gm1 * (QR[4] - 0.5 * rhoR * v2R);
+  double hL = (QL[4] + PL) / rhoL;
+  double hR = (QR[4] + PR) / rhoR;
+  double rr = sqrt(rhoL * rhoR);
+  double sigma = rr / (rhoL + rr);
+  double ur = uL + sigma * (uR - uL);
+  double vr = vL + sigma * (vR - vL);
+  double wr = wL + sigma * (wR - wL);
+  double hr = hL + sigma * (hR - hL);
+  double v2hr = 0.5 * (ur * ur + vr * vr + wr * wr);
+  double Qr[5] = {rr, rr * ur, rr * vr, rr * wr, rr / gamma * (hr + gm1 * v2hr)};
+  // Eigensystem on the Roe state
+  const double nx = a[0], ny = a[1], nz = a[2];
+  double rho = Qr[0];
+  double u = Qr[1] / rho, v = Qr[2] / rho, w = Qr[3] / rho;
+  double thetaf = u * nx + v * ny + w * nz;
+  double theta = thetaf + 0.0;
+  double v2h = 0.5 * (u * u + v * v + w * w);
+  double P = gm1 * (Qr[4] - rho * v2h);
+  double c2 = gamma * P / rho;
+  double c = sqrt(c2);
+  double e0 = theta, e3 = theta + c, e4 = theta - c;
+  // entropy fix (uses the L/R states' own pressures, as the reference does)
+  double thetaL = uL * nx + vL * ny + wL * nz + 0.0;
+  double thetaR = uR * nx + vR * ny + wR * nz + 0.0;
+  double cR = sqrt(gamma * PR / rhoR);
+  double cL = sqrt(gamma * PL / rhoL);
+  double eps;
+  eps = dmax(0.0, dmax(e0 - thetaL, thetaR - e0));
+  e0 = (fabs(e0) < eps) ? 0.5 * (e0 * e0 / eps + eps) : fabs(e0);
+  eps = dmax(0.0, dmax(e3 - (thetaL + cL), (thetaR + cR) - e3));
+  e3 = (fabs(e3) < eps) ? 0.5 * (e3 * e3 / eps + eps) : fabs(e3);
+  eps = dmax(0.0, dmax(e4 - (thetaL - cL), (thetaR - cR) - e4));
+  e4 = (fabs(e4) < eps) ? 0.5 * (e4 * e4 / eps + eps) : fabs(e4);
+  double dQ[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) dQ[i] = QR[i] - QL[i];
+  // dv = Tinv * dQ, row by row (MatVecMult order, ucs/matrix.h:63-74)
+  double Ti[5];
+  double dv[5];
+  Ti[0] = nx - nz * v / rho + ny * w / rho - nx / c2 * v2h * gm1;
+  Ti[1] = nx / c2 * u * gm1;
+  Ti[2] = nz / rho + nx / c2 * v * gm1;
+  Ti[3] = -ny / rho + nx / c2 * w * gm1;
+  Ti[4] = -nx / c2 * gm1;
+  dv[0] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
+  Ti[0] = ny + nz * u / rho - nx * w / rho - ny / c2 * v2h * gm1;
+  Ti[1] = -nz / rho + ny / c2 * u * gm1;
+  Ti[2] = ny / c2 * v * gm1;
+  Ti[3] = nx / rho + ny / c2 * w * gm1;
+  Ti[4] = -ny / c2 * gm1;
+  dv[1] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
+  Ti[0] = nz - ny * u / rho + nx * v / rho - nz / c2 * v2h * gm1;
+  Ti[1] = ny / rho + nz / c2 * u * gm1;
+  Ti[2] = -nx / rho + nz / c2 * v * gm1;
+  Ti[3] = nz / c2 * w * gm1;
+  Ti[4] = -nz / c2 * gm1;
+  dv[2] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
+  Ti[0] = -0.5 / rho * (thetaf - gm1 * v2h / c);
+  Ti[1] = 0.5 / rho * (nx - gm1 * u / c);
+  Ti[2] = 0.5 / rho * (ny - gm1 * v / c);
+  Ti[3] = 0.5 / rho * (nz - gm1 * w / c);
+  Ti[4] = 0.5 / rho * (gm1 / c);
+  dv[3] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
+  Ti[0] = 0.5 / rho * (thetaf + gm1 * v2h / c);
+  Ti[1] = -0.5 / rho * (nx + gm1 * u / c);
+  Ti[2] = -0.5 / rho * (ny + gm1 * v / c);
+  Ti[3] = -0.5 / rho * (nz + gm1 * w / c);
+  Ti[4] = +0.5 / rho * (gm1 / c);
+  dv[4] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
+  dv[0] *= fabs(e0);
+  dv[1] *= fabs(e0);
+  dv[2] *= fabs(e0);
+  dv[3] *= fabs(e3);
+  dv[4] *= fabs(e4);
+  // dr = T * dv, row by row
+  double dr[5];
+  dr[0] = nx * dv[0] + ny * dv[1] + nz * dv[2] + rho / c * dv[3] + rho / c * dv[4];
+  dr[1] = u * nx * dv[0] + (u * ny - rho * nz) * dv[1] + (u * nz + rho * ny) * dv[2] +
+          rho * (u / c + nx) * dv[3] + rho * (u / c - nx) * dv[4];
+  dr[2] = (v * nx + rho * nz) * dv[0] + v * ny * dv[1] + (v * nz - rho * nx) * dv[2] +
+          rho * (v / c + ny) * dv[3] + rho * (v / c - ny) * dv[4];
+  dr[3] = (w * nx - rho * ny) * dv[0] + (w * ny + rho * nx) * dv[1] + w * nz * dv[2] +
+          rho * (w / c + nz) * dv[3] + rho * (w / c - nz) * dv[4];
+  dr[4] = (v2h * nx + rho * (v * nz - w * ny)) * dv[0] + (v2h * ny + rho * (w * nx - u * nz)) * dv[1] +
+          (v2h * nz + rho * (u * ny - v * nx)) * dv[2] + rho * (v2h / c + thetaf + c / gm1) * dv[3] +
+          rho * (v2h / c - thetaf + c / gm1) * dv[4];
+  double fL[5], fR[5];
+  phys_flux(QL, a, gm1, fL);
+  phys_flux(QR, a, gm1, fR);
+  const double area = a[3];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) F[i] = 0.5 * area * (fL[i] + fR[i] - dr[i]);
+  if (sym2d == 1) F[1] = 0.0;
+  else if (sym2d == 2) F[2] = 0.0;
+  else if (sym2d == 3) F[3] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    if (isnan(F[i])) F[i] = 0.0;
+}
+
+// MaxEigenvalue on the edge-averaged state (ucs/compressible.tcc:798-821)
+PCFD_DEV double max_eig(const double* Q, const double* a, double gamma) {
+  const double gm1 = gamma - 1.0;
+  double rho = Q[0];
+  double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+  double v2h = 0.5 * (u * u + v * v + w * w);
+  double P = gm1 * (Q[4] - rho * v2h);
+  double c = sqrt(gamma * P / rho);
+  double th = theta_of(Q, a);
+  double e4 = th + c, e5 = th - c;
+  return dmax(fabs(e4), fabs(e5));
+}
+
+// BC type ids (ucs/bc_defines.h:4-30)
+enum : int {
+  BC_PARALLEL = 0, BC_DIRICHLET = 1, BC_NEUMANN = 2, BC_IMPERMEABLE = 3, BC_NOSLIP = 4,
+  BC_FARFIELD_VISCOUS = 5, BC_FARFIELD = 6, BC_SONIC_INFLOW = 7, BC_SONIC_OUTFLOW = 8, BC_SYMMETRY = 9
+};
+
+// GetFarfieldBoundaryVariables (ucs/compressible.tcc:1245-1372): 10 fixed-point
+// sub-iterations on the phantom QR using the updated average state.
+// pL / pinf are the auxiliary pressures of QL / Qinf.
+PCFD_DEV void farfield_state(const double* QL, double pL, double* QR, const double* Qinf, double pinf,
+                             const double* a, double gamma) {
+  const double gm1 = gamma - 1.0;
+  const double nx = a[0], ny = a[1], nz = a[2];
+  for (int subit = 0; subit < 10; ++subit) {
+    double t[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) t[i] = (QL[i] + QR[i]) / 2.0;
+    double theta = theta_of(t, a);
+    double rhoi = QL[0];
+    double ui = QL[1] / QL[0] + 0.0, vi = QL[2] / QL[0] + 0.0, wi = QL[3] / QL[0] + 0.0;
+    double rhoinf = Qinf[0];
+    double uinf = Qinf[1] / Qinf[0] + 0.0, vinf = Qinf[2] / Qinf[0] + 0.0, winf = Qinf[3] / Qinf[0] + 0.0;
+    double pavg = comp_pressure(t, gm1);
+    double rhoavg = t[0];
+    double c2avg = gamma * (pavg / rhoavg);
+    double cavg = sqrt(c2avg);
+    if (theta == 0.0) return;
+    double mn = fabs(theta / cavg);
+    if (theta > 0.0 && mn >= 1.0) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) QR[i] = QL[i];
+    } else if (theta < 0.0 && mn >= 1.0) {
+#pragma unroll
+      for (int i = 0; i < 5; ++i) QR[i] = Qinf[i];
+    } else if (theta > 0.0 && mn < 1.0) {
+      double pb = pinf;
+      double rhob = rhoi + (pb - pL) / c2avg;
+      double tt = (pb - pL) / (rhoavg * cavg);
+      double ub = ui - nx * tt, vb = vi - ny * tt, wb = wi - nz * tt;
+      QR[0] = rhob; QR[1] = rhob * ub; QR[2] = rhob * vb; QR[3] = rhob * wb;
+      QR[4] = pb / (gamma - 1.0) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
+    } else if (theta < 0.0 && mn < 1.0) {
+      double pb = 0.5 * (pinf + pL + rhoavg * cavg * (nx * (uinf - ui) + ny * (vinf - vi) + nz * (winf - wi)));
+      double rhob = rhoinf + (pb - pinf) / c2avg;
+      double tt = (pb - pinf) / (rhoavg * cavg);
+      double ub = uinf + nx * tt, vb = vinf + ny * tt, wb = winf + nz * tt;
+      QR[0] = rhob; QR[1] = rhob * ub; QR[2] = rhob * vb; QR[3] = rhob * wb;
+      QR[4] = pb / (gamma - 1.0) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
+    } else {
+      return;
+    }
+  }
+}
+
+// GetInviscidWallBoundaryVariables, CVBC branch (ucs/compressible.tcc:1374-1472)
+PCFD_DEV void wall_state(const double* QL, double pL, double* QR, const double* a, double gamma) {
+  const double gm1 = gamma - 1.0;
+  const double nx = a[0], ny = a[1], nz = a[2];
+  const double thL = theta_of(QL, a);
+  for (int subit = 0; subit < 10; ++subit) {
+    double t[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) t[i] = (QL[i] + QR[i]) / 2.0;
+    double rhoi = QL[0];
+    double ui = QL[1] / QL[0] + 0.0, vi = QL[2] / QL[0] + 0.0, wi = QL[3] / QL[0] + 0.0;
+    double rhoavg = t[0];
+    t[1] += rhoavg * 0.0;
+    t[2] += rhoavg * 0.0;
+    t[3] += rhoavg * 0.0;
+    double pavg = comp_pressure(t, gm1);
+    double c2avg = gamma * (pavg / rhoavg);
+    double cavg = sqrt(c2avg);
+    double pb = pL + rhoavg * cavg * thL;
+    double rhob = rhoi + (pb - pL) / c2avg;
+    double tt = (pb - pL) / (rhoavg * cavg);
+    double ub = ui - nx * tt, vb = vi - ny * tt, wb = wi - nz * tt;
+    QR[0] = rhob; QR[1] = rhob * ub; QR[2] = rhob * vb; QR[3] = rhob * wb;
+    QR[4] = pb / (gamma - 1.0) + 0.5 * rhob * (ub * ub + vb * vb + wb * wb);
+  }
+}
+
+// Dirichlet-type BCs set both the phantom and the interior node to Qref
+// (ucs/bc.tcc:1076-1083).  Callers handle them explicitly: merging that
+// path with the QL-dependent ones below was mis-compiled for gfx950 (the
+// phantom store picked up the pre-update interior state).
+PCFD_DEV bool is_dirichlet(int type) { return type == BC_SONIC_INFLOW || type == BC_DIRICHLET; }
+
+// CalculateBoundaryVariables for the non-Dirichlet BC types on this path
+// (ucs/bc.tcc:1058-1398).  QL is read only.  Returns false for types it does
+// not handle (the phantom is then left untouched).
+PCFD_DEV bool bc_state(int type, const double* QL, double* QR, const double* Qref, double pref, const double* a,
+                       double gamma) {
+  const double gm1 = gamma - 1.0;
+  if (type == BC_SONIC_OUTFLOW || type == BC_NEUMANN) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) QR[i] = QL[i];
+    return true;
+  } else if (type == BC_FARFIELD) {
+    farfield_state(QL, aux_pressure(QL, gm1), QR, Qref, pref, a, gamma);
+    return true;
+  } else if (type == BC_IMPERMEABLE || type == BC_SYMMETRY) {
+    wall_state(QL, aux_pressure(QL, gm1), QR, a, gamma);
+    return true;
+  }
+  return false;
+}
+
+// ExtrapolateVariables with ExtrapolateCorrection (ucs/compressible.tcc:1038-1052,
+// ucs/eqnset.h:231-238): Qho = q + (chi/2 dQ + (1-chi) grad.dx) * phi
+PCFD_DEV double extrap1(double q, double dQ, double gx, double gy, double gz, const double* dx, double chi,
+                        double phi) {
+  double corr = 0.5 * chi * dQ + (1.0 - chi) * (gx * dx[0] + gy * dx[1] + gz * dx[2]);
+  return q + corr * phi;
+}
+
+// BadExtrapolation (ucs/compressible.tcc:1054-1078)
+PCFD_DEV bool bad_state(const double* Q, double gamma) {
+  double r = Q[0];
+  double u = Q[1] / r, v = Q[2] / r, w = Q[3] / r, E = Q[4];
+  double v2h = 0.5 * (u * u + v * v + w * w);
+  double p = (gamma - 1.0) * (E - r * v2h);
+  if (p < 1.0e-10) return true;
+  if (r < 0.0) return true;
+  if (E < 1.0e-10) return true;
+  return false;
+}
+
+// RoeVariables state only (ucs/compressible.tcc:533-578)
+PCFD_DEV void roe_state(const double* QL, const double* QR, double gamma, double* Qr) {
+  const double gm1 = gamma - 1.0;
+  double rhoL = QL[0], rhoR = QR[0];
+  double uL = QL[1] / QL[0], uR = QR[1] / QR[0];
+  double vL = QL[2] / QL[0], vR = QR[2] / QR[0];
+  double wL = QL[3] / QL[0], wR = QR[3] / QR[0];
+  double v2L = uL * uL + vL * vL + wL * wL;
+  double v2R = uR * uR + vR * vR + wR * wR;
+  double PL = gm1 * (QL[4] - 0.5 * rhoL * v2L);
+  double PR = gm1 * (QR[4] - 0.5 * rhoR * v2R);
+  double hL = (QL[4] + PL) / rhoL;
+  double hR = (QR[4] + PR) / rhoR;
+  double rr = sqrt(rhoL * rhoR);
+  double sigma = rr / (rhoL + rr);
+  double ur = uL + sigma * (uR - uL);
+  double vr = vL + sigma * (vR - vL);
+  double wr = wL + sigma * (wR - wL);
+  double hr = hL + sigma * (hR - hL);
+  double v2hr = 0.5 * (ur * ur + vr * vr + wr * wr);
+  Qr[0] = rr;
+  Qr[1] = rr * ur;
+  Qr[2] = rr * vr;
+  Qr[3] = rr * wr;
+  Qr[4] = rr / gamma * (hr + gm1 * v2hr);
+}
+
+// ComputeLSQCoefficients (ucs/gradient.tcc:140-168)
+PCFD_DEV void lsq_coeff(const double* s, const double* d, double* we) {
+  double r11 = s[0], r12 = s[1], r13 = s[2];
+  double r12_r11 = (r11 == 0.0) ? 0.0 : r12 / r11;
+  double r22 = s[3] - r12 * r12_r11;
+  double r23 = s[4] - r12_r11 * r13;
+  double r13_r11 = (r11 == 0.0) ? 0.0 : r13 / r11;
+  double r23_r22 = (r22 == 0.0) ? 0.0 : r23 / r22;
+  double r33 = s[5] - r13 * r13_r11 - r23 * r23_r22;
+  double dykdx = (d[1] - (r12_r11)*d[0]);
+  we[2] = (r33 == 0.0) ? 0.0 : (d[2] - r13_r11 * d[0] - r23_r22 * dykdx) / r33;
+  we[1] = (r22 == 0.0) ? 0.0 : (dykdx - r23 * we[2]) / r22;
+  we[0] = (r11 == 0.0) ? 0.0 : (d[0] - r12 * we[1] - r13 * we[2]) / r11;
+}
+
+// CompressibleEqnSet::ApplyDQ clipping (ucs/compressible.tcc:928-993)
+PCFD_DEV void apply_dq(const double* dQ, double* Q, double gamma) {
+  const double gm1 = gamma - 1.0;
+  const double minP = 1.0e-10, minRho = 1.0e-10, minE = 1.0e-10;
+  if (Q[0] + dQ[0] < 0.0) Q[0] = minRho;
+  else Q[0] += dQ[0];
+  if (Q[4] + dQ[4] < 0.0) Q[4] = minE;
+  else Q[4] += dQ[4];
+  double rho = Q[0];
+  double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho, E = Q[4];
+  double v2 = u * u + v * v + w * w;
+  if (E < 0.5 * rho * v2) {
+    double v2mod = 2.0 * (E - minP / gm1);
+    double frac = 0.0;
+    if (v2mod > 0.0) frac = sqrt(v2mod / v2);
+    u *= frac; v *= frac; w *= frac;
+    Q[1] = rho * u; Q[2] = rho * v; Q[3] = rho * w;
+  } else {
+    Q[1] += dQ[1]; Q[2] += dQ[2]; Q[3] += dQ[3];
+  }
+}
+
+}  // namespace pcfd

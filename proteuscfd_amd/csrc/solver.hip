@@ -1,0 +1,181 @@
+// Host driver of the steady implicit (or explicit) iteration, in the
+// reference's phase order:
+//   PreTimeAdvance   (ucs/solutionSpace.tcc:572-636): Jacobian with the
+//                    previous dt, UpdateCFL, ComputeTimesteps
+//   NewtonIterate    (ucs/solutionSpace.tcc:639-904): UpdateBCs, gradient,
+//                    limiter, residual + norms, PrepareSGS + SGS (or the
+//                    explicit update), NaN rows, ApplyDQ
+//   PostTimeAdvance  (ucs/solutionSpace.tcc:907-916): iter++
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+
+#include "common.h"
+
+namespace pcfd {
+void phase_begin(Ctx& c);
+void phase_end(Ctx& c, int k);
+void ensure_matrix_alloc(Ctx& c);
+}  // namespace pcfd
+
+using namespace pcfd;
+
+struct pcfd_ctx {
+  Ctx c;
+};
+
+namespace {
+
+// Param::UpdateCFL, prescribed ramp (ucs/param.tcc:718-725)
+double cfl_at(const pcfd_params& p, int iter) {
+  if (p.cfl_ramp > 0 && iter < p.cfl_ramp) return (p.cfl - p.cfl_start) / (double)p.cfl_ramp * (double)iter + p.cfl_start;
+  return p.cfl;
+}
+
+void iterate(Ctx& c, double* resid) {
+  const pcfd_params& p = c.prm;
+  // PreTimeAdvance
+  if (p.nsgs > 0 && (c.iter % p.jac_freq == 0 || c.iter == 1)) {
+    ensure_matrix_alloc(c);
+    phase_begin(c);
+    launch_jacobian(c);
+    phase_end(c, 5);
+    c.jac_valid = true;
+    c.lu_done = false;
+  }
+  c.cfl = cfl_at(p, c.iter);
+  phase_begin(c);
+  launch_timestep(c, c.cfl);
+  phase_end(c, 4);
+  // NewtonIterate
+  phase_begin(c);
+  launch_update_bcs(c);
+  phase_end(c, 0);
+  const bool ho = p.sorder > 1 && c.iter > p.nfirst;
+  if (ho) {
+    phase_begin(c);
+    launch_gradient(c, p.gradient_terms);
+    phase_end(c, 1);
+  }
+  if ((c.iter % p.limiter_refresh == 0) && p.sorder > 1 && p.limiter && c.iter > p.nfirst &&
+      c.iter < p.limiter_freeze) {
+    phase_begin(c);
+    launch_limiter(c);
+    phase_end(c, 2);
+  }
+  phase_begin(c);
+  launch_residual(c, ho ? 1 : 0);
+  phase_end(c, 3);
+  launch_norms(c, resid);
+  if (p.nsgs > 0) {
+    if (!c.lu_done) {
+      phase_begin(c);
+      launch_prepare_sgs(c);
+      phase_end(c, 6);
+      c.lu_done = true;
+    }
+    phase_begin(c);
+    launch_sgs(c, p.nsgs, nullptr);
+    phase_end(c, 7);
+    phase_begin(c);
+    launch_apply_dq(c, 0, nullptr);
+    phase_end(c, 8);
+  } else {
+    phase_begin(c);
+    launch_apply_dq(c, 1, nullptr);
+    phase_end(c, 8);
+  }
+  c.iter++;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcfd_reset_iteration(pcfd_ctx* ctx) {
+  if (!ctx) return PCFD_ERR_ARG;
+  Ctx& c = ctx->c;
+  if (!c.have_params) {
+    c.err = "pcfd_set_params has not been called";
+    return PCFD_ERR_STATE;
+  }
+  try {
+    c.iter = 1;
+    c.jac_valid = false;
+    c.lu_done = false;
+    // Limiter ctor fills 1.0 (ucs/limiters.tcc:39-43); PreIterate computes dt
+    std::vector<double> ones(5 * (size_t)c.nnode, 1.0);
+    PCFD_HIP(hipMemcpy(c.d.lim, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice));
+    c.cfl = cfl_at(c.prm, c.iter);
+    launch_timestep(c, c.cfl);
+    PCFD_HIP(hipStreamSynchronize(c.stream));
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return PCFD_ERR_HIP;
+  }
+  return PCFD_OK;
+}
+
+int pcfd_get_iteration(const pcfd_ctx* ctx) { return ctx ? ctx->c.iter : -1; }
+
+int pcfd_iterate(pcfd_ctx* ctx, double resid[6]) {
+  if (!ctx) return PCFD_ERR_ARG;
+  Ctx& c = ctx->c;
+  if (!c.have_params) {
+    c.err = "pcfd_set_params has not been called";
+    return PCFD_ERR_STATE;
+  }
+  try {
+    double r[6];
+    iterate(c, r);
+    if (resid)
+      for (int k = 0; k < 6; ++k) resid[k] = r[k];
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return PCFD_ERR_HIP;
+  }
+  return PCFD_OK;
+}
+
+int pcfd_solve(pcfd_ctx* ctx, int max_iter, double drop, double* history, int* niter, double* secs) {
+  if (!ctx) return PCFD_ERR_ARG;
+  Ctx& c = ctx->c;
+  if (!c.have_params) {
+    c.err = "pcfd_set_params has not been called";
+    return PCFD_ERR_STATE;
+  }
+  if (max_iter < 1 || !(drop > 0.0)) {
+    c.err = "pcfd_solve: max_iter >= 1 and drop > 0 required";
+    return PCFD_ERR_ARG;
+  }
+  try {
+    auto t0 = std::chrono::steady_clock::now();
+    double first = -1.0;
+    int n = 0;
+    for (; n < max_iter; ++n) {
+      double r[6];
+      iterate(c, r);
+      if (history) history[n] = r[0];
+      if (!std::isfinite(r[0])) {
+        c.err = "pcfd_solve: residual diverged";
+        if (niter) *niter = n + 1;
+        return PCFD_ERR_STATE;
+      }
+      if (n == 0) first = r[0];
+      if (r[0] <= drop * first) {
+        ++n;
+        break;
+      }
+    }
+    PCFD_HIP(hipStreamSynchronize(c.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    if (niter) *niter = n;
+    if (secs) *secs = std::chrono::duration<double>(t1 - t0).count();
+  } catch (const std::exception& e) {
+    c.err = e.what();
+    return PCFD_ERR_HIP;
+  }
+  return PCFD_OK;
+}
+
+}  // extern "C"

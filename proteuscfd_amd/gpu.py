@@ -1,0 +1,267 @@
+"""ctypes binding of libpcfd.so (include/pcfd.h) -- the HIP path.
+
+There is no CPU fallback: if the library (or a GPU) is missing every call
+raises.  Arrays cross the boundary in the reference layouts (AoS q with
+nvars per node, qgrad[node*27+term*3+dir], l/b/x[node*5+k]).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .cases import Case, Params
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+
+PLAIN_FP64, CHOSEN_FP32 = 0, 1
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nbnode", C.c_int), ("nedge", C.c_int), ("edges", ip), ("edge_a", dp),
+                ("bedges", ip), ("bedge_a", dp), ("bfactag", ip), ("xyz", dp), ("vol", dp), ("nfactag", C.c_int),
+                ("bc_type", ip), ("qref", dp)]
+
+
+class CParams(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("sorder", C.c_int), ("nfirst", C.c_int), ("chi", C.c_double),
+                ("limiter", C.c_int), ("venkat_K", C.c_double), ("limiter_refresh", C.c_int),
+                ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_ramp", C.c_int),
+                ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int),
+                ("gradient_terms", C.c_int)]
+
+
+SYMBOLS = [
+    "pcfd_last_error", "pcfd_version", "pcfd_create", "pcfd_destroy", "pcfd_set_params", "pcfd_info",
+    "pcfd_upload_q", "pcfd_download_q", "pcfd_update_bcs", "pcfd_gradient", "pcfd_limiter", "pcfd_set_limiter",
+    "pcfd_residual", "pcfd_timestep", "pcfd_set_timestep", "pcfd_jacobian", "pcfd_download_jacobian",
+    "pcfd_prepare_sgs", "pcfd_sgs", "pcfd_sgs_order", "pcfd_apply_dq", "pcfd_explicit_update", "pcfd_iterate",
+    "pcfd_reset_iteration", "pcfd_get_iteration", "pcfd_solve", "pcfd_phase_times", "pcfd_enable_phase_timing",
+    "pcfd_synchronize",
+]
+
+_lib = None
+
+
+def lib_path() -> str:
+    return os.path.join(HERE, "libpcfd.so")
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = lib_path()
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: the HIP path is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        vp = C.c_void_p
+        L.pcfd_last_error.argtypes = [vp]
+        L.pcfd_last_error.restype = C.c_char_p
+        L.pcfd_version.restype = C.c_char_p
+        L.pcfd_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(vp)]
+        L.pcfd_destroy.argtypes = [vp]
+        L.pcfd_set_params.argtypes = [vp, C.POINTER(CParams)]
+        L.pcfd_info.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.pcfd_upload_q.argtypes = [vp, dp, C.c_int]
+        L.pcfd_download_q.argtypes = [vp, dp, C.c_int]
+        L.pcfd_update_bcs.argtypes = [vp]
+        L.pcfd_gradient.argtypes = [vp, dp]
+        L.pcfd_limiter.argtypes = [vp, dp]
+        L.pcfd_set_limiter.argtypes = [vp, dp]
+        L.pcfd_residual.argtypes = [vp, C.c_int, dp, dp]
+        L.pcfd_timestep.argtypes = [vp, C.c_double, dp, dp]
+        L.pcfd_set_timestep.argtypes = [vp, dp]
+        L.pcfd_jacobian.argtypes = [vp]
+        L.pcfd_download_jacobian.argtypes = [vp, ip, ip, dp]
+        L.pcfd_prepare_sgs.argtypes = [vp]
+        L.pcfd_sgs.argtypes = [vp, C.c_int, dp, dp]
+        L.pcfd_sgs_order.argtypes = [vp, ip]
+        L.pcfd_apply_dq.argtypes = [vp, ip]
+        L.pcfd_explicit_update.argtypes = [vp, ip]
+        L.pcfd_iterate.argtypes = [vp, dp]
+        L.pcfd_reset_iteration.argtypes = [vp]
+        L.pcfd_get_iteration.argtypes = [vp]
+        L.pcfd_solve.argtypes = [vp, C.c_int, C.c_double, dp, ip, dp]
+        L.pcfd_phase_times.argtypes = [vp, dp, C.c_int]
+        L.pcfd_enable_phase_timing.argtypes = [vp, C.c_int]
+        L.pcfd_synchronize.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def to_cparams(p: Params, gradient_terms: int = 9) -> CParams:
+    return CParams(p.gamma, p.sorder, p.nfirst, p.chi, p.limiter, p.venkat_K, p.limiter_refresh, p.limiter_freeze,
+                   p.cfl, p.cfl_start, p.cfl_ramp, p.nsgs, p.symmetry2d, p.jac_freq, p.precision, gradient_terms)
+
+
+class Context:
+    """One pcfd context on the current HIP device."""
+
+    def __init__(self, case: Case, params: Params | None = None, gradient_terms: int = 9):
+        m = case.mesh
+        self.case = case
+        self.nnode, self.nbnode = m.nnode, m.nbedge
+        self._keep = dict(edges=_i32(m.edges), edge_a=_f64(m.edge_a), bedges=_i32(m.bedges),
+                          bedge_a=_f64(m.bedge_a), bfactag=_i32(m.bfactag), xyz=_f64(m.xyz), vol=_f64(m.vol),
+                          bc_type=_i32(case.bc_type), qref=_f64(case.qref[:, :5]))
+        k = self._keep
+        desc = MeshDesc(m.nnode, m.nbedge, m.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
+                        _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), len(k["bc_type"]),
+                        _i(k["bc_type"]), _d(k["qref"]))
+        h = C.c_void_p()
+        rc = lib().pcfd_create(C.byref(desc), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pcfd_create failed ({rc}): {lib().pcfd_last_error(None).decode()}")
+        self.h = h
+        self.set_params(params or case.params, gradient_terms)
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {lib().pcfd_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pcfd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p: Params, gradient_terms: int = 9):
+        self.params = p
+        cp = to_cparams(p, gradient_terms)
+        self._chk(lib().pcfd_set_params(self.h, C.byref(cp)), "pcfd_set_params")
+
+    def info(self):
+        nc = C.c_int()
+        nb = C.c_longlong()
+        by = C.c_longlong()
+        self._chk(lib().pcfd_info(self.h, C.byref(nc), C.byref(nb), C.byref(by)), "pcfd_info")
+        return dict(ncolor=nc.value, nblocks=nb.value, device_bytes=by.value)
+
+    def upload_q(self, q):
+        q = _f64(q)
+        self._chk(lib().pcfd_upload_q(self.h, _d(q), q.shape[1]), "pcfd_upload_q")
+
+    def download_q(self, nvars=10):
+        q = np.zeros((self.nnode + self.nbnode, nvars))
+        self._chk(lib().pcfd_download_q(self.h, _d(q), nvars), "pcfd_download_q")
+        return q
+
+    def update_bcs(self):
+        self._chk(lib().pcfd_update_bcs(self.h), "pcfd_update_bcs")
+
+    def gradient(self, download=True):
+        g = np.zeros((self.nnode, 27)) if download else None
+        self._chk(lib().pcfd_gradient(self.h, _d(g) if download else None), "pcfd_gradient")
+        return g
+
+    def limiter(self, download=True):
+        l = np.zeros((self.nnode, 5)) if download else None
+        self._chk(lib().pcfd_limiter(self.h, _d(l) if download else None), "pcfd_limiter")
+        return l
+
+    def set_limiter(self, l):
+        self._chk(lib().pcfd_set_limiter(self.h, _d(_f64(l))), "pcfd_set_limiter")
+
+    def residual(self, it=1, download=True):
+        b = np.zeros((self.nnode, 5)) if download else None
+        norms = np.zeros(6)
+        self._chk(lib().pcfd_residual(self.h, it, _d(b) if download else None, _d(norms)), "pcfd_residual")
+        return b, norms
+
+    def timestep(self, cfl, download=True):
+        dt = np.zeros(self.nnode) if download else None
+        dtmin = C.c_double()
+        self._chk(lib().pcfd_timestep(self.h, cfl, _d(dt) if download else None, C.byref(dtmin)), "pcfd_timestep")
+        return dt, dtmin.value
+
+    def set_timestep(self, dt):
+        self._chk(lib().pcfd_set_timestep(self.h, _d(_f64(dt))), "pcfd_set_timestep")
+
+    def jacobian(self):
+        self._chk(lib().pcfd_jacobian(self.h), "pcfd_jacobian")
+
+    def download_jacobian(self, ia, ja):
+        ia, ja = _i32(ia), _i32(ja)
+        M = np.zeros((len(ja), 5, 5))
+        self._chk(lib().pcfd_download_jacobian(self.h, _i(ia), _i(ja), _d(M)), "pcfd_download_jacobian")
+        return M
+
+    def prepare_sgs(self):
+        self._chk(lib().pcfd_prepare_sgs(self.h), "pcfd_prepare_sgs")
+
+    def sgs(self, nsgs, download=True):
+        x = np.zeros((self.nnode, 5)) if download else None
+        delta = C.c_double()
+        self._chk(lib().pcfd_sgs(self.h, nsgs, _d(x) if download else None, C.byref(delta)), "pcfd_sgs")
+        return x, delta.value
+
+    def sgs_order(self):
+        o = np.zeros(self.nnode, dtype=np.int32)
+        self._chk(lib().pcfd_sgs_order(self.h, _i(o)), "pcfd_sgs_order")
+        return o
+
+    def apply_dq(self):
+        n = C.c_int()
+        self._chk(lib().pcfd_apply_dq(self.h, C.byref(n)), "pcfd_apply_dq")
+        return n.value
+
+    def explicit_update(self):
+        n = C.c_int()
+        self._chk(lib().pcfd_explicit_update(self.h, C.byref(n)), "pcfd_explicit_update")
+        return n.value
+
+    def reset_iteration(self):
+        self._chk(lib().pcfd_reset_iteration(self.h), "pcfd_reset_iteration")
+
+    @property
+    def iteration(self):
+        return lib().pcfd_get_iteration(self.h)
+
+    def iterate(self):
+        r = np.zeros(6)
+        self._chk(lib().pcfd_iterate(self.h, _d(r)), "pcfd_iterate")
+        return r
+
+    def solve(self, max_iter, drop=1e-8):
+        hist = np.zeros(max_iter)
+        n = C.c_int()
+        secs = C.c_double()
+        rc = lib().pcfd_solve(self.h, max_iter, drop, _d(hist), C.byref(n), C.byref(secs))
+        self._chk(rc, "pcfd_solve")
+        return hist[: n.value], n.value, secs.value
+
+    def phase_times(self, reset=False):
+        ms = np.zeros(9)
+        self._chk(lib().pcfd_phase_times(self.h, _d(ms), int(reset)), "pcfd_phase_times")
+        return dict(zip(["bcs", "gradient", "limiter", "residual", "timestep", "jacobian", "prepare_sgs", "sgs",
+                         "update"], ms))
+
+    def enable_phase_timing(self, on=True):
+        self._chk(lib().pcfd_enable_phase_timing(self.h, int(on)), "pcfd_enable_phase_timing")
+
+    def synchronize(self):
+        self._chk(lib().pcfd_synchronize(self.h), "pcfd_synchronize")
