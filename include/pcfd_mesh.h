@@ -94,6 +94,17 @@ typedef struct pcfd_mesh pcfd_mesh;
 
 void pcfd_mesh_default_spec(int kind, pcfd_mesh_spec* spec);
 int  pcfd_mesh_build(const pcfd_mesh_spec* spec, pcfd_mesh** out);
+/* Median-dual build from an external element list (what ucs.x's
+ * BuildMaps + CalcMetrics do after a mesh reader, ucs/mesh.tcc:264-312,
+ * 1576-1596).  Volume elements PCFD_TET/PYRAMID/PRISM/HEX in the local
+ * numbering of the tables in mesh.cpp (tet any orientation, others with
+ * their bottom face counter-clockwise seen from inside or outside: a
+ * negatively oriented element is reflected); boundary faces PCFD_TRI/QUAD
+ * are re-wound outward against their owning element (FixBoundaryWindings,
+ * ucs/mesh.tcc:2336-2410).  Node ids are 0-based. */
+int  pcfd_mesh_build_from(int nnode, const double* xyz, int nelem, const int* elem_type, const int* elem_ptr,
+                          const int* elem_nodes, int nbface, const int* bface_type, const int* bface_ptr,
+                          const int* bface_nodes, const int* bface_tag, int rcm, pcfd_mesh** out);
 const pcfd_mesh_data* pcfd_mesh_get(const pcfd_mesh* m);
 void pcfd_mesh_free(pcfd_mesh* m);
 const char* pcfd_mesh_error(void);

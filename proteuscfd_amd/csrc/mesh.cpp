@@ -67,6 +67,14 @@ const ElemTopo kTet = {4, 4, 6, {3, 3, 3, 3, 0, 0},
 const ElemTopo kPrism = {6, 5, 9, {3, 3, 4, 4, 4, 0},
                          {{0, 2, 1, 0}, {3, 4, 5, 0}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}},
                          {{0, 1}, {1, 2}, {2, 0}, {3, 4}, {4, 5}, {5, 3}, {0, 3}, {1, 4}, {2, 5}}};
+// pyramid: base 0,1,2,3 counter-clockwise seen from the apex 4
+const ElemTopo kPyramid = {5, 5, 8, {4, 3, 3, 3, 3, 0},
+                           {{0, 3, 2, 1}, {0, 1, 4, 0}, {1, 2, 4, 0}, {2, 3, 4, 0}, {3, 0, 4, 0}},
+                           {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 4}, {1, 4}, {2, 4}, {3, 4}}};
+// hexahedron: bottom 0,1,2,3 counter-clockwise seen from the top, node k+4 above k
+const ElemTopo kHex = {8, 6, 12, {4, 4, 4, 4, 4, 4},
+                       {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}},
+                       {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}}};
 
 struct Work {
   int kind = 0;
@@ -264,7 +272,46 @@ int generate(const pcfd_mesh_spec& s, Work& w) {
 // ----------------------------------------------------------------------------
 // graph utilities
 // ----------------------------------------------------------------------------
-const ElemTopo& topo(int t) { return t == PCFD_PRISM ? kPrism : kTet; }
+const ElemTopo& topo(int t) {
+  switch (t) {
+    case PCFD_PRISM: return kPrism;
+    case PCFD_PYRAMID: return kPyramid;
+    case PCFD_HEX: return kHex;
+    default: return kTet;
+  }
+}
+
+// element volume through its outward faces (divergence theorem, apex at the centroid)
+double elem_volume(const Work& w, int type, const int* n) {
+  const ElemTopo& T = topo(type);
+  Vec3 x[8];
+  Vec3 g = {0, 0, 0};
+  for (int k = 0; k < T.nnode; ++k) {
+    x[k] = P(w, n[k]);
+    g = g + x[k];
+  }
+  g = (1.0 / T.nnode) * g;
+  double ev = 0.0;
+  for (int f = 0; f < T.nface; ++f) {
+    int nf = T.face_n[f];
+    Vec3 c = {0, 0, 0};
+    for (int k = 0; k < nf; ++k) c = c + x[T.face[f][k]];
+    c = (1.0 / nf) * c;
+    for (int k = 0; k < nf; ++k) ev += det3(x[T.face[f][k]] - g, x[T.face[f][(k + 1) % nf]] - g, c - g) / 6.0;
+  }
+  return ev;
+}
+
+// reflect an element's local numbering (reverses its orientation)
+void reflect(int type, int* n) {
+  switch (type) {
+    case PCFD_TET: std::swap(n[1], n[2]); break;
+    case PCFD_PYRAMID: std::swap(n[1], n[3]); break;
+    case PCFD_PRISM: std::swap(n[1], n[2]); std::swap(n[4], n[5]); break;
+    case PCFD_HEX: std::swap(n[1], n[3]); std::swap(n[5], n[7]); break;
+    default: break;
+  }
+}
 
 // sorted unique edge keys (n0<<32 | n1) from the element edges
 std::vector<uint64_t> edge_keys(const Work& w) {
@@ -380,6 +427,8 @@ void pcfd_mesh_default_spec(int kind, pcfd_mesh_spec* s) {
   }
 }
 
+static int finish(Work& w, int rcm, pcfd_mesh** out);
+
 int pcfd_mesh_build(const pcfd_mesh_spec* spec, pcfd_mesh** out) {
   if (!spec || !out) {
     g_err = "pcfd_mesh_build: null argument";
@@ -388,12 +437,93 @@ int pcfd_mesh_build(const pcfd_mesh_spec* spec, pcfd_mesh** out) {
   *out = nullptr;
   Work w;
   if (generate(*spec, w) != 0) return -1;
+  return finish(w, spec->rcm, out);
+}
+
+int pcfd_mesh_build_from(int nnode, const double* xyz, int nelem, const int* elem_type, const int* elem_ptr,
+                         const int* elem_nodes, int nbface, const int* bface_type, const int* bface_ptr,
+                         const int* bface_nodes, const int* bface_tag, int rcm, pcfd_mesh** out) {
+  if (!out || nnode <= 0 || !xyz || nelem <= 0 || !elem_type || !elem_ptr || !elem_nodes || nbface < 0 ||
+      (nbface > 0 && (!bface_type || !bface_ptr || !bface_nodes || !bface_tag))) {
+    g_err = "pcfd_mesh_build_from: bad arguments";
+    return -1;
+  }
+  *out = nullptr;
+  Work w;
+  w.xyz.assign(xyz, xyz + 3 * (size_t)nnode);
+  w.eptr.assign(1, 0);
+  w.bptr.assign(1, 0);
+  for (int e = 0; e < nelem; ++e) {
+    int t = elem_type[e];
+    int nn = elem_ptr[e + 1] - elem_ptr[e];
+    if ((t != PCFD_TET && t != PCFD_PYRAMID && t != PCFD_PRISM && t != PCFD_HEX) || nn != topo(t).nnode) {
+      g_err = "pcfd_mesh_build_from: element " + std::to_string(e) + " has an unsupported type/size";
+      return -1;
+    }
+    int n[8];
+    for (int k = 0; k < nn; ++k) {
+      n[k] = elem_nodes[elem_ptr[e] + k];
+      if (n[k] < 0 || n[k] >= nnode) {
+        g_err = "pcfd_mesh_build_from: element node out of range";
+        return -1;
+      }
+    }
+    if (elem_volume(w, t, n) < 0.0) reflect(t, n);
+    add_elem(w, t, n, nn);
+  }
+  // element faces keyed by their sorted node set -> owning element, to wind
+  // the boundary faces outward (FixBoundaryWindings, ucs/mesh.tcc:2336-2410)
+  std::vector<std::pair<std::vector<int>, int>> fmap;
+  for (int e = 0; e < nelem; ++e) {
+    const ElemTopo& T = topo(w.etype[e]);
+    const int* n = &w.enodes[w.eptr[e]];
+    for (int f = 0; f < T.nface; ++f) {
+      std::vector<int> key;
+      for (int k = 0; k < T.face_n[f]; ++k) key.push_back(n[T.face[f][k]]);
+      std::sort(key.begin(), key.end());
+      fmap.push_back({key, e});
+    }
+  }
+  std::sort(fmap.begin(), fmap.end());
+  for (int f = 0; f < nbface; ++f) {
+    int t = bface_type[f];
+    int nf = bface_ptr[f + 1] - bface_ptr[f];
+    if ((t != PCFD_TRI || nf != 3) && (t != PCFD_QUAD || nf != 4)) {
+      g_err = "pcfd_mesh_build_from: boundary face " + std::to_string(f) + " is not a tri/quad";
+      return -1;
+    }
+    int n[4];
+    for (int k = 0; k < nf; ++k) n[k] = bface_nodes[bface_ptr[f] + k];
+    std::vector<int> key(n, n + nf);
+    std::sort(key.begin(), key.end());
+    auto it = std::lower_bound(fmap.begin(), fmap.end(), std::make_pair(key, -1));
+    if (it == fmap.end() || it->first != key) {
+      g_err = "pcfd_mesh_build_from: boundary face " + std::to_string(f) + " matches no element face";
+      return -1;
+    }
+    int e = it->second;
+    const ElemTopo& T = topo(w.etype[e]);
+    Vec3 g = {0, 0, 0};
+    for (int k = 0; k < T.nnode; ++k) g = g + P(w, w.enodes[w.eptr[e] + k]);
+    g = (1.0 / T.nnode) * g;
+    Vec3 c = {0, 0, 0};
+    for (int k = 0; k < nf; ++k) c = c + P(w, n[k]);
+    c = (1.0 / nf) * c;
+    Vec3 a = {0, 0, 0};
+    for (int k = 0; k < nf; ++k) a = a + cross(P(w, n[k]) - c, P(w, n[(k + 1) % nf]) - c);
+    if (dot(a, c - g) < 0.0) std::reverse(n + 1, n + nf);
+    add_bface(w, t, n, nf, bface_tag[f]);
+  }
+  return finish(w, rcm, out);
+}
+
+static int finish(Work& w, int rcm, pcfd_mesh** out) {
   const int nn = (int)(w.xyz.size() / 3);
 
   // optional RCM renumbering of nodes
   std::vector<int> perm(nn);
   std::iota(perm.begin(), perm.end(), 0);
-  if (spec->rcm) {
+  if (rcm) {
     std::vector<uint64_t> keys = edge_keys(w);
     perm = rcm_order(nn, keys);
     std::vector<int> inv(nn);
@@ -421,7 +551,7 @@ int pcfd_mesh_build(const pcfd_mesh_spec* spec, pcfd_mesh** out) {
   for (size_t e = 0; e < ne; ++e) {
     const ElemTopo& T = topo(w.etype[e]);
     const int* n = &w.enodes[w.eptr[e]];
-    Vec3 x[6];
+    Vec3 x[8];
     Vec3 g = {0, 0, 0};
     for (int k = 0; k < T.nnode; ++k) {
       x[k] = P(w, n[k]);

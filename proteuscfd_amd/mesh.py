@@ -64,6 +64,10 @@ def lib():
         L.pcfd_mesh_error.restype = C.c_char_p
         L.pcfd_color_nodes.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.pcfd_color_nodes.restype = C.c_int
+        ipt, dpt = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.pcfd_mesh_build_from.argtypes = [C.c_int, dpt, C.c_int, ipt, ipt, ipt, C.c_int, ipt, ipt, ipt, ipt,
+                                           C.c_int, C.POINTER(C.c_void_p)]
+        L.pcfd_mesh_build_from.restype = C.c_int
         _lib = L
     return _lib
 
@@ -85,10 +89,13 @@ def _arr(ptr, n, dtype):
 class Mesh:
     """Numpy view of a generated mesh (all arrays copied, C object freed)."""
 
-    def __init__(self, spec: MeshSpec):
-        h = C.c_void_p()
-        if lib().pcfd_mesh_build(C.byref(spec), C.byref(h)) != 0:
-            raise RuntimeError(lib().pcfd_mesh_error().decode())
+    def __init__(self, spec: MeshSpec | None, _handle=None):
+        if _handle is None:
+            h = C.c_void_p()
+            if lib().pcfd_mesh_build(C.byref(spec), C.byref(h)) != 0:
+                raise RuntimeError(lib().pcfd_mesh_error().decode())
+        else:
+            h = _handle
         try:
             d = lib().pcfd_mesh_get(h).contents
             self.spec = spec
@@ -118,6 +125,26 @@ class Mesh:
     @classmethod
     def build(cls, kind: int, **kw) -> "Mesh":
         return cls(default_spec(kind, **kw))
+
+    @classmethod
+    def from_elements(cls, xyz, elems, bfaces, rcm=False) -> "Mesh":
+        """elems: list of (type, nodes); bfaces: list of (type, nodes, tag); 0-based ids."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        et = np.array([t for t, _ in elems], dtype=np.int32)
+        ep = np.concatenate([[0], np.cumsum([len(n) for _, n in elems])]).astype(np.int32)
+        en = np.concatenate([np.asarray(n) for _, n in elems]).astype(np.int32)
+        bt = np.array([t for t, _, _ in bfaces], dtype=np.int32)
+        bp = np.concatenate([[0], np.cumsum([len(n) for _, n, _ in bfaces])]).astype(np.int32)
+        bn = np.concatenate([np.asarray(n) for _, n, _ in bfaces]).astype(np.int32)
+        bg = np.array([g for _, _, g in bfaces], dtype=np.int32)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        h = C.c_void_p()
+        rc = lib().pcfd_mesh_build_from(len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_double)), len(et), ip(et),
+                                        ip(ep), ip(en), len(bt), ip(bt), ip(bp), ip(bn), ip(bg), int(rcm),
+                                        C.byref(h))
+        if rc != 0:
+            raise RuntimeError(lib().pcfd_mesh_error().decode())
+        return cls(None, _handle=h)
 
     def colour(self) -> tuple[np.ndarray, int]:
         col = np.empty(self.nnode, dtype=np.int32)

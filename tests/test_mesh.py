@@ -1,0 +1,123 @@
+"""Mesh set-up (edges, half-edges, median-dual metrics, RCM, colouring) on the CPU.
+
+Pinned by: the reference's own known answers on its test meshes
+(cube.msh V=1, nhex.ugrid V=0.00990099), closed-form counts of the Kuhn
+split, exact domain volumes, control-volume closure, and the identity
+sum(dual volumes) == sum(element volumes).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from proteuscfd_amd import mesh as M
+
+from .mesh_readers import read_gmsh2, read_ugrid
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def kuhn_counts(nx, ny, nz):
+    """Node/edge/boundary counts of an nx*ny*nz hex grid split into Kuhn tets."""
+    nodes = (nx + 1) * (ny + 1) * (nz + 1)
+    axis = nx * (ny + 1) * (nz + 1) + (nx + 1) * ny * (nz + 1) + (nx + 1) * (ny + 1) * nz
+    face = nx * ny * (nz + 1) + nx * (ny + 1) * nz + (nx + 1) * ny * nz
+    body = nx * ny * nz
+    tris = 4 * (nx * ny + nx * nz + ny * nz)
+    return nodes, axis + face + body, tris, 3 * tris
+
+
+def closure(m):
+    s = np.zeros((m.nnode, 3))
+    A = m.edge_a[:, :3] * m.edge_a[:, 3:4]
+    np.add.at(s, m.edges[:, 0], A)
+    np.add.at(s, m.edges[:, 1], -A)
+    np.add.at(s, m.bedges[:, 0], m.bedge_a[:, :3] * m.bedge_a[:, 3:4])
+    return np.abs(s).max()
+
+
+def test_reference_cube_msh_volume():
+    xyz, elems, bfaces = read_gmsh2(os.path.join(GOLD, "cube.msh"))
+    assert len(xyz) == 14 and len(elems) + len(bfaces) == 48
+    m = M.Mesh.from_elements(xyz, elems, bfaces)
+    assert abs(m.total_volume - 1.0) < 1e-7
+    assert closure(m) < 1e-12
+    assert (m.vol > 0).all()
+
+
+def test_reference_nhex_ugrid_volume():
+    xyz, elems, bfaces = read_ugrid(os.path.join(GOLD, "nhex.ugrid"))
+    assert len(xyz) == 4276 and len(elems) + len(bfaces) == 10244
+    m = M.Mesh.from_elements(xyz, elems, bfaces)
+    assert abs(m.total_volume - 0.00990099) < 1e-7
+    assert closure(m) < 1e-12
+    assert (m.vol > 0).all()
+
+
+@pytest.mark.parametrize("n", [(60, 30, 2), (12, 5, 3)])  # ramp kink x=0.5 on a grid line
+def test_wedge_counts_volume_closure(n):
+    nx, ny, nz = n
+    m = M.Mesh.build(M.WEDGE, nx=nx, ny=ny, nz=nz)
+    nodes, edges, tris, he = kuhn_counts(nx, ny, nz)
+    assert (m.nnode, m.nedge, m.nbface, m.nbedge) == (nodes, edges, tris, he)
+    assert m.nelem == 6 * nx * ny * nz
+    # channel area: 3*1 minus the ramp triangle under tan(15)*(x-0.5) on [0.5,3]
+    area = 3.0 - math.tan(math.radians(15.0)) * 2.5 ** 2 / 2.0
+    assert abs(m.total_volume - 0.1 * area) < 1e-12
+    assert abs(m.total_volume - m.elem_volume) < 1e-13
+    assert closure(m) < 1e-13
+    assert (m.vol > 0).all() and (m.edge_a[:, 3] > 0).all()
+    np.testing.assert_allclose(np.linalg.norm(m.edge_a[:, :3], axis=1), 1.0, atol=1e-15)
+
+
+def test_cfg1_counts_match_survey():
+    # 5,673 nodes, 31,232 edges, 7,920 boundary triangles, 23,760 half-edges
+    m = M.Mesh.build(M.WEDGE)
+    assert (m.nnode, m.nedge, m.nbface, m.nbedge) == (5673, 31232, 7920, 23760)
+
+
+def test_bump_rcm_and_boundary_areas():
+    m = M.Mesh.build(M.BUMP, nx=14, ny=14, nz=14)
+    nodes, edges, tris, he = kuhn_counts(14, 14, 14)
+    assert (m.nnode, m.nedge, m.nbedge) == (nodes, edges, he)
+    # RCM is a permutation and keeps metrics valid
+    assert sorted(m.perm.tolist()) == list(range(m.nnode))
+    assert closure(m) < 1e-13
+    assert abs(m.total_volume - m.elem_volume) < 1e-13
+    # bandwidth of the RCM numbering is far below the lexicographic one
+    bw = np.abs(m.edges[:, 1] - m.edges[:, 0]).max()
+    assert bw < 15 * 15 * 3
+    # half-edge areas on a flat face sum to the face area (x=0 face: 1x1)
+    a = m.bedge_a
+    for tag in (1, 2, 3, 4, 6):
+        sel = m.bfactag == tag
+        assert abs(a[sel, 3].sum() - 1.0) < 1e-12
+
+
+def test_plate_prisms():
+    m = M.Mesh.build(M.PLATE, nx=8, ny=3, nz=10, nlayers=5, first_h=1e-3, growth=1.2, rcm=0)
+    assert (m.elem_type == M.PRISM).sum() == 2 * 8 * 3 * 5
+    assert closure(m) < 1e-13
+    assert abs(m.total_volume - m.spec.lx * m.spec.ly * m.spec.lz) < 1e-12
+    assert (m.vol > 0).all()
+
+
+def test_colouring_is_proper():
+    m = M.Mesh.build(M.BUMP, nx=10, ny=10, nz=10)
+    col, nc = m.colour()
+    assert nc >= 2 and col.min() == 0 and col.max() == nc - 1
+    assert (col[m.edges[:, 0]] != col[m.edges[:, 1]]).all()
+
+
+def test_external_mesh_orientation_is_repaired():
+    """A tet given with negative orientation and an inward-wound boundary face
+    give the same metrics as the positively oriented input."""
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    faces = [(0, [0, 2, 1], 1), (0, [0, 1, 3], 2), (0, [0, 3, 2], 3), (0, [1, 2, 3], 4)]
+    good = M.Mesh.from_elements(xyz, [(M.TET, [0, 1, 2, 3])], faces)
+    bad = M.Mesh.from_elements(xyz, [(M.TET, [0, 2, 1, 3])], [(t, n[::-1], g) for t, n, g in faces])
+    np.testing.assert_allclose(bad.vol, good.vol, rtol=1e-14)
+    np.testing.assert_allclose(good.vol, 1.0 / 24.0, rtol=1e-14)
+    np.testing.assert_allclose(bad.edge_a, good.edge_a, rtol=1e-14, atol=1e-16)
+    np.testing.assert_allclose(np.sort(bad.bedge_a[:, 3]), np.sort(good.bedge_a[:, 3]), rtol=1e-14)
