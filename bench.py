@@ -1,0 +1,243 @@
+"""Benchmark: time to an 8-order residual drop, chosen form (FP32 Jacobian +
+SGS, FP64 residual/gradients/norms) vs the plain FP64 form, on the 1.03M-node
+second-order Euler bump case, on one MI355X.
+
+A "step" here is one complete steady solve from freestream until
+residGlobal <= 1e-8 * residGlobal(iteration 1); --steps K times K solves of
+each form, --warmup W runs W untimed implicit iterations first.  Prints one
+JSON line (rank 0).  Per-kernel device times come from HIP events on the
+context stream (pcfd_phase_times); the CPU baseline times the oracle on the
+host on a bounded sample (see cpu_baseline.sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time to 8-order residual drop, chosen vs plain FP64 (s); edge fluxes/s; HBM%"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(m, nblocks, prec_bytes):
+    nn, ne, nb = m.nnode, m.nedge, m.nbedge
+    return {
+        # SURVEY-defined compulsory bytes (indices int32, state FP64)
+        "residual_2nd": 264 * nn + 40 * ne + 84 * nb,
+        "residual_1st": 80 * nn + 40 * ne + 84 * nb,
+        "sgs_sweep": nblocks * 25 * prec_bytes + 8 * 15 * nn,
+        "jacobian": nblocks * 25 * prec_bytes + 80 * nn + 40 * ne,
+        "gradient5": nn * 8 * (5 + 6 + 3 + 15) + 8 * ne,
+        "limiter": nn * 8 * (5 + 15 + 3 + 1 + 5) + 8 * ne,
+    }
+
+
+def phase_profile(ctx, case, iters):
+    """Run `iters` implicit iterations with per-phase HIP-event timing."""
+    ctx.upload_q(case.initial_state())
+    ctx.reset_iteration()
+    ctx.enable_phase_timing(True)
+    ctx.phase_times(reset=True)
+    for _ in range(iters):
+        ctx.iterate()
+    pt = ctx.phase_times(reset=True)
+    ctx.enable_phase_timing(False)
+    return {k: v / iters for k, v in pt.items()}
+
+
+def timed_solves(ctx, case, k, max_iter, drop):
+    times, iters, hists = [], [], []
+    for _ in range(k):
+        ctx.upload_q(case.initial_state())
+        ctx.reset_iteration()
+        ctx.synchronize()
+        hist, nit, secs = ctx.solve(max_iter, drop)
+        times.append(secs)
+        iters.append(nit)
+        hists.append(hist)
+    return times, iters, hists
+
+
+def cpu_baseline(case, budget_s):
+    """Oracle (single-threaded FP64 restatement of the reference) on the host:
+    one full implicit iteration of the same mesh and parameters, timed per
+    phase.  Returns seconds per iteration and the residual time."""
+    import oracle as O
+    pb = O.Problem.from_case(case, clip_jacobi=True)
+    t0 = time.time()
+    s = O.Solver(pb, case.initial_state())
+    init = time.time() - t0
+    t0 = time.time()
+    s.step()
+    it_s = time.time() - t0
+    tm = s.timers()
+    s.close()
+    return it_s, tm, init
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed full solves per precision form")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed implicit iterations before timing")
+    ap.add_argument("--n", type=int, default=100, help="hex cells per direction of the bump mesh")
+    ap.add_argument("--max-iter", type=int, default=1000)
+    ap.add_argument("--drop", type=float, default=1e-8)
+    ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--iters-only", type=int, default=0,
+                    help="profiling mode: run this many iterations of each form instead of full solves")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world > 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from proteuscfd_amd import cases, gpu
+    from proteuscfd_amd.cases import Params
+
+    t0 = time.time()
+    base = dict(sorder=2, limiter=3, venkat_K=1.0, cfl=50.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
+    case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
+    m = case.mesh
+    log(f"[bench] mesh bump {args.n}^3: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
+        f"({time.time() - t0:.1f}s)")
+
+    results = {}
+    for prec, name in [(0, "plain_fp64"), (1, "chosen_fp32")]:
+        p = Params(**base, precision=prec)
+        ctx = gpu.Context(case, p, gradient_terms=5)
+        info = ctx.info()
+        # warmup
+        ctx.upload_q(case.initial_state())
+        ctx.reset_iteration()
+        for _ in range(args.warmup):
+            ctx.iterate()
+        ctx.synchronize()
+        prof = phase_profile(ctx, case, args.profile_iters)
+        log(f"[bench] {name} phase ms/iter: " + ", ".join(f"{k}={v:.3f}" for k, v in prof.items()))
+        if args.iters_only:
+            ctx.upload_q(case.initial_state())
+            ctx.reset_iteration()
+            ctx.synchronize()
+            t = time.time()
+            for _ in range(args.iters_only):
+                r = ctx.iterate()
+            ctx.synchronize()
+            times, iters, hists = [time.time() - t], [args.iters_only], [np.array([r[0]])]
+        else:
+            times, iters, hists = timed_solves(ctx, case, args.steps, args.max_iter, args.drop)
+        h = hists[-1]
+        drop = float(h[-1] / h[0]) if len(h) > 1 else 1.0
+        log(f"[bench] {name}: {iters} iterations, {['%.3f' % x for x in times]} s, drop {drop:.2e}")
+        qfinal = ctx.download_q()
+        results[name] = dict(times=times, iters=iters, prof=prof, info=info, drop=drop, q=qfinal)
+        ctx.close()
+
+    plain, chosen = results["plain_fp64"], results["chosen_fp32"]
+    t_plain = float(np.mean(plain["times"]))
+    t_chosen = float(np.mean(chosen["times"]))
+    nblocks = plain["info"]["nblocks"]
+    ab64 = algorithmic_bytes(m, nblocks, 8)
+    ab32 = algorithmic_bytes(m, nblocks, 4)
+
+    def gbs(nbytes, ms):
+        return nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+
+    pp, pc = plain["prof"], chosen["prof"]
+    nsgs = base["nsgs"]
+    sweeps = 2 * nsgs
+    kern = {
+        "residual_2nd_fp64": dict(ms=pp["residual"], gbs=gbs(ab64["residual_2nd"], pp["residual"])),
+        "sgs_sweep_fp64": dict(ms=pp["sgs"] / sweeps, gbs=gbs(ab64["sgs_sweep"], pp["sgs"] / sweeps)),
+        "sgs_sweep_fp32": dict(ms=pc["sgs"] / sweeps, gbs=gbs(ab32["sgs_sweep"], pc["sgs"] / sweeps)),
+        "jacobian_fp64": dict(ms=pp["jacobian"], gbs=gbs(ab64["jacobian"], pp["jacobian"])),
+        "jacobian_fp32": dict(ms=pc["jacobian"], gbs=gbs(ab32["jacobian"], pc["jacobian"])),
+        "gradient": dict(ms=pp["gradient"], gbs=gbs(ab64["gradient5"], pp["gradient"])),
+        "limiter": dict(ms=pp["limiter"], gbs=gbs(ab64["limiter"], pp["limiter"])),
+    }
+    for v in kern.values():
+        v["frac"] = v["gbs"] / HBM_PEAK_GBS
+    per_iter_ms = {k: sum(v.values()) for k, v in [("plain", pp), ("chosen", pc)]}
+    dominant = max(["sgs", "jacobian", "residual"], key=lambda k: pc[k])
+    dom_key = {"sgs": "sgs_sweep_fp32", "jacobian": "jacobian_fp32", "residual": "residual_2nd_fp64"}[dominant]
+    dom = kern[dom_key]
+    edge_flux = (m.nedge + m.nbedge) / (pp["residual"] * 1e-3)
+    # converged-field error of the chosen form against the plain FP64 path
+    qp, qc = plain["q"][: m.nnode, :5], chosen["q"][: m.nnode, :5]
+    field_err = {v: float(np.abs(qc[:, k] - qp[:, k]).max() / np.abs(qp[:, k]).max())
+                 for k, v in enumerate(["rho", "rhou", "rhov", "rhow", "E"])}
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline: one oracle iteration on the host ...")
+        it_s, tm, init = cpu_baseline(case, args.cpu_budget)
+        n_iter = int(np.mean(plain["iters"]))
+        cpu = {
+            "value": it_s * n_iter, "unit": "s (estimated: measured s/iteration x GPU plain iteration count)",
+            "cores": 1, "kind": "port",
+            "sample": f"1 implicit iteration of the same 1.03M-node case ({it_s:.1f} s: jacobian "
+                      f"{tm['jacobian']:.1f} s, residual {tm['residual']:.2f} s, gradient+limiter "
+                      f"{tm['gradient']:.2f} s, SGS {tm['linear']:.1f} s) x {n_iter} iterations",
+            "edge_flux_per_s": (m.nedge + m.nbedge) / tm["residual"] if tm["residual"] > 0 else None,
+            "s_per_iteration": it_s,
+        }
+        log(f"[bench] cpu: {it_s:.2f} s/iteration")
+
+    out = {
+        "metric": METRIC,
+        "value": t_chosen,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps if not args.iters_only else 1,
+        "warmup": args.warmup,
+        "ms_per_step": t_chosen * 1e3,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64 residual/gradients/norms, f32 Jacobian+SGS (plain path all f64)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
+                        f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
+                        f"M=0.5 farfield, weighted LSQ, modified Venkat K=1, first-order FD Jacobian, "
+                        f"8 SGS, CFL 1->50 over 20 iterations",
+            "step": "one full steady solve from freestream to an 8-order residual drop",
+            "parallelism": "single GPU",
+        },
+        "plain_fp64_s": t_plain,
+        "chosen_fp32_s": t_chosen,
+        "speedup_chosen_vs_plain": t_plain / t_chosen if t_chosen > 0 else None,
+        "iterations": {"plain": plain["iters"], "chosen": chosen["iters"]},
+        "residual_drop": {"plain": plain["drop"], "chosen": chosen["drop"]},
+        "edge_fluxes_per_s": edge_flux,
+        "cell_iterations_per_s": m.nnode * float(np.mean(chosen["iters"])) / t_chosen,
+        "chosen_vs_plain_field_max_rel_err": field_err,
+        "per_iteration_ms": per_iter_ms,
+        "phase_ms": {"plain": pp, "chosen": pc},
+        "kernels": kern,
+        "roofline": {"bound": "hbm", "kernel": dom_key, "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": None},
+        "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true>", "achieved": kern["residual_2nd_fp64"]["gbs"],
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern["residual_2nd_fp64"]["frac"]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
