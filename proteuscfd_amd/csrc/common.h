@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -11,28 +12,43 @@
 namespace pcfd {
 
 constexpr int kBlock = 256;  // threads per workgroup for node kernels (4 wavefronts)
+constexpr int kW = 64;       // wavefront width = interleave width of the ELL/SELL layouts
 
-// Device arrays.  State is structure-of-arrays: component k of node i lives
-// at base[k*stride + i].
+// Device arrays.  Node state is structure-of-arrays: component k of node i
+// lives at base[k*stride + i].
+//
+// gELL (node-gather lists): nodes are cut into chunks of 64 consecutive ids;
+// chunk c holds gwidth[c] slots, slot s of lane l (node 64c+l) at
+// index (gbase[c]+s)*64 + l.  Slots list the node's neighbours in ascending
+// id, -1 after the last one.  Consecutive lanes read consecutive addresses.
+//
+// SELL (block matrix): rows grouped by colour, within a colour sorted by
+// degree (descending) then id, padded to whole chunks of 64 rows (srow = -1
+// for padding).  Chunk k stores its off-diagonal blocks as
+// [slot][entry 0..24][lane] from sbase[k]*25*64, the diagonal block as
+// [entry][lane] at k*25*64; slot s of a row is its s-th neighbour in
+// ascending id, scol = -1 for padding.
 struct Dev {
   // geometry
   double* xyz = nullptr;    // 3 x nnode
   double* vol = nullptr;    // nnode
   double* sw = nullptr;     // 6 x nnode, weighted LSQ moments
-  // node-gather CSR over interior edges, neighbours ascending
-  int* row_ptr = nullptr;   // nnode+1
-  int* col = nullptr;       // nnz (= 2*nedge)
-  double* nrm = nullptr;    // 4 x nnz: canonical (n0->n1) unit normal + |a| of the entry's edge
-  int* blk_t = nullptr;     // nnz: block index of the transposed entry (col row, this node col)
+  // gELL gather lists
+  int* gwidth = nullptr;    // nchunk_g
+  int* gbase = nullptr;     // nchunk_g
+  int* gcol = nullptr;      // nslot_g*64
+  double* gnrm = nullptr;   // 4 x nslot_g*64: canonical (n0->n1) unit normal + |a|
   // half-edges
   int* he_ptr = nullptr;    // nnode+1: half-edges of node i, in half-edge order
   int* he_id = nullptr;     // nbnode
   int* he_tag = nullptr;    // nbnode
   double* he_a = nullptr;   // 4 x nbnode
   int* bnd_nodes = nullptr; // nodes with >= 1 half-edge
+  int* bidx = nullptr;      // nnode: index into bnd_nodes, -1 for nodes without half-edges
+  double* jdb = nullptr;    // 25 x nbnd boundary part of the Jacobian diagonal
   // boundary-condition tables per face tag
   int* bc_type = nullptr;   // ntag
-  double* qref = nullptr;   // 5 x ntag (conservative), + 1 x ntag aux pressure at [5*ntag]
+  double* qref = nullptr;   // 5 x ntag (conservative), aux pressure at [5*ntag + tag]
   // state
   double* q = nullptr;      // 5 x ntot (interior then phantoms)
   double* grad = nullptr;   // 27 x nnode (term*3+dir)
@@ -40,15 +56,20 @@ struct Dev {
   double* lim2 = nullptr;   // 5 x nnode (pre-clip copy)
   double* b = nullptr;      // 5 x nnode
   double* dt = nullptr;     // nnode
-  double* xd = nullptr;     // 5 x nnode (x in FP64 mode)
-  float* xf = nullptr;      // 5 x nnode (x in FP32 mode)
-  // block CRS: row i blocks at [blk_ptr[i], blk_ptr[i+1]), diagonal first
-  int* blk_ptr = nullptr;   // nnode+1
-  double* Md = nullptr;     // nblocks*25 (FP64 mode)
-  float* Mf = nullptr;      // nblocks*25 (FP32 mode)
-  int* piv = nullptr;       // 5 x nnode
-  // colouring
-  int* color_nodes = nullptr;
+  // SELL block matrix
+  int* srow = nullptr;      // nrow_pad: node of each padded row (-1 = padding)
+  int* swidth = nullptr;    // nchunk_s
+  int* sbase = nullptr;     // nchunk_s
+  int* scol = nullptr;      // nslot_s*64
+  double* Md = nullptr;     // nslot_s*64*25 (FP64 mode)
+  float* Mf = nullptr;      // nslot_s*64*25 (FP32 mode)
+  double* Dd = nullptr;     // nchunk_s*64*25
+  float* Df = nullptr;      // nchunk_s*64*25
+  int* piv = nullptr;       // nrow_pad, 3-bit packed row permutation
+  double* bsd = nullptr;    // 5 x nrow_pad right-hand side in SELL row order
+  float* bsf = nullptr;
+  double* x8d = nullptr;    // 8 x nnode AoS correction (5 used), FP64 mode
+  float* x8f = nullptr;     // 8 x nnode AoS correction, FP32 mode
   // reductions
   double* red = nullptr;    // partial sums
   double* red_out = nullptr;
@@ -60,10 +81,14 @@ struct Ctx {
   hipStream_t stream = nullptr;
   std::string err;
   int nnode = 0, nbnode = 0, nedge = 0, ntot = 0, ntag = 0, nnz = 0, nbnd = 0;
-  long long nblocks = 0;
+  int nchunk_g = 0, nchunk_s = 0, nrow_pad = 0;
+  long long nslot_g = 0, nslot_s = 0;
+  long long nblocks = 0;  // nnode + 2*nedge (reference count)
   int ncolor = 0;
-  std::vector<int> color_ptr;  // host copy
+  std::vector<int> color_chunk;  // [ncolor+1] chunk range of each colour
   std::vector<int> h_bc_type;
+  // host copies used by downloads
+  std::vector<int> h_row_ptr, h_col, h_srow, h_spos, h_sbase;
   pcfd_params prm{};
   bool have_params = false;
   int iter = 1;
@@ -79,13 +104,18 @@ struct Ctx {
   long long device_bytes = 0;
 };
 
-#define PCFD_HIP(call)                                                                   \
-  do {                                                                                   \
-    hipError_t e_ = (call);                                                              \
+#define PCFD_HIP(call)                                                                                   \
+  do {                                                                                                   \
+    hipError_t e_ = (call);                                                                              \
     if (e_ != hipSuccess) throw std::runtime_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 inline int nblk(long long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
+
+// gELL traversal helpers: slot s of node i
+__host__ __device__ inline long long gell_index(const int* gbase, int i, int s) {
+  return ((long long)gbase[i >> 6] + s) * kW + (i & 63);
+}
 
 // launchers (each enqueues on ctx.stream; throw std::runtime_error on failure)
 void launch_lsq_moments(Ctx& c);

@@ -1,4 +1,4 @@
-// C ABI of libpcfd.so: context lifetime, transfers and single phases.
+// C ABI of libpcfd.so: context lifetime, layouts, transfers and single phases.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -39,10 +39,12 @@ void dfree(T*& p) {
 
 void free_all(Ctx& c) {
   Dev& d = c.d;
-  dfree(d.xyz); dfree(d.vol); dfree(d.sw); dfree(d.row_ptr); dfree(d.col); dfree(d.nrm); dfree(d.blk_t);
-  dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bc_type);
-  dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt); dfree(d.xd);
-  dfree(d.xf); dfree(d.blk_ptr); dfree(d.Md); dfree(d.Mf); dfree(d.piv); dfree(d.color_nodes); dfree(d.red);
+  dfree(d.xyz); dfree(d.vol); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
+  dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
+  dfree(d.jdb); dfree(d.bc_type);
+  dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
+  dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
+  dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.x8d); dfree(d.x8f); dfree(d.red);
   dfree(d.red_out); dfree(d.icount);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -68,6 +70,14 @@ int guarded(pcfd_ctx* ctx, F&& f) {
   }
 }
 
+template <typename T>
+void fetch(std::vector<double>& h, const T* src, size_t n) {
+  std::vector<T> t(n);
+  PCFD_HIP(hipMemcpy(t.data(), src, n * sizeof(T), hipMemcpyDeviceToHost));
+  h.resize(n);
+  for (size_t k = 0; k < n; ++k) h[k] = (double)t[k];
+}
+
 int need_params(Ctx& c) {
   if (!c.have_params) {
     c.err = "pcfd_set_params has not been called";
@@ -77,11 +87,17 @@ int need_params(Ctx& c) {
 }
 
 void ensure_matrix(Ctx& c) {
+  const size_t off = (size_t)c.nslot_s * kW * 25, dg = (size_t)c.nchunk_s * kW * 25;
   if (c.prm.precision == PCFD_CHOSEN_FP32) {
-    if (!c.d.Mf) dalloc(c, &c.d.Mf, (size_t)c.nblocks * 25);
-    if (!c.d.xf) dalloc(c, &c.d.xf, 5 * (size_t)c.nnode);
+    if (!c.d.Mf) dalloc(c, &c.d.Mf, off);
+    if (!c.d.Df) dalloc(c, &c.d.Df, dg);
+    if (!c.d.bsf) dalloc(c, &c.d.bsf, 5 * (size_t)c.nrow_pad);
+    if (!c.d.x8f) dalloc(c, &c.d.x8f, 8 * (size_t)c.nnode);
   } else {
-    if (!c.d.Md) dalloc(c, &c.d.Md, (size_t)c.nblocks * 25);
+    if (!c.d.Md) dalloc(c, &c.d.Md, off);
+    if (!c.d.Dd) dalloc(c, &c.d.Dd, dg);
+    if (!c.d.bsd) dalloc(c, &c.d.bsd, 5 * (size_t)c.nrow_pad);
+    if (!c.d.x8d) dalloc(c, &c.d.x8d, 8 * (size_t)c.nnode);
   }
 }
 
@@ -105,7 +121,7 @@ void ensure_matrix_alloc(Ctx& c) { ensure_matrix(c); }
 
 extern "C" {
 
-const char* pcfd_version(void) { return "pcfd-gfx950 0.1"; }
+const char* pcfd_version(void) { return "pcfd-gfx950 0.2"; }
 
 const char* pcfd_last_error(const pcfd_ctx* ctx) {
   if (!ctx) return g_create_err.c_str();
@@ -154,50 +170,78 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     c.ntot = nn + nb;
     c.ntag = nt;
     c.nnz = 2 * ne;
-    // node-gather CSR, neighbours ascending, with the canonical edge normal
-    std::vector<int> deg(nn + 1, 0);
-    for (int e = 0; e < ne; ++e) {
-      deg[m->edges[2 * e] + 1]++;
-      deg[m->edges[2 * e + 1] + 1]++;
-    }
+    // ---- node adjacency, neighbours ascending, with the canonical edge id
     std::vector<int> row_ptr(nn + 1, 0);
-    for (int i = 0; i < nn; ++i) row_ptr[i + 1] = row_ptr[i] + deg[i + 1];
-    std::vector<std::pair<int, int>> ent(c.nnz);
-    std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
     for (int e = 0; e < ne; ++e) {
-      int a = m->edges[2 * e], b = m->edges[2 * e + 1];
-      ent[fill[a]++] = {b, e};
-      ent[fill[b]++] = {a, e};
+      row_ptr[m->edges[2 * e] + 1]++;
+      row_ptr[m->edges[2 * e + 1] + 1]++;
     }
-    for (int i = 0; i < nn; ++i) std::sort(ent.begin() + row_ptr[i], ent.begin() + row_ptr[i + 1]);
-    std::vector<int> col(c.nnz);
-    std::vector<double> nrm(4 * (size_t)c.nnz);
-    for (int p = 0; p < c.nnz; ++p) {
-      col[p] = ent[p].first;
-      int e = ent[p].second;
-      for (int k = 0; k < 4; ++k) nrm[(size_t)k * c.nnz + p] = m->edge_a[4 * (size_t)e + k];
+    for (int i = 0; i < nn; ++i) row_ptr[i + 1] += row_ptr[i];
+    std::vector<std::pair<int, int>> ent(c.nnz);
+    {
+      std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
+      for (int e = 0; e < ne; ++e) {
+        int a = m->edges[2 * e], b = m->edges[2 * e + 1];
+        ent[fill[a]++] = {b, e};
+        ent[fill[b]++] = {a, e};
+      }
     }
-    for (int i = 0; i < nn; ++i)
+    for (int i = 0; i < nn; ++i) {
+      std::sort(ent.begin() + row_ptr[i], ent.begin() + row_ptr[i + 1]);
       for (int p = row_ptr[i] + 1; p < row_ptr[i + 1]; ++p)
-        if (col[p] == col[p - 1]) throw std::runtime_error("pcfd_create: duplicate edge between nodes");
+        if (ent[p].first == ent[p - 1].first) throw std::runtime_error("pcfd_create: duplicate edge between nodes");
+    }
+    auto deg = [&](int i) { return row_ptr[i + 1] - row_ptr[i]; };
+    // ---- gELL: chunks of 64 consecutive nodes
+    c.nchunk_g = (nn + kW - 1) / kW;
+    std::vector<int> gwidth(c.nchunk_g, 0), gbase(c.nchunk_g, 0);
+    long long slots = 0;
+    for (int k = 0; k < c.nchunk_g; ++k) {
+      int w = 0;
+      for (int i = k * kW; i < std::min(nn, (k + 1) * kW); ++i) w = std::max(w, deg(i));
+      gwidth[k] = w;
+      gbase[k] = (int)slots;
+      slots += w;
+    }
+    if (slots * kW > (1LL << 31)) throw std::runtime_error("pcfd_create: mesh too large for 32-bit slot offsets");
+    c.nslot_g = slots;
+    std::vector<int> gcol((size_t)slots * kW, -1);
+    std::vector<double> gnrm(4 * (size_t)slots * kW, 0.0);
+    const size_t gstride = (size_t)slots * kW;
+    for (int i = 0; i < nn; ++i) {
+      for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+        size_t idx = ((size_t)gbase[i / kW] + (p - row_ptr[i])) * kW + (i % kW);
+        gcol[idx] = ent[p].first;
+        int e = ent[p].second;
+        for (int k = 0; k < 4; ++k) gnrm[k * gstride + idx] = m->edge_a[4 * (size_t)e + k];
+      }
+    }
+    c.h_row_ptr = row_ptr;
+    c.h_col.resize(c.nnz);
+    for (int p = 0; p < c.nnz; ++p) c.h_col[p] = ent[p].first;
     ent.clear();
     ent.shrink_to_fit();
-    // half-edges per node in half-edge order
-    std::vector<int> hdeg(nn + 1, 0);
-    for (int h = 0; h < nb; ++h) hdeg[m->bedges[2 * h] + 1]++;
+    // ---- half-edges per node in half-edge order
     std::vector<int> he_ptr(nn + 1, 0);
-    for (int i = 0; i < nn; ++i) he_ptr[i + 1] = he_ptr[i] + hdeg[i + 1];
-    std::vector<int> he_id(nb), hfill(he_ptr.begin(), he_ptr.end() - 1);
-    for (int h = 0; h < nb; ++h) he_id[hfill[m->bedges[2 * h]]++] = h;
-    std::vector<int> bnd;
+    for (int h = 0; h < nb; ++h) he_ptr[m->bedges[2 * h] + 1]++;
+    for (int i = 0; i < nn; ++i) he_ptr[i + 1] += he_ptr[i];
+    std::vector<int> he_id(nb);
+    {
+      std::vector<int> hfill(he_ptr.begin(), he_ptr.end() - 1);
+      for (int h = 0; h < nb; ++h) he_id[hfill[m->bedges[2 * h]]++] = h;
+    }
+    std::vector<int> bnd, bidx(nn, -1);
     for (int i = 0; i < nn; ++i)
-      if (he_ptr[i + 1] > he_ptr[i]) bnd.push_back(i);
+      if (he_ptr[i + 1] > he_ptr[i]) {
+        bidx[i] = (int)bnd.size();
+        bnd.push_back(i);
+      }
     c.nbnd = (int)bnd.size();
     std::vector<int> he_tag(m->bfactag, m->bfactag + nb);
     std::vector<double> he_a(4 * (size_t)nb);
     for (int h = 0; h < nb; ++h)
       for (int k = 0; k < 4; ++k) he_a[(size_t)k * nb + h] = m->bedge_a[4 * (size_t)h + k];
-    // SoA geometry
+    // ---- SoA geometry and BC tables
     std::vector<double> xyz(3 * (size_t)nn);
     for (int i = 0; i < nn; ++i)
       for (int k = 0; k < 3; ++k) xyz[(size_t)k * nn + i] = m->xyz[3 * (size_t)i + k];
@@ -206,33 +250,69 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     std::vector<double> qref(6 * (size_t)nt, 0.0);
     for (int t = 0; t < nt; ++t)
       for (int k = 0; k < 5; ++k) qref[(size_t)k * nt + t] = m->qref[5 * (size_t)t + k];
-    // block-CRS pattern: row i = [diag, neighbours ascending]
-    std::vector<int> blk_ptr(nn + 1);
-    for (int i = 0; i <= nn; ++i) blk_ptr[i] = row_ptr[i] + i;
     c.nblocks = (long long)nn + c.nnz;
-    // colouring for the SGS sweeps
+    // ---- colouring and the SELL row order
     std::vector<int> colour(nn);
     int ncol = pcfd_color_nodes(nn, ne, m->edges, colour.data());
     if (ncol < 0) throw std::runtime_error(pcfd_mesh_error());
     c.ncolor = ncol;
-    c.color_ptr.assign(ncol + 1, 0);
-    for (int i = 0; i < nn; ++i) c.color_ptr[colour[i] + 1]++;
-    for (int k = 0; k < ncol; ++k) c.color_ptr[k + 1] += c.color_ptr[k];
-    std::vector<int> cnodes(nn), cfill(c.color_ptr.begin(), c.color_ptr.end() - 1);
-    for (int i = 0; i < nn; ++i) cnodes[cfill[colour[i]]++] = i;
+    std::vector<std::vector<int>> members(ncol);
+    for (int i = 0; i < nn; ++i) members[colour[i]].push_back(i);
+    std::vector<int> srow;
+    c.color_chunk.assign(ncol + 1, 0);
+    for (int k = 0; k < ncol; ++k) {
+      auto& v = members[k];
+      std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return deg(a) > deg(b); });
+      srow.insert(srow.end(), v.begin(), v.end());
+      while (srow.size() % kW) srow.push_back(-1);
+      c.color_chunk[k + 1] = (int)(srow.size() / kW);
+    }
+    c.nrow_pad = (int)srow.size();
+    c.nchunk_s = c.nrow_pad / kW;
+    std::vector<int> swidth(c.nchunk_s, 0), sbase(c.nchunk_s, 0), spos(nn, -1);
+    slots = 0;
+    for (int k = 0; k < c.nchunk_s; ++k) {
+      int w = 0;
+      for (int l = 0; l < kW; ++l) {
+        int i = srow[(size_t)k * kW + l];
+        if (i >= 0) {
+          w = std::max(w, deg(i));
+          spos[i] = k * kW + l;
+        }
+      }
+      swidth[k] = w;
+      sbase[k] = (int)slots;
+      slots += w;
+    }
+    if (slots * kW * 25 > (1LL << 40)) throw std::runtime_error("pcfd_create: matrix too large");
+    c.nslot_s = slots;
+    std::vector<int> scol((size_t)slots * kW, -1);
+    for (int k = 0; k < c.nchunk_s; ++k)
+      for (int l = 0; l < kW; ++l) {
+        int i = srow[(size_t)k * kW + l];
+        if (i < 0) continue;
+        for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p)
+          scol[((size_t)sbase[k] + (p - row_ptr[i])) * kW + l] = c.h_col[p];
+      }
+    c.h_srow = srow;
+    c.h_spos = spos;
+    c.h_sbase = sbase;
 
     Dev& d = c.d;
     dput(c, &d.xyz, xyz);
     dput(c, &d.vol, vol);
     dalloc(c, &d.sw, 6 * (size_t)nn);
-    dput(c, &d.row_ptr, row_ptr);
-    dput(c, &d.col, col);
-    dput(c, &d.nrm, nrm);
+    dput(c, &d.gwidth, gwidth);
+    dput(c, &d.gbase, gbase);
+    dput(c, &d.gcol, gcol);
+    dput(c, &d.gnrm, gnrm);
     dput(c, &d.he_ptr, he_ptr);
     dput(c, &d.he_id, he_id);
     dput(c, &d.he_tag, he_tag);
     dput(c, &d.he_a, he_a);
     dput(c, &d.bnd_nodes, bnd);
+    dput(c, &d.bidx, bidx);
+    dalloc(c, &d.jdb, 25 * (size_t)std::max(c.nbnd, 1));
     dput(c, &d.bc_type, c.h_bc_type);
     dput(c, &d.qref, qref);
     dalloc(c, &d.q, 5 * (size_t)c.ntot);
@@ -241,10 +321,11 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     dalloc(c, &d.lim2, 5 * (size_t)nn);
     dalloc(c, &d.b, 5 * (size_t)nn);
     dalloc(c, &d.dt, (size_t)nn);
-    dalloc(c, &d.xd, 5 * (size_t)nn);
-    dput(c, &d.blk_ptr, blk_ptr);
-    dalloc(c, &d.piv, (size_t)nn);
-    dput(c, &d.color_nodes, cnodes);
+    dput(c, &d.srow, srow);
+    dput(c, &d.swidth, swidth);
+    dput(c, &d.sbase, sbase);
+    dput(c, &d.scol, scol);
+    dalloc(c, &d.piv, (size_t)c.nrow_pad);
     dalloc(c, &d.red, 6 * (size_t)std::max(nblk(nn), 1024));
     dalloc(c, &d.red_out, 8);
     dalloc(c, &d.icount, 4);
@@ -470,35 +551,34 @@ int pcfd_download_jacobian(pcfd_ctx* ctx, const int* ia, const int* ja, double* 
       c.err = "pcfd_download_jacobian: no unfactored Jacobian on the device";
       return PCFD_ERR_STATE;
     }
-    std::vector<double> h((size_t)c.nblocks * 25);
+    std::vector<double> off, dg;
+    const size_t noff = (size_t)c.nslot_s * kW * 25, ndg = (size_t)c.nchunk_s * kW * 25;
     if (c.prm.precision == PCFD_CHOSEN_FP32) {
-      std::vector<float> f(h.size());
-      PCFD_HIP(hipMemcpy(f.data(), c.d.Mf, f.size() * sizeof(float), hipMemcpyDeviceToHost));
-      for (size_t k = 0; k < f.size(); ++k) h[k] = f[k];
+      fetch(off, c.d.Mf, noff);
+      fetch(dg, c.d.Df, ndg);
     } else {
-      PCFD_HIP(hipMemcpy(h.data(), c.d.Md, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      fetch(off, c.d.Md, noff);
+      fetch(dg, c.d.Dd, ndg);
     }
-    std::vector<int> row_ptr(c.nnode + 1), col(c.nnz);
-    PCFD_HIP(hipMemcpy(row_ptr.data(), c.d.row_ptr, row_ptr.size() * sizeof(int), hipMemcpyDeviceToHost));
-    PCFD_HIP(hipMemcpy(col.data(), c.d.col, col.size() * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < c.nnode; ++i) {
-      for (int k = ia[i]; k < ia[i + 1]; ++k) {
-        int j = ja[k];
-        long long src = -1;
+      const int r = c.h_spos[i], k = r / kW, l = r % kW;
+      for (int b = ia[i]; b < ia[i + 1]; ++b) {
+        const int j = ja[b];
+        double* out = M + (size_t)b * 25;
         if (j == i) {
-          src = row_ptr[i] + i;
-        } else {
-          const int* b = col.data() + row_ptr[i];
-          const int* e = col.data() + row_ptr[i + 1];
-          const int* f = std::lower_bound(b, e, j);
-          if (f != e && *f == j) src = (long long)row_ptr[i] + i + 1 + (f - b);
+          for (int e = 0; e < 25; ++e) out[e] = dg[((size_t)k * 25 + e) * kW + l];
+          continue;
         }
-        if (src < 0) {
+        const int* cb = c.h_col.data() + c.h_row_ptr[i];
+        const int* ce = c.h_col.data() + c.h_row_ptr[i + 1];
+        const int* f = std::lower_bound(cb, ce, j);
+        if (f == ce || *f != j) {
           c.err = "pcfd_download_jacobian: block (" + std::to_string(i) + "," + std::to_string(j) +
                   ") is not in the mesh graph";
           return PCFD_ERR_ARG;
         }
-        std::memcpy(M + (size_t)k * 25, h.data() + (size_t)src * 25, 25 * sizeof(double));
+        const size_t s = (size_t)c.h_sbase[k] + (size_t)(f - cb);
+        for (int e = 0; e < 25; ++e) out[e] = off[(s * 25 + e) * kW + l];
       }
     }
     return PCFD_OK;
@@ -530,15 +610,12 @@ int pcfd_sgs(pcfd_ctx* ctx, int nsgs, double* x_out, double* delta) {
     }
     launch_sgs(c, nsgs, delta);
     if (x_out) {
-      if (c.prm.precision == PCFD_CHOSEN_FP32) {
-        std::vector<float> f(5 * (size_t)c.nnode);
-        PCFD_HIP(hipMemcpyAsync(f.data(), c.d.xf, f.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-        PCFD_HIP(hipStreamSynchronize(c.stream));
-        for (int i = 0; i < c.nnode; ++i)
-          for (int k = 0; k < 5; ++k) x_out[(size_t)i * 5 + k] = f[(size_t)k * c.nnode + i];
-        return PCFD_OK;
-      }
-      return download_soa(c, c.d.xd, 5, x_out);
+      std::vector<double> h;
+      if (c.prm.precision == PCFD_CHOSEN_FP32) fetch(h, c.d.x8f, 8 * (size_t)c.nnode);
+      else fetch(h, c.d.x8d, 8 * (size_t)c.nnode);
+      for (int i = 0; i < c.nnode; ++i)
+        for (int k = 0; k < 5; ++k) x_out[(size_t)i * 5 + k] = h[(size_t)i * 8 + k];
+      return PCFD_OK;
     }
     PCFD_HIP(hipStreamSynchronize(c.stream));
     return PCFD_OK;
@@ -548,7 +625,9 @@ int pcfd_sgs(pcfd_ctx* ctx, int nsgs, double* x_out, double* delta) {
 int pcfd_sgs_order(pcfd_ctx* ctx, int* order) {
   return guarded(ctx, [&](Ctx& c) -> int {
     if (!order) return PCFD_ERR_ARG;
-    PCFD_HIP(hipMemcpy(order, c.d.color_nodes, sizeof(int) * c.nnode, hipMemcpyDeviceToHost));
+    int k = 0;
+    for (int r : c.h_srow)
+      if (r >= 0) order[k++] = r;
     return PCFD_OK;
   });
 }

@@ -7,7 +7,8 @@
 // reaches that node -- evaluating every edge function with the canonical
 // (n0, n1) argument order.  Both ends of an edge therefore see bit-identical
 // edge values, sums come out in the reference's order, and no
-// floating-point atomics are needed.
+// floating-point atomics are needed.  Neighbour lists use the gELL layout
+// (common.h): the s-th neighbours of 64 consecutive nodes are contiguous.
 #include <stdexcept>
 
 #include "common.h"
@@ -15,16 +16,33 @@
 
 namespace pcfd {
 
+struct Gell {
+  const int* width;
+  const int* base;
+  const int* col;
+  const double* nrm;
+  long long stride;  // nslot_g*64, component stride of nrm
+};
+
+__device__ __forceinline__ long long gbase_of(const Gell& g, int i) {
+  return (long long)g.base[i >> 6] * kW + (i & 63);
+}
+
+static Gell gell(const Ctx& c) { return Gell{c.d.gwidth, c.d.gbase, c.d.gcol, c.d.gnrm, c.nslot_g * kW}; }
+
 // ---------------------------------------------------------------------------
 // weighted LSQ moments: Kernel_LSQ_CoefficientsWeighted (ucs/gradient.tcc:460-503)
-__global__ void __launch_bounds__(kBlock) k_lsq_moments(int nnode, const int* __restrict__ row_ptr, const int* __restrict__ col,
-                              const double* __restrict__ xyz, double* __restrict__ sw) {
+__global__ void __launch_bounds__(kBlock) k_lsq_moments(int nnode, Gell g, const double* __restrict__ xyz,
+                                                       double* __restrict__ sw) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double s[6] = {0, 0, 0, 0, 0, 0};
   double xi = xyz[i], yi = xyz[nnode + i], zi = xyz[2 * nnode + i];
-  for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-    int j = col[k];
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int k = 0; k < w; ++k) {
+    const int j = g.col[b + (long long)k * kW];
+    if (j < 0) break;
     double xj = xyz[j], yj = xyz[nnode + j], zj = xyz[2 * nnode + j];
     double dx0, dx1, dx2;
     if (i < j) { dx0 = xi - xj; dx1 = yi - yj; dx2 = zi - zj; }
@@ -42,8 +60,8 @@ __global__ void __launch_bounds__(kBlock) k_lsq_moments(int nnode, const int* __
 }
 
 void launch_lsq_moments(Ctx& c) {
-  hipLaunchKernelGGL(k_lsq_moments, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.d.row_ptr,
-                     c.d.col, c.d.xyz, c.d.sw);
+  hipLaunchKernelGGL(k_lsq_moments, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, gell(c), c.d.xyz,
+                     c.d.sw);
   PCFD_HIP(hipGetLastError());
 }
 
@@ -52,11 +70,14 @@ void launch_lsq_moments(Ctx& c) {
 // half-edges in half-edge order, so a Dirichlet/sonic-inflow overwrite of the
 // node state is seen by the node's later half-edges exactly as in the
 // reference's sequential loop.  Each phantom is written by its owner only.
-__global__ void __launch_bounds__(kBlock) k_update_bcs(int nbnd, int nnode, int ntot, int ntag, const int* __restrict__ bnd_nodes,
-                             const int* __restrict__ he_ptr, const int* __restrict__ he_id,
-                             const int* __restrict__ he_tag, const double* __restrict__ he_a, int nbnode,
-                             const int* __restrict__ bc_type, const double* __restrict__ qref, double gamma,
-                             double* __restrict__ q) {
+__global__ void __launch_bounds__(kBlock) k_update_bcs(int nbnd, int nnode, int ntot, int ntag,
+                                                      const int* __restrict__ bnd_nodes,
+                                                      const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                      const int* __restrict__ he_tag,
+                                                      const double* __restrict__ he_a, int nbnode,
+                                                      const int* __restrict__ bc_type,
+                                                      const double* __restrict__ qref, double gamma,
+                                                      double* __restrict__ q) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
   int i = bnd_nodes[t];
@@ -108,39 +129,55 @@ template <int NT>
 __device__ __forceinline__ void grad_terms(const double* Q, double gm1, double gamma, double* t) {
   t[0] = Q[0]; t[1] = Q[1]; t[2] = Q[2]; t[3] = Q[3]; t[4] = Q[4];
   if (NT > 5) {
-    t[5] = gamma * comp_pressure(Q, gm1) / Q[0];
-    t[6] = Q[1] / Q[0];
-    t[7] = Q[2] / Q[0];
-    t[8] = Q[3] / Q[0];
+    const double ir = 1.0 / Q[0];
+    const double u = Q[1] * ir, v = Q[2] * ir, w = Q[3] * ir;
+    t[5] = gamma * (gm1 * (Q[4] - Q[0] * (0.5 * (u * u + v * v + w * w)))) * ir;
+    t[6] = u;
+    t[7] = v;
+    t[8] = w;
   }
 }
 
 // Gradient::Compute, weighted LSQ (ucs/gradient.tcc:56-112, 250-316) and the
-// symmetry-plane projection Bkernel_Symmetry_Fix (:544-565).
+// symmetry-plane projection Bkernel_Symmetry_Fix (:544-565).  The per-node
+// Gram-Schmidt factors of ComputeLSQCoefficients (:140-168) depend only on
+// the node's moments and are formed once per node.
 template <int NT>
-__global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, const int* __restrict__ row_ptr, const int* __restrict__ col,
-                           const double* __restrict__ xyz, const double* __restrict__ sw,
-                           const double* __restrict__ q, const int* __restrict__ he_ptr,
-                           const int* __restrict__ he_id, const int* __restrict__ he_tag,
-                           const double* __restrict__ he_a, int nbnode, const int* __restrict__ bc_type,
-                           double gamma, double* __restrict__ grad) {
+__global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g, const double* __restrict__ xyz,
+                                                    const double* __restrict__ sw, const double* __restrict__ q,
+                                                    const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                    const int* __restrict__ he_tag,
+                                                    const double* __restrict__ he_a, int nbnode,
+                                                    const int* __restrict__ bc_type, double gamma,
+                                                    double* __restrict__ grad) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   const double gm1 = gamma - 1.0;
-  double g[NT * 3];
+  double gr[NT * 3];
 #pragma unroll
-  for (int k = 0; k < NT * 3; ++k) g[k] = 0.0;
-  double s[6];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) s[k] = sw[k * nnode + i];
+  for (int k = 0; k < NT * 3; ++k) gr[k] = 0.0;
+  // ComputeLSQCoefficients factors
+  const double r11 = sw[i], r12 = sw[nnode + i], r13 = sw[2 * nnode + i];
+  const double r12_r11 = (r11 == 0.0) ? 0.0 : r12 / r11;
+  const double r22 = sw[3 * nnode + i] - r12 * r12_r11;
+  const double r23 = sw[4 * nnode + i] - r12_r11 * r13;
+  const double r13_r11 = (r11 == 0.0) ? 0.0 : r13 / r11;
+  const double r23_r22 = (r22 == 0.0) ? 0.0 : r23 / r22;
+  const double r33 = sw[5 * nnode + i] - r13 * r13_r11 - r23 * r23_r22;
+  const double i11 = (r11 == 0.0) ? 0.0 : 1.0 / r11;
+  const double i22 = (r22 == 0.0) ? 0.0 : 1.0 / r22;
+  const double i33 = (r33 == 0.0) ? 0.0 : 1.0 / r33;
   double Qi[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) Qi[k] = q[k * ntot + i];
   double ti[NT];
   grad_terms<NT>(Qi, gm1, gamma, ti);
   double xi = xyz[i], yi = xyz[nnode + i], zi = xyz[2 * nnode + i];
-  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-    int j = col[p];
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int s = 0; s < w; ++s) {
+    const int j = g.col[b + (long long)s * kW];
+    if (j < 0) break;
     double Qj[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Qj[k] = q[k * ntot + j];
@@ -155,15 +192,18 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, const 
     double weight = 1.0 / sqrt(dx2);
     d[0] *= weight; d[1] *= weight; d[2] *= weight;
     if (!left) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
-    double we[3];
-    lsq_coeff(s, d, we);
+    const double dykdx = d[1] - r12_r11 * d[0];
+    const double we2 = (d[2] - r13_r11 * d[0] - r23_r22 * dykdx) * i33;
+    const double we1 = (dykdx - r23 * we2) * i22;
+    const double we0 = (d[0] - r12 * we1 - r13 * we2) * i11;
+    const double we[3] = {we0, we1, we2};
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       double dq = left ? weight * (tj[t] - ti[t]) : weight * (ti[t] - tj[t]);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        if (left) g[3 * t + k] += -we[k] * dq;
-        else      g[3 * t + k] += +we[k] * dq;
+        if (left) gr[3 * t + k] += -we[k] * dq;
+        else      gr[3 * t + k] += +we[k] * dq;
       }
     }
   }
@@ -173,25 +213,25 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, const 
     double a0 = he_a[h], a1 = he_a[nbnode + h], a2 = he_a[2 * nbnode + h];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      double dt = g[3 * t] * a0 + g[3 * t + 1] * a1 + g[3 * t + 2] * a2;
-      g[3 * t] -= dt * a0;
-      g[3 * t + 1] -= dt * a1;
-      g[3 * t + 2] -= dt * a2;
+      double dt = gr[3 * t] * a0 + gr[3 * t + 1] * a1 + gr[3 * t + 2] * a2;
+      gr[3 * t] -= dt * a0;
+      gr[3 * t + 1] -= dt * a1;
+      gr[3 * t + 2] -= dt * a2;
     }
   }
 #pragma unroll
-  for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = g[k];
+  for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = gr[k];
 }
 
 void launch_gradient(Ctx& c, int nterms) {
   if (nterms == 9) {
-    hipLaunchKernelGGL(k_gradient<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.d.row_ptr, c.d.col, c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag,
-                       c.d.he_a, c.nbnode, c.d.bc_type, c.prm.gamma, c.d.grad);
+    hipLaunchKernelGGL(k_gradient<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
+                       c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
+                       c.prm.gamma, c.d.grad);
   } else {
-    hipLaunchKernelGGL(k_gradient<5>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.d.row_ptr, c.d.col, c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag,
-                       c.d.he_a, c.nbnode, c.d.bc_type, c.prm.gamma, c.d.grad);
+    hipLaunchKernelGGL(k_gradient<5>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
+                       c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
+                       c.prm.gamma, c.d.grad);
   }
   PCFD_HIP(hipGetLastError());
 }
@@ -200,11 +240,10 @@ void launch_gradient(Ctx& c, int nterms) {
 // Limiter::Compute (ucs/limiters.tcc:52-132): neighbour min/max with both
 // bounds starting at 0 (:59-63), then the per-edge-end limiter value, the
 // minimum over the node's edges.  Writes the pre-clip limiter to lim2.
-__global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int type, double K3, double chi,
-                          const int* __restrict__ row_ptr, const int* __restrict__ col,
-                          const double* __restrict__ xyz, const double* __restrict__ vol,
-                          const double* __restrict__ q, const double* __restrict__ grad,
-                          double* __restrict__ lim2) {
+__global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int type, double K3, double chi, Gell g,
+                                                   const double* __restrict__ xyz, const double* __restrict__ vol,
+                                                   const double* __restrict__ q, const double* __restrict__ grad,
+                                                   double* __restrict__ lim2) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double qi[5], qmin[5], qmax[5], phi[5], gi[15];
@@ -217,9 +256,11 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
-  const int p0 = row_ptr[i], p1 = row_ptr[i + 1];
-  for (int p = p0; p < p1; ++p) {
-    int j = col[p];
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int s = 0; s < w; ++s) {
+    const int j = g.col[b + (long long)s * kW];
+    if (j < 0) break;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       double v = q[k * ntot + j];
@@ -229,8 +270,9 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
   }
   double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
   const double ep2 = 6.0 * 3.141592653589793 * vol[i] * K3;
-  for (int p = p0; p < p1; ++p) {
-    int j = col[p];
+  for (int s = 0; s < w; ++s) {
+    const int j = g.col[b + (long long)s * kW];
+    if (j < 0) break;
     const bool left = i < j;
     double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
     double dx[3];
@@ -267,14 +309,23 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
   for (int k = 0; k < 5; ++k) lim2[k * nnode + i] = phi[k];
 }
 
+// BadExtrapolation (ucs/compressible.tcc:1054-1078) with one reciprocal
+__device__ __forceinline__ bool bad_state_r(const double* Q, double gamma) {
+  const double r = Q[0], ir = 1.0 / r;
+  const double u = Q[1] * ir, v = Q[2] * ir, w = Q[3] * ir, E = Q[4];
+  const double p = (gamma - 1.0) * (E - r * (0.5 * (u * u + v * v + w * w)));
+  return (p < 1.0e-10) || (r < 0.0) || (E < 1.0e-10);
+}
+
 // Kernel_PressureClip (ucs/limiters.tcc:736-815) with every edge seeing the
 // pre-clip limiter (the reference's sequential edge loop lets an earlier
 // edge's zeroing change a later edge's extrapolation; the two agree whenever
 // no clip chain forms), then negative limiters -> 0 (:118-124).
-__global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, double chi, double gamma, const int* __restrict__ row_ptr,
-                                const int* __restrict__ col, const double* __restrict__ xyz,
-                                const double* __restrict__ q, const double* __restrict__ grad,
-                                const double* __restrict__ lim2, double* __restrict__ lim) {
+__global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, double chi, double gamma, Gell g,
+                                                         const double* __restrict__ xyz,
+                                                         const double* __restrict__ q,
+                                                         const double* __restrict__ grad,
+                                                         const double* __restrict__ lim2, double* __restrict__ lim) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   bool zero = false;
@@ -285,8 +336,11 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
     li[k] = lim2[k * nnode + i];
   }
   double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
-  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-    int j = col[p];
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int s = 0; s < w; ++s) {
+    const int j = g.col[b + (long long)s * kW];
+    if (j < 0) break;
     const bool left = i < j;
     double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
     const int L = left ? i : j, R = left ? j : i;
@@ -304,11 +358,11 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
       QR[k] = extrap1(qR, -dQ, grad[(3 * k) * nnode + R], grad[(3 * k + 1) * nnode + R],
                       grad[(3 * k + 2) * nnode + R], mdx, chi, lim2[k * nnode + R]);
     }
-    if (left && bad_state(QL, gamma)) zero = true;
-    if (!left && bad_state(QR, gamma)) zero = true;
+    if (left && bad_state_r(QL, gamma)) zero = true;
+    if (!left && bad_state_r(QR, gamma)) zero = true;
     double Qr[5];
     roe_state(QL, QR, gamma, Qr);
-    if (bad_state(Qr, gamma)) zero = true;
+    if (bad_state_r(Qr, gamma)) zero = true;
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
@@ -322,10 +376,10 @@ void launch_limiter(Ctx& c) {
   const pcfd_params& p = c.prm;
   double K3 = p.venkat_K * p.venkat_K * p.venkat_K;
   hipLaunchKernelGGL(k_limiter, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.limiter, K3,
-                     p.chi, c.d.row_ptr, c.d.col, c.d.xyz, c.d.vol, c.d.q, c.d.grad, c.d.lim2);
+                     p.chi, gell(c), c.d.xyz, c.d.vol, c.d.q, c.d.grad, c.d.lim2);
   PCFD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
-                     p.gamma, c.d.row_ptr, c.d.col, c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
+                     p.gamma, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
   PCFD_HIP(hipGetLastError());
 }
 
@@ -335,9 +389,7 @@ void launch_limiter(Ctx& c) {
 // (:298-387, b[L] -= F on the unextrapolated state vs the phantom).
 // Also writes per-block partial sums of b^2 (total + per equation).
 template <bool HO>
-__global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double gamma, double chi, int sym2d,
-                                                    const int* __restrict__ row_ptr, const int* __restrict__ col,
-                                                    const double* __restrict__ nrm, int nnz,
+__global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double gamma, double chi, int sym2d, Gell g,
                                                     const double* __restrict__ xyz, const double* __restrict__ q,
                                                     const double* __restrict__ grad, const double* __restrict__ lim,
                                                     const int* __restrict__ he_ptr, const int* __restrict__ he_id,
@@ -356,36 +408,38 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
       for (int k = 0; k < 5; ++k) li[k] = lim[k * nnode + i];
       xi[0] = xyz[i]; xi[1] = xyz[nnode + i]; xi[2] = xyz[2 * nnode + i];
     }
-    for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-      int j = col[p];
+    const long long base = gbase_of(g, i);
+    const int w = g.width[i >> 6];
+    for (int s = 0; s < w; ++s) {
+      const long long idx = base + (long long)s * kW;
+      const int j = g.col[idx];
+      if (j < 0) break;
       const bool left = i < j;
-      double a[4] = {nrm[p], nrm[nnz + p], nrm[2 * nnz + p], nrm[3 * nnz + p]};
+      double a[4] = {g.nrm[idx], g.nrm[g.stride + idx], g.nrm[2 * g.stride + idx], g.nrm[3 * g.stride + idx]};
       double qj[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) qj[k] = q[k * ntot + j];
       double QL[5], QR[5];
       if (HO) {
-        double gj[15], lj[5];
-#pragma unroll
-        for (int k = 0; k < 15; ++k) gj[k] = grad[k * nnode + j];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) lj[k] = lim[k * nnode + j];
         double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
         const double* xL = left ? xi : xj;
         const double* xR = left ? xj : xi;
-        const double* qL = left ? qi : qj;
-        const double* qR = left ? qj : qi;
-        const double* gL = left ? gi : gj;
-        const double* gR = left ? gj : gi;
-        const double* lL = left ? li : lj;
-        const double* lR = left ? lj : li;
         double dx[3] = {0.5 * (xR[0] - xL[0]), 0.5 * (xR[1] - xL[1]), 0.5 * (xR[2] - xL[2])};
-        double mdx[3] = {-dx[0], -dx[1], -dx[2]};
+        // own end: extrapolate toward the edge midpoint; neighbour end: same
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-          double dQ = qR[k] - qL[k];
-          QL[k] = extrap1(qL[k], dQ, gL[3 * k], gL[3 * k + 1], gL[3 * k + 2], dx, chi, lL[k]);
-          QR[k] = extrap1(qR[k], -dQ, gR[3 * k], gR[3 * k + 1], gR[3 * k + 2], mdx, chi, lR[k]);
+          const double qL = left ? qi[k] : qj[k];
+          const double qR = left ? qj[k] : qi[k];
+          const double dQ = qR - qL;
+          const double gj0 = grad[(3 * k) * nnode + j], gj1 = grad[(3 * k + 1) * nnode + j],
+                       gj2 = grad[(3 * k + 2) * nnode + j];
+          const double lj = lim[k * nnode + j];
+          const double gL0 = left ? gi[3 * k] : gj0, gL1 = left ? gi[3 * k + 1] : gj1, gL2 = left ? gi[3 * k + 2] : gj2;
+          const double gR0 = left ? gj0 : gi[3 * k], gR1 = left ? gj1 : gi[3 * k + 1], gR2 = left ? gj2 : gi[3 * k + 2];
+          const double lL = left ? li[k] : lj, lR = left ? lj : li[k];
+          double mdx[3] = {-dx[0], -dx[1], -dx[2]};
+          QL[k] = extrap1(qL, dQ, gL0, gL1, gL2, dx, chi, lL);
+          QR[k] = extrap1(qR, -dQ, gR0, gR1, gR2, mdx, chi, lR);
         }
       } else {
 #pragma unroll
@@ -395,7 +449,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
         }
       }
       double F[5];
-      roe_flux(QL, QR, a, gamma, sym2d, F);
+      roe_flux_fast(QL, QR, a, gamma, sym2d, F);
 #pragma unroll
       for (int k = 0; k < 5; ++k) bi[k] += left ? -F[k] : F[k];
     }
@@ -406,7 +460,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
 #pragma unroll
       for (int k = 0; k < 5; ++k) QR[k] = q[k * ntot + nnode + h];
       double F[5];
-      roe_flux(qi, QR, a, gamma, sym2d, F);
+      roe_flux_fast(qi, QR, a, gamma, sym2d, F);
 #pragma unroll
       for (int k = 0; k < 5; ++k) bi[k] += -F[k];
     }
@@ -438,7 +492,8 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
 }
 
 // final reduction of the per-block partial sums (single block, fixed order)
-__global__ void __launch_bounds__(kBlock) k_reduce6(int nparts, const double* __restrict__ red, double* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_reduce6(int nparts, const double* __restrict__ red,
+                                                   double* __restrict__ out) {
   __shared__ double sh[kBlock];
   for (int k = 0; k < 6; ++k) {
     double s = 0.0;
@@ -459,12 +514,12 @@ void launch_residual(Ctx& c, int ho) {
   int nb = nblk(c.nnode);
   if (ho) {
     hipLaunchKernelGGL(k_residual<true>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
-                       p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.xyz, c.d.q, c.d.grad, c.d.lim,
-                       c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.b, c.d.red);
+                       p.symmetry2d, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a,
+                       c.nbnode, c.d.b, c.d.red);
   } else {
     hipLaunchKernelGGL(k_residual<false>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
-                       p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.xyz, c.d.q, c.d.grad, c.d.lim,
-                       c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.b, c.d.red);
+                       p.symmetry2d, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a,
+                       c.nbnode, c.d.b, c.d.red);
   }
   PCFD_HIP(hipGetLastError());
   c.nred = nb;
@@ -485,25 +540,39 @@ void launch_norms(Ctx& c, double* host6) {
 }
 
 // ---------------------------------------------------------------------------
+// MaxEigenvalue (ucs/compressible.tcc:798-821) with one reciprocal
+__device__ __forceinline__ double max_eig_r(const double* Q, const double* a, double gamma) {
+  const double rho = Q[0], ir = 1.0 / rho;
+  const double u = Q[1] * ir, v = Q[2] * ir, w = Q[3] * ir;
+  const double P = (gamma - 1.0) * (Q[4] - rho * (0.5 * (u * u + v * v + w * w)));
+  const double c = sqrt(gamma * P * ir);
+  const double th = u * a[0] + v * a[1] + w * a[2];
+  return dmax(fabs(th + c), fabs(th - c));
+}
+
 // ComputeTimesteps, local time stepping (ucs/timestep.tcc:6-143)
-__global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double gamma, double cfl, const int* __restrict__ row_ptr,
-                           const int* __restrict__ col, const double* __restrict__ nrm, int nnz,
-                           const double* __restrict__ q, const double* __restrict__ vol,
-                           const int* __restrict__ he_ptr, const int* __restrict__ he_id,
-                           const double* __restrict__ he_a, int nbnode, double* __restrict__ dt) {
+__global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double gamma, double cfl, Gell g,
+                                                    const double* __restrict__ q, const double* __restrict__ vol,
+                                                    const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                    const double* __restrict__ he_a, int nbnode,
+                                                    double* __restrict__ dt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double qi[5];
 #pragma unroll
   for (int k = 0; k < 5; ++k) qi[k] = q[k * ntot + i];
   double s = 0.0;
-  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-    int j = col[p];
-    double a[4] = {nrm[p], nrm[nnz + p], nrm[2 * nnz + p], nrm[3 * nnz + p]};
+  const long long base = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int p = 0; p < w; ++p) {
+    const long long idx = base + (long long)p * kW;
+    const int j = g.col[idx];
+    if (j < 0) break;
+    double a[4] = {g.nrm[idx], g.nrm[g.stride + idx], g.nrm[2 * g.stride + idx], g.nrm[3 * g.stride + idx]};
     double t[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) t[k] = i < j ? 0.5 * (qi[k] + q[k * ntot + j]) : 0.5 * (q[k * ntot + j] + qi[k]);
-    s += max_eig(t, a, gamma) * a[3];
+    for (int k = 0; k < 5; ++k) t[k] = 0.5 * (qi[k] + q[k * ntot + j]);
+    s += max_eig_r(t, a, gamma) * a[3];
   }
   for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
     int h = he_id[p];
@@ -511,7 +580,7 @@ __global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double
     double t[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) t[k] = 0.5 * (qi[k] + q[k * ntot + nnode + h]);
-    s += max_eig(t, a, gamma) * a[3];
+    s += max_eig_r(t, a, gamma) * a[3];
   }
   dt[i] = cfl * (vol[i] / s);
 }
@@ -530,9 +599,8 @@ __global__ void __launch_bounds__(kBlock) k_min_partial(int n, const double* __r
 }
 
 void launch_timestep(Ctx& c, double cfl) {
-  hipLaunchKernelGGL(k_timestep, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma,
-                     cfl, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.q, c.d.vol, c.d.he_ptr, c.d.he_id, c.d.he_a,
-                     c.nbnode, c.d.dt);
+  hipLaunchKernelGGL(k_timestep, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma, cfl,
+                     gell(c), c.d.q, c.d.vol, c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.dt);
   PCFD_HIP(hipGetLastError());
 }
 

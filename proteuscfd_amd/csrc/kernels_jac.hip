@@ -7,11 +7,12 @@
 // off-diagonal blocks onto the diagonal (Kernel_Diag_NumJac :436-456), and
 // ContributeTemporalTerms adds vol/dt (:213-250).
 //
-// Here one thread owns one block row: it writes the row's off-diagonal
-// blocks and accumulates its own diagonal (boundary part first, then edges
-// in ascending neighbour order, then the temporal term), so every block has
-// a single writer.  Storage type T is double (plain) or float (chosen form);
-// the finite differences are always evaluated in FP64.
+// Here one thread owns one block row of the SELL matrix (threads run in
+// SELL row order, so 64 lanes write 64 consecutive addresses): it writes
+// the row's off-diagonal blocks and accumulates its own diagonal (boundary
+// part first, then edges in ascending neighbour order, then the temporal
+// term), so every block has a single writer.  Storage type T is double
+// (plain) or float (chosen form); the differences are always taken in FP64.
 #include <stdexcept>
 
 #include "common.h"
@@ -19,133 +20,191 @@
 
 namespace pcfd {
 
-template <typename T>
-__global__ void __launch_bounds__(kBlock) k_jacobian(int nnode, int ntot, int ntag, double gamma, int sym2d,
-                                                    const int* __restrict__ row_ptr, const int* __restrict__ col,
-                                                    const double* __restrict__ nrm, int nnz,
-                                                    const int* __restrict__ he_ptr, const int* __restrict__ he_id,
-                                                    const int* __restrict__ he_tag, const double* __restrict__ he_a,
-                                                    int nbnode, const int* __restrict__ bc_type,
-                                                    const double* __restrict__ qref, const double* __restrict__ vol,
-                                                    const double* __restrict__ dt, double* __restrict__ q,
-                                                    const int* __restrict__ blk_ptr, T* __restrict__ M) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnode) return;
-  const double h = 1.0e-8;
-  double D[25];
-#pragma unroll
-  for (int k = 0; k < 25; ++k) D[k] = 0.0;
-  double qi[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) qi[k] = q[k * ntot + i];
+struct JacArgs {
+  int nnode, ntot, ntag, nbnode, nrow_pad, sym2d;
+  double gamma;
+  const int *gwidth, *gbase, *gcol;
+  const double* gnrm;
+  long long gstride;
+  const int *he_ptr, *he_id, *he_tag;
+  const double* he_a;
+  const int* bc_type;
+  const double* qref;
+  const double *vol, *dt;
+  double* q;
+  const int *srow, *sbase;
+};
 
-  // --- boundary half-edges (Bkernel_NumJac), sequential per node ---
+// Five lanes per block row: lane c of a row's group owns perturbation
+// column c -- it writes column c of every block of the row and accumulates
+// column c of the diagonal.  Each block entry is produced by exactly the
+// same finite difference as a one-thread-per-row evaluation, with a fifth
+// of the live state per lane.  A wavefront holds 12 rows (lanes 60-63 idle).
+__device__ __forceinline__ void perturb(const double* Q, int c, double h, double* QP) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) QP[k] = Q[k] + (k == c ? h : 0.0);
+}
+
+// Boundary half-edges (Bkernel_NumJac), sequential per boundary node, five
+// lanes per node as in k_jacobian.  Writes the node's boundary part of the
+// diagonal block to Db[t*25 + r*5 + c] (t = boundary-node index) and the
+// re-evaluated phantom states (lane 0).
+__global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd,
+                                                        double* __restrict__ Db) {
+  const int lane64 = threadIdx.x & 63;
+  if (lane64 >= 60) return;
+  const int t = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 12 + lane64 / 5;
+  const int c = lane64 % 5;
+  if (t >= nbnd) return;
+  const int i = bnd_nodes[t];
+  const double h = 1.0e-8;
+  const double gamma = A.gamma;
+  double Dc[5] = {0, 0, 0, 0, 0};
   {
-    double QL[5] = {qi[0], qi[1], qi[2], qi[3], qi[4]};
-    for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
-      int hh = he_id[p];
-      int tag = he_tag[hh];
-      int type = bc_type[tag];
-      int ph = nnode + hh;
-      double a[4] = {he_a[hh], he_a[nbnode + hh], he_a[2 * nbnode + hh], he_a[3 * nbnode + hh]};
+    double QL[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) QL[k] = A.q[k * A.ntot + i];
+    for (int p = A.he_ptr[i]; p < A.he_ptr[i + 1]; ++p) {
+      const int hh = A.he_id[p];
+      const int tag = A.he_tag[hh];
+      const int type = A.bc_type[tag];
+      const int ph = A.nnode + hh;
+      const double a[4] = {A.he_a[hh], A.he_a[A.nbnode + hh], A.he_a[2 * A.nbnode + hh],
+                           A.he_a[3 * A.nbnode + hh]};
       double QR[5], Qr[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        QR[k] = q[k * ntot + ph];
-        Qr[k] = qref[k * ntag + tag];
+        QR[k] = A.q[k * A.ntot + ph];
+        Qr[k] = A.qref[k * A.ntag + tag];
       }
-      double pref = qref[5 * ntag + tag];
+      const double pref = A.qref[5 * A.ntag + tag];
       if (is_dirichlet(type)) {
-        // QL = QR = Qref, and every perturbed evaluation resets both to
-        // Qref again, so the reference's contribution is exactly zero
+        // QL = QR = Qref and every perturbed evaluation resets both to Qref,
+        // so the reference's contribution is exactly zero
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           QL[k] = Qr[k];
-          q[k * ntot + ph] = Qr[k];
+          if (c == 0) A.q[k * A.ntot + ph] = Qr[k];
         }
         continue;
       }
-      bool handled = bc_state(type, QL, QR, Qr, pref, a, gamma);
-      if (handled) {
+      if (bc_state(type, QL, QR, Qr, pref, a, gamma) && c == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) q[k * ntot + ph] = QR[k];
+        for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
       }
       double fS[5];
-      roe_flux(QL, QR, a, gamma, sym2d, fS);
+      roe_flux_fast(QL, QR, a, gamma, A.sym2d, fS);
+      double QPL[5];
+      perturb(QL, c, h, QPL);
+      double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
+      bc_state(type, QPL, QPR, Qr, pref, a, gamma);
+      double fL[5];
+      roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
-      for (int c = 0; c < 5; ++c) {
-        double QPL[5] = {QL[0], QL[1], QL[2], QL[3], QL[4]};
-        QPL[c] += h;
-        double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
-        bc_state(type, QPL, QPR, Qr, pref, a, gamma);
-        double fL[5];
-        roe_flux(QPL, QPR, a, gamma, sym2d, fL);
-#pragma unroll
-        for (int r = 0; r < 5; ++r) D[r * 5 + c] += (fL[r] - fS[r]) / h;
-      }
+      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) / h;
     }
   }
-  // --- interior edges (Kernel_NumJac), row i blocks + diagonal column sums ---
-  const int b0 = blk_ptr[i];
-  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-    int j = col[p];
+#pragma unroll
+  for (int rr = 0; rr < 5; ++rr) Db[(long long)t * 25 + rr * 5 + c] = Dc[rr];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __restrict__ bidx,
+                                                    const double* __restrict__ Db, T* __restrict__ Moff,
+                                                    T* __restrict__ Dg) {
+  const int lane64 = threadIdx.x & 63;
+  if (lane64 >= 60) return;
+  const int r = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 12 + lane64 / 5;
+  const int c = lane64 % 5;
+  if (r >= A.nrow_pad) return;
+  const int i = A.srow[r];
+  if (i < 0) return;
+  const int chunk = r >> 6, lane = r & 63;
+  const double h = 1.0e-8;
+  const double gamma = A.gamma;
+  double Dc[5] = {0, 0, 0, 0, 0};  // column c of the diagonal block
+  const int bt = bidx[i];
+  if (bt >= 0) {
+    // boundary contributions come first, as Bdriver runs before Kernel_Diag_NumJac
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) Dc[rr] = Db[(long long)bt * 25 + rr * 5 + c];
+  }
+  double qi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) qi[k] = A.q[k * A.ntot + i];
+  // --- interior edges (Kernel_NumJac): row blocks + diagonal column sums ---
+  const long long gb = (long long)A.gbase[i >> 6] * kW + (i & 63);
+  const int w = A.gwidth[i >> 6];
+  T* ob = Moff + (long long)A.sbase[chunk] * 25 * kW + lane + (long long)c * kW;
+  for (int s = 0; s < w; ++s) {
+    const long long gi = gb + (long long)s * kW;
+    const int j = A.gcol[gi];
+    if (j < 0) break;
     const bool left = i < j;
-    double a[4] = {nrm[p], nrm[nnz + p], nrm[2 * nnz + p], nrm[3 * nnz + p]};
+    const double a[4] = {A.gnrm[gi], A.gnrm[A.gstride + gi], A.gnrm[2 * A.gstride + gi], A.gnrm[3 * A.gstride + gi]};
     double qj[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) qj[k] = q[k * ntot + j];
+    for (int k = 0; k < 5; ++k) qj[k] = A.q[k * A.ntot + j];
     const double* QL = left ? qi : qj;
     const double* QR = left ? qj : qi;
-    double fS[5];
-    roe_flux(QL, QR, a, gamma, sym2d, fS);
-    T* blk = M + (size_t)(b0 + 1 + (p - row_ptr[i])) * 25;
+    RoeSide SL, SR, SP;
+    roe_side(QL, a, gamma, SL);
+    roe_side(QR, a, gamma, SR);
+    double fS[5], fP[5], QP[5];
+    roe_flux_s(QL, SL, QR, SR, a, gamma, A.sym2d, fS);
+    T* blk = ob + (long long)s * 25 * kW;
+    // the three evaluations are retired one after another (scheduler
+    // fences keep them from being interleaved, which would triple the
+    // live registers)
+    __builtin_amdgcn_sched_barrier(0);
+    perturb(QL, c, h, QP);
+    roe_side(QP, a, gamma, SP);
+    roe_flux_s(QP, SP, QR, SR, a, gamma, A.sym2d, fP);
 #pragma unroll
-    for (int c = 0; c < 5; ++c) {
-      double QPL[5] = {QL[0], QL[1], QL[2], QL[3], QL[4]};
-      double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
-      QPL[c] += h;
-      QPR[c] += h;
-      double fL[5], fR[5];
-      roe_flux(QPL, QR, a, gamma, sym2d, fL);
-      roe_flux(QL, QPR, a, gamma, sym2d, fR);
-#pragma unroll
-      for (int r = 0; r < 5; ++r) {
-        double aRL = (fS[r] - fL[r]) / h;  // A[R][L] = -dF/dQL
-        double aLR = (fR[r] - fS[r]) / h;  // A[L][R] = +dF/dQR
-        if (left) {
-          blk[r * 5 + c] = (T)aLR;
-          D[r * 5 + c] += -aRL;
-        } else {
-          blk[r * 5 + c] = (T)aRL;
-          D[r * 5 + c] += -aLR;
-        }
-      }
+    for (int rr = 0; rr < 5; ++rr) {
+      const double aRL = (fS[rr] - fP[rr]) / h;  // A[R][L] = -dF/dQL
+      if (left) Dc[rr] += -aRL;
+      else blk[rr * 5 * kW] = (T)aRL;
     }
+    __builtin_amdgcn_sched_barrier(0);
+    perturb(QR, c, h, QP);
+    roe_side(QP, a, gamma, SP);
+    roe_flux_s(QL, SL, QP, SP, a, gamma, A.sym2d, fP);
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) {
+      const double aLR = (fP[rr] - fS[rr]) / h;  // A[L][R] = +dF/dQR
+      if (left) blk[rr * 5 * kW] = (T)aLR;
+      else Dc[rr] += -aLR;
+    }
+    __builtin_amdgcn_sched_barrier(0);
   }
   // --- temporal term, steady pseudo-time: A_ii += 1*vol/dtau ---
-  const double vt = 1.0 * vol[i] / dt[i];
+  const double vt = 1.0 * A.vol[i] / A.dt[i];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) D[k * 5 + k] += vt;
-  T* d = M + (size_t)b0 * 25;
+  for (int rr = 0; rr < 5; ++rr)
+    if (rr == c) Dc[rr] += vt;
+  T* dg = Dg + (long long)chunk * 25 * kW + lane + (long long)c * kW;
 #pragma unroll
-  for (int k = 0; k < 25; ++k) d[k] = (T)D[k];
+  for (int rr = 0; rr < 5; ++rr) dg[rr * 5 * kW] = (T)Dc[rr];
 }
 
 // Dirichlet / sonic-inflow overwrite of boundary node states, deferred until
 // all rows have read the pre-update states (the reference applies it inside
 // Bkernel_NumJac after Kernel_NumJac has run).
-__global__ void __launch_bounds__(kBlock) k_commit_dirichlet(int nbnd, int ntot, int ntag, const int* __restrict__ bnd_nodes,
-                                   const int* __restrict__ he_ptr, const int* __restrict__ he_id,
-                                   const int* __restrict__ he_tag, const int* __restrict__ bc_type,
-                                   const double* __restrict__ qref, double* __restrict__ q) {
+__global__ void __launch_bounds__(kBlock) k_commit_dirichlet(int nbnd, int ntot, int ntag,
+                                                            const int* __restrict__ bnd_nodes,
+                                                            const int* __restrict__ he_ptr,
+                                                            const int* __restrict__ he_id,
+                                                            const int* __restrict__ he_tag,
+                                                            const int* __restrict__ bc_type,
+                                                            const double* __restrict__ qref, double* __restrict__ q) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
   int i = bnd_nodes[t];
   int last = -1;
   for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
     int tag = he_tag[he_id[p]];
-    int ty = bc_type[tag];
-    if (ty == BC_SONIC_INFLOW || ty == BC_DIRICHLET) last = tag;
+    if (is_dirichlet(bc_type[tag])) last = tag;
   }
   if (last >= 0) {
 #pragma unroll
@@ -155,17 +214,23 @@ __global__ void __launch_bounds__(kBlock) k_commit_dirichlet(int nbnd, int ntot,
 
 void launch_jacobian(Ctx& c) {
   const pcfd_params& p = c.prm;
-  if (p.precision == PCFD_CHOSEN_FP32) {
-    hipLaunchKernelGGL(k_jacobian<float>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.ntag,
-                       p.gamma, p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.he_ptr, c.d.he_id,
-                       c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type, c.d.qref, c.d.vol, c.d.dt, c.d.q, c.d.blk_ptr,
-                       c.d.Mf);
-  } else {
-    hipLaunchKernelGGL(k_jacobian<double>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.ntag, p.gamma, p.symmetry2d, c.d.row_ptr, c.d.col, c.d.nrm, c.nnz, c.d.he_ptr, c.d.he_id,
-                       c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type, c.d.qref, c.d.vol, c.d.dt, c.d.q, c.d.blk_ptr,
-                       c.d.Md);
+  JacArgs A{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase, c.d.gcol,
+            c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type, c.d.qref, c.d.vol,
+            c.d.dt, c.d.q, c.d.srow, c.d.sbase};
+  if (c.nbnd > 0) {
+    const long long bw = (c.nbnd + 11) / 12;
+    hipLaunchKernelGGL(k_jac_boundary, dim3(nblk(bw * kW)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd,
+                       c.d.jdb);
+    PCFD_HIP(hipGetLastError());
   }
+  const long long waves = (c.nrow_pad + 11) / 12;  // 12 rows x 5 lanes per wavefront
+  const int grid = nblk(waves * kW);
+  if (p.precision == PCFD_CHOSEN_FP32)
+    hipLaunchKernelGGL(k_jacobian<float>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.Mf,
+                       c.d.Df);
+  else
+    hipLaunchKernelGGL(k_jacobian<double>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.Md,
+                       c.d.Dd);
   PCFD_HIP(hipGetLastError());
   if (c.nbnd > 0) {
     hipLaunchKernelGGL(k_commit_dirichlet, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.ntot, c.ntag,
@@ -181,13 +246,14 @@ void launch_jacobian(Ctx& c) {
 // indices, no scratch), which performs the same arithmetic.  Row r of the
 // stored factor is original row perm[r]; perm is packed 3 bits per entry.
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_prepare_sgs(int nnode, const int* __restrict__ blk_ptr, T* __restrict__ M, int* __restrict__ piv) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nnode) return;
-  T* a = M + (size_t)blk_ptr[i] * 25;
+__global__ void __launch_bounds__(kBlock) k_prepare_sgs(int nrow_pad, const int* __restrict__ srow, T* __restrict__ Dg,
+                                                       int* __restrict__ piv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow_pad || srow[r] < 0) return;
+  T* a = Dg + (long long)(r >> 6) * 25 * kW + (r & 63);
   T A[25];
 #pragma unroll
-  for (int k = 0; k < 25; ++k) A[k] = a[k];
+  for (int k = 0; k < 25; ++k) A[k] = a[k * kW];
   int p[5] = {0, 1, 2, 3, 4};
   int row = 0;  // persists across columns, like the reference's
 #pragma unroll
@@ -201,7 +267,6 @@ __global__ void __launch_bounds__(kBlock) k_prepare_sgs(int nnode, const int* __
         row = j;
       }
     }
-    // swap rows c and row (and their permutation entries)
 #pragma unroll
     for (int j = c + 1; j < 5; ++j) {
       if (j == row) {
@@ -225,17 +290,17 @@ __global__ void __launch_bounds__(kBlock) k_prepare_sgs(int nnode, const int* __
       for (int k = c + 1; k < 5; ++k) A[j * 5 + k] -= A[j * 5 + c] * A[c * 5 + k];
   }
 #pragma unroll
-  for (int k = 0; k < 25; ++k) a[k] = A[k];
-  piv[i] = p[0] | (p[1] << 3) | (p[2] << 6) | (p[3] << 9) | (p[4] << 12);
+  for (int k = 0; k < 25; ++k) a[k * kW] = A[k];
+  piv[r] = p[0] | (p[1] << 3) | (p[2] << 6) | (p[3] << 9) | (p[4] << 12);
 }
 
 void launch_prepare_sgs(Ctx& c) {
   if (c.prm.precision == PCFD_CHOSEN_FP32)
-    hipLaunchKernelGGL(k_prepare_sgs<float>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.d.blk_ptr,
-                       c.d.Mf, c.d.piv);
+    hipLaunchKernelGGL(k_prepare_sgs<float>, dim3(nblk(c.nrow_pad)), dim3(kBlock), 0, c.stream, c.nrow_pad, c.d.srow,
+                       c.d.Df, c.d.piv);
   else
-    hipLaunchKernelGGL(k_prepare_sgs<double>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode,
-                       c.d.blk_ptr, c.d.Md, c.d.piv);
+    hipLaunchKernelGGL(k_prepare_sgs<double>, dim3(nblk(c.nrow_pad)), dim3(kBlock), 0, c.stream, c.nrow_pad,
+                       c.d.srow, c.d.Dd, c.d.piv);
   PCFD_HIP(hipGetLastError());
 }
 

@@ -9,23 +9,24 @@
 namespace pcfd {
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_apply_dq(int nnode, int ntot, double gamma, T* __restrict__ x, double* __restrict__ q,
-                           int* __restrict__ count) {
+__global__ void __launch_bounds__(kBlock) k_apply_dq(int nnode, int ntot, double gamma, T* __restrict__ x8,
+                                                    double* __restrict__ q, int* __restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double dq[5], Q[5];
   bool zero = false;
+  T* xi = x8 + (long long)i * 8;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    dq[k] = (double)x[k * nnode + i];
-    if (isnan(dq[k]) || isinf(dq[k])) zero = true;
+    dq[k] = (double)xi[k];
+    if (__builtin_isnan(dq[k]) || __builtin_isinf(dq[k])) zero = true;
     Q[k] = q[k * ntot + i];
   }
   if (zero) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       dq[k] = 0.0;
-      x[k * nnode + i] = (T)0;
+      xi[k] = (T)0;
     }
     atomicAdd(count, 1);
   }
@@ -37,8 +38,8 @@ __global__ void __launch_bounds__(kBlock) k_apply_dq(int nnode, int ntot, double
 // ExplicitSolve x = b*dt/vol with ApplyDQ, then NewtonIterate's update, which
 // applies the same x a second time (the reference calls ApplyDQ in both).
 __global__ void __launch_bounds__(kBlock) k_explicit(int nnode, int ntot, double gamma, const double* __restrict__ b,
-                           const double* __restrict__ dt, const double* __restrict__ vol, double* __restrict__ x,
-                           double* __restrict__ q, int* __restrict__ count) {
+                                                    const double* __restrict__ dt, const double* __restrict__ vol,
+                                                    double* __restrict__ q, int* __restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double dq[5], Q[5];
@@ -51,7 +52,7 @@ __global__ void __launch_bounds__(kBlock) k_explicit(int nnode, int ntot, double
   bool zero = false;
 #pragma unroll
   for (int k = 0; k < 5; ++k)
-    if (isnan(dq[k]) || isinf(dq[k])) zero = true;
+    if (__builtin_isnan(dq[k]) || __builtin_isinf(dq[k])) zero = true;
   if (zero) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) dq[k] = 0.0;
@@ -59,26 +60,23 @@ __global__ void __launch_bounds__(kBlock) k_explicit(int nnode, int ntot, double
   }
   apply_dq(dq, Q, gamma);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    x[k * nnode + i] = dq[k];
-    q[k * ntot + i] = Q[k];
-  }
+  for (int k = 0; k < 5; ++k) q[k * ntot + i] = Q[k];
 }
 
-__global__ void __launch_bounds__(kBlock) k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_int(int* p) { *p = 0; }
 
 void launch_apply_dq(Ctx& c, int explicit_mode, int* nzeroed) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c.stream, c.d.icount);
   PCFD_HIP(hipGetLastError());
   if (explicit_mode) {
     hipLaunchKernelGGL(k_explicit, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma,
-                       c.d.b, c.d.dt, c.d.vol, c.d.xd, c.d.q, c.d.icount);
+                       c.d.b, c.d.dt, c.d.vol, c.d.q, c.d.icount);
   } else if (c.prm.precision == PCFD_CHOSEN_FP32) {
     hipLaunchKernelGGL(k_apply_dq<float>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.prm.gamma, c.d.xf, c.d.q, c.d.icount);
+                       c.prm.gamma, c.d.x8f, c.d.q, c.d.icount);
   } else {
     hipLaunchKernelGGL(k_apply_dq<double>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.prm.gamma, c.d.xd, c.d.q, c.d.icount);
+                       c.prm.gamma, c.d.x8d, c.d.q, c.d.icount);
   }
   PCFD_HIP(hipGetLastError());
   if (nzeroed) {

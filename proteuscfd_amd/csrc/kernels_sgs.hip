@@ -3,7 +3,10 @@
 // The reference sweeps rows 0..n-1 then n-1..0 with the newest x.  Rows of
 // one colour share no off-diagonal block, so sweeping colour by colour is
 // exactly Gauss-Seidel in the colour-sorted row order (and its reverse on the
-// way back); within a colour all rows update in parallel.
+// way back); within a colour all rows update in parallel.  Blocks are read
+// from the SELL layout (one lane per row, the 64 rows of a chunk read one
+// 512-byte (FP64) / 256-byte (FP32) line per block entry), x is kept as
+// 8-wide AoS so a neighbour's 5 values come from one line.
 #include <stdexcept>
 
 #include "common.h"
@@ -42,40 +45,78 @@ __device__ __forceinline__ void lu_solve5(const T* A, int pk, T* b) {
   }
 }
 
+// b (node-order SoA, FP64) -> SELL row order [chunk][eq][lane] in type T
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_sgs_color(int n, const int* __restrict__ nodes, int nnode,
-                                                     const int* __restrict__ row_ptr, const int* __restrict__ col,
-                                                     const int* __restrict__ blk_ptr, const T* __restrict__ M,
-                                                     const int* __restrict__ piv, const double* __restrict__ b,
-                                                     T* __restrict__ x) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const int i = nodes[t];
-  T rhs[5];
+__global__ void __launch_bounds__(kBlock) k_pack_b(int nrow_pad, int nnode, const int* __restrict__ srow,
+                                                  const double* __restrict__ b, T* __restrict__ bs) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow_pad) return;
+  const int i = srow[r];
+  T* o = bs + (long long)(r >> 6) * 5 * kW + (r & 63);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) rhs[k] = (T)b[k * nnode + i];
-  const int b0 = blk_ptr[i] + 1 - row_ptr[i];
-  for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
-    const int j = col[p];
-    const T* blk = M + (size_t)(b0 + p) * 25;
-    T xj[5];
+  for (int k = 0; k < 5; ++k) o[k * kW] = (i >= 0) ? (T)b[k * nnode + i] : (T)0;
+}
+
+// Four lanes per row: lanes {l, l+16, l+32, l+48} of a wavefront share a row
+// and take every fourth slot, so a colour launch has four times as many
+// waves in flight; their partial sums are combined with a fixed butterfly
+// (every lane ends with the same value) before lane group 0 solves the
+// diagonal block.  A wavefront covers 16 consecutive SELL rows.
+constexpr int kTPR = 4;
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const int* __restrict__ srow,
+                                                     const int* __restrict__ swidth, const int* __restrict__ sbase,
+                                                     const int* __restrict__ scol, const T* __restrict__ Moff,
+                                                     const T* __restrict__ Dg, const int* __restrict__ piv,
+                                                     const T* __restrict__ bs, T* __restrict__ x8) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = (threadIdx.x & 63) >> 4;
+  const int local = (t >> 6) * 16 + (threadIdx.x & 15);
+  const bool inr = local < nrows;
+  const int r = row0 + (inr ? local : 0);
+  const int i = inr ? srow[r] : -1;
+  const int chunk = r >> 6, lane = r & 63;
+  T acc[5] = {0, 0, 0, 0, 0};
+  if (i >= 0) {
+    const int w = swidth[chunk];
+    const long long cb = (long long)sbase[chunk] * kW + lane;
+    const T* mb = Moff + (long long)sbase[chunk] * 25 * kW + lane;
+    for (int s = sub; s < w; s += kTPR) {
+      const int j = scol[cb + (long long)s * kW];
+      if (j < 0) break;
+      const T* xj = x8 + (long long)j * 8;
+      const T x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3], x4 = xj[4];
+      const T* blk = mb + (long long)s * 25 * kW;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) xj[k] = x[k * nnode + j];
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-      T v = blk[r * 5 + 0] * xj[0];
-#pragma unroll
-      for (int c = 1; c < 5; ++c) v += blk[r * 5 + c] * xj[c];
-      rhs[r] -= v;
+      for (int rr = 0; rr < 5; ++rr) {
+        T v = blk[(rr * 5 + 0) * kW] * x0;
+        v += blk[(rr * 5 + 1) * kW] * x1;
+        v += blk[(rr * 5 + 2) * kW] * x2;
+        v += blk[(rr * 5 + 3) * kW] * x3;
+        v += blk[(rr * 5 + 4) * kW] * x4;
+        acc[rr] += v;
+      }
     }
   }
-  const T* D = M + (size_t)blk_ptr[i] * 25;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    acc[k] += __shfl_xor(acc[k], 16);
+    acc[k] += __shfl_xor(acc[k], 32);
+  }
+  if (i < 0 || sub != 0) return;
+  T rhs[5];
+  const T* bb = bs + (long long)chunk * 5 * kW + lane;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) rhs[k] = bb[k * kW] - acc[k];
+  const T* dg = Dg + (long long)chunk * 25 * kW + lane;
   T A[25];
 #pragma unroll
-  for (int k = 0; k < 25; ++k) A[k] = D[k];
-  lu_solve5(A, piv[i], rhs);
+  for (int k = 0; k < 25; ++k) A[k] = dg[k * kW];
+  lu_solve5(A, piv[r], rhs);
+  T* xo = x8 + (long long)i * 8;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) x[k * nnode + i] = rhs[k];
+  for (int k = 0; k < 5; ++k) xo[k] = rhs[k];
 }
 
 template <typename T>
@@ -85,12 +126,15 @@ __global__ void __launch_bounds__(kBlock) k_zero(long long n, T* __restrict__ x)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_sumsq_partial(long long n, const T* __restrict__ x, double* __restrict__ red) {
+__global__ void __launch_bounds__(kBlock) k_sumsq_x8(int nnode, const T* __restrict__ x8, double* __restrict__ red) {
   __shared__ double sh[kBlock];
   double s = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    double v = (double)x[i];
-    s += v * v;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnode; i += gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double v = (double)x8[(long long)i * 8 + k];
+      s += v * v;
+    }
   }
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -102,39 +146,42 @@ __global__ void __launch_bounds__(kBlock) k_sumsq_partial(long long n, const T* 
 }
 
 template <typename T>
-static double xnorm(Ctx& c, const T* x) {
+static double xnorm(Ctx& c, const T* x8) {
   const int nb = 512;
-  long long n = 5LL * c.nnode;
-  hipLaunchKernelGGL(k_sumsq_partial<T>, dim3(nb), dim3(kBlock), 0, c.stream, n, x, c.d.red);
+  hipLaunchKernelGGL(k_sumsq_x8<T>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, x8, c.d.red);
   PCFD_HIP(hipGetLastError());
   double h[nb];
   PCFD_HIP(hipMemcpyAsync(h, c.d.red, sizeof(h), hipMemcpyDeviceToHost, c.stream));
   PCFD_HIP(hipStreamSynchronize(c.stream));
   double s = 0.0;
   for (int k = 0; k < nb; ++k) s += h[k];
-  return std::sqrt(s) / (double)n;
+  return std::sqrt(s) / (double)(5LL * c.nnode);
 }
 
 template <typename T>
-static void sgs_impl(Ctx& c, int nsgs, const T* M, T* x, double* delta) {
-  long long n5 = 5LL * c.nnode;
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n5)), dim3(kBlock), 0, c.stream, n5, x);
+static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8, double* delta) {
+  long long n8 = 8LL * c.nnode;
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n8)), dim3(kBlock), 0, c.stream, n8, x8);
+  PCFD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pack_b<T>, dim3(nblk(c.nrow_pad)), dim3(kBlock), 0, c.stream, c.nrow_pad, c.nnode, c.d.srow,
+                     c.d.b, bs);
   PCFD_HIP(hipGetLastError());
   double xold = 0.0, xn = 0.0;
   for (int it = 0; it < nsgs; ++it) {
     for (int dir = 0; dir < 2; ++dir) {
       for (int cc = 0; cc < c.ncolor; ++cc) {
-        int col = dir == 0 ? cc : c.ncolor - 1 - cc;
-        int beg = c.color_ptr[col], cnt = c.color_ptr[col + 1] - beg;
-        if (cnt == 0) continue;
-        hipLaunchKernelGGL(k_sgs_color<T>, dim3(nblk(cnt)), dim3(kBlock), 0, c.stream, cnt,
-                           c.d.color_nodes + beg, c.nnode, c.d.row_ptr, c.d.col, c.d.blk_ptr, M, c.d.piv, c.d.b, x);
+        const int col = dir == 0 ? cc : c.ncolor - 1 - cc;
+        const int row0 = c.color_chunk[col] * kW;
+        const int nrows = (c.color_chunk[col + 1] - c.color_chunk[col]) * kW;
+        if (nrows == 0) continue;
+        hipLaunchKernelGGL(k_sgs_color<T>, dim3(nblk((long long)nrows * kTPR)), dim3(kBlock), 0, c.stream, row0,
+                           nrows, c.d.srow, c.d.swidth, c.d.sbase, c.d.scol, Moff, Dg, c.d.piv, bs, x8);
         PCFD_HIP(hipGetLastError());
       }
     }
     if (delta && it >= nsgs - 2) {
       xold = xn;
-      xn = xnorm(c, x);
+      xn = xnorm(c, x8);
     }
   }
   if (delta) *delta = std::fabs(xold - xn);
@@ -142,9 +189,9 @@ static void sgs_impl(Ctx& c, int nsgs, const T* M, T* x, double* delta) {
 
 void launch_sgs(Ctx& c, int nsgs, double* delta) {
   if (c.prm.precision == PCFD_CHOSEN_FP32)
-    sgs_impl<float>(c, nsgs, c.d.Mf, c.d.xf, delta);
+    sgs_impl<float>(c, nsgs, c.d.Mf, c.d.Df, c.d.bsf, c.d.x8f, delta);
   else
-    sgs_impl<double>(c, nsgs, c.d.Md, c.d.xd, delta);
+    sgs_impl<double>(c, nsgs, c.d.Md, c.d.Dd, c.d.bsd, c.d.x8d, delta);
 }
 
 }  // namespace pcfd

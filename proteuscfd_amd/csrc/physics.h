@@ -161,7 +161,94 @@ PCFD_DEV void roe_flux(const double* QL, const double* QR, const double* a, doub
   else if (sym2d == 3) F[3] = 0.0;
 #pragma unroll
   for (int i = 0; i < 5; ++i)
-    if (isnan(F[i])) F[i] = 0.0;
+    if (__builtin_isnan(F[i])) F[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Roe flux in wave-strength form.  Same Roe average, eigenvalues,
+// Harten-Hyman type-2 entropy fix (ucs/compressible.tcc:92-230) and NaN
+// zeroing as roe_flux above, but the dissipation T*|L|*Tinv*dQ is assembled
+// from the characteristic wave strengths of the primitive jumps (the Roe
+// average makes the two forms identical; results differ by rounding only)
+// and every division by rho, c, c^2 is a multiplication by a reciprocal.
+// Per-state quantities live in RoeSide so a finite-difference Jacobian can
+// reuse the unperturbed side.
+struct RoeSide {
+  double ir, u, v, w, p, h, th, c;
+};
+
+PCFD_DEV void roe_side(const double* Q, const double* n, double gamma, RoeSide& s) {
+  const double gm1 = gamma - 1.0;
+  s.ir = 1.0 / Q[0];
+  s.u = Q[1] * s.ir;
+  s.v = Q[2] * s.ir;
+  s.w = Q[3] * s.ir;
+  const double v2 = s.u * s.u + s.v * s.v + s.w * s.w;
+  s.p = gm1 * (Q[4] - 0.5 * Q[0] * v2);
+  s.h = (Q[4] + s.p) * s.ir;
+  s.th = s.u * n[0] + s.v * n[1] + s.w * n[2];
+  s.c = sqrt(gamma * s.p * s.ir);
+}
+
+PCFD_DEV double hh_fix(double e, double eL, double eR) {
+  const double eps = dmax(0.0, dmax(e - eL, eR - e));
+  return (fabs(e) < eps) ? 0.5 * (e * e / eps + eps) : fabs(e);
+}
+
+PCFD_DEV void roe_flux_s(const double* QL, const RoeSide& L, const double* QR, const RoeSide& R, const double* a,
+                         double gamma, int sym2d, double* F) {
+  const double gm1 = gamma - 1.0;
+  const double nx = a[0], ny = a[1], nz = a[2];
+  const double rr = sqrt(QL[0] * QR[0]);
+  const double sigma = rr / (QL[0] + rr);
+  const double u = L.u + sigma * (R.u - L.u);
+  const double v = L.v + sigma * (R.v - L.v);
+  const double w = L.w + sigma * (R.w - L.w);
+  const double h = L.h + sigma * (R.h - L.h);
+  const double v2h = 0.5 * (u * u + v * v + w * w);
+  const double c2 = gm1 * (h - v2h);
+  const double c = sqrt(c2);
+  const double ic2 = 1.0 / c2;
+  const double th = u * nx + v * ny + w * nz;
+  const double l0 = hh_fix(th, L.th, R.th);
+  const double l3 = hh_fix(th + c, L.th + L.c, R.th + R.c);
+  const double l4 = hh_fix(th - c, L.th - L.c, R.th - R.c);
+  const double dr = QR[0] - QL[0];
+  const double du = R.u - L.u, dv = R.v - L.v, dw = R.w - L.w;
+  const double dp = R.p - L.p;
+  const double dV = du * nx + dv * ny + dw * nz;
+  const double rc = rr * c;
+  const double t4 = l4 * (0.5 * (dp - rc * dV) * ic2);  // theta - c wave
+  const double t3 = l3 * (0.5 * (dp + rc * dV) * ic2);  // theta + c wave
+  const double t0 = l0 * (dr - dp * ic2);               // entropy wave
+  const double ts = l0 * rr;                            // shear waves
+  double d[5];
+  d[0] = t4 + t3 + t0;
+  d[1] = t4 * (u - c * nx) + t3 * (u + c * nx) + t0 * u + ts * (du - dV * nx);
+  d[2] = t4 * (v - c * ny) + t3 * (v + c * ny) + t0 * v + ts * (dv - dV * ny);
+  d[3] = t4 * (w - c * nz) + t3 * (w + c * nz) + t0 * w + ts * (dw - dV * nz);
+  d[4] = t4 * (h - c * th) + t3 * (h + c * th) + t0 * v2h + ts * (u * du + v * dv + w * dw - th * dV);
+  const double rtL = QL[0] * L.th, rtR = QR[0] * R.th;
+  const double hA = 0.5 * a[3];
+  F[0] = hA * (rtL + rtR - d[0]);
+  F[1] = hA * ((L.u * rtL + L.p * nx) + (R.u * rtR + R.p * nx) - d[1]);
+  F[2] = hA * ((L.v * rtL + L.p * ny) + (R.v * rtR + R.p * ny) - d[2]);
+  F[3] = hA * ((L.w * rtL + L.p * nz) + (R.w * rtR + R.p * nz) - d[3]);
+  F[4] = hA * (L.h * rtL + R.h * rtR - d[4]);
+  if (sym2d == 1) F[1] = 0.0;
+  else if (sym2d == 2) F[2] = 0.0;
+  else if (sym2d == 3) F[3] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+    if (__builtin_isnan(F[i])) F[i] = 0.0;
+}
+
+PCFD_DEV void roe_flux_fast(const double* QL, const double* QR, const double* a, double gamma, int sym2d,
+                            double* F) {
+  RoeSide L, R;
+  roe_side(QL, a, gamma, L);
+  roe_side(QR, a, gamma, R);
+  roe_flux_s(QL, L, QR, R, a, gamma, sym2d, F);
 }
 
 // MaxEigenvalue on the edge-averaged state (ucs/compressible.tcc:798-821)
