@@ -38,7 +38,7 @@ def algorithmic_bytes(m, nblocks, prec_bytes):
         "residual_1st": 80 * nn + 40 * ne + 84 * nb,
         "sgs_sweep": nblocks * 25 * prec_bytes + 8 * 15 * nn,
         "jacobian": nblocks * 25 * prec_bytes + 80 * nn + 40 * ne,
-        "gradient5": nn * 8 * (5 + 6 + 3 + 15) + 8 * ne,
+        "gradient9": nn * 8 * (10 + 6 + 3 + 27) + 8 * ne,
         "limiter": nn * 8 * (5 + 15 + 3 + 1 + 5) + 8 * ne,
     }
 
@@ -67,6 +67,37 @@ def timed_solves(ctx, case, k, max_iter, drop):
         iters.append(nit)
         hists.append(hist)
     return times, iters, hists
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sgs_color_bump100_traffic.txt")
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per unit of `kernel_key` measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size) and
+    recorded in the profile summary; None when no measurement is recorded."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == f"traffic_bytes[{kernel_key}]":
+                return float(parts[1])
+    return None
+
+
+def copy_calibration(nbytes):
+    """A plain device copy of known size (torch's copy kernel), run first so
+    a counter pass can scale FETCH_SIZE/WRITE_SIZE to bytes."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: {nbytes} bytes read + {nbytes} bytes written per copy")
+    del a, b
 
 
 def cpu_baseline(case, budget_s):
@@ -98,6 +129,9 @@ def main():
     ap.add_argument("--iters-only", type=int, default=0,
                     help="profiling mode: run this many iterations of each form instead of full solves")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run a device copy of BYTES first (for counter scaling)")
+    ap.add_argument("--venkat-k", type=float, default=5.0, help="modified-Venkat K (the workload's K=5)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     args = ap.parse_args()
 
@@ -107,11 +141,13 @@ def main():
               "(the sharding design is described in DESIGN.md). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     from proteuscfd_amd import cases, gpu
     from proteuscfd_amd.cases import Params
 
     t0 = time.time()
-    base = dict(sorder=2, limiter=3, venkat_K=1.0, cfl=50.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
+    base = dict(sorder=2, limiter=3, venkat_K=args.venkat_k, cfl=50.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
     case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
     m = case.mesh
     log(f"[bench] mesh bump {args.n}^3: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
@@ -120,7 +156,7 @@ def main():
     results = {}
     for prec, name in [(0, "plain_fp64"), (1, "chosen_fp32")]:
         p = Params(**base, precision=prec)
-        ctx = gpu.Context(case, p, gradient_terms=5)
+        ctx = gpu.Context(case, p, gradient_terms=9)
         info = ctx.info()
         # warmup
         ctx.upload_q(case.initial_state())
@@ -161,22 +197,25 @@ def main():
     pp, pc = plain["prof"], chosen["prof"]
     nsgs = base["nsgs"]
     sweeps = 2 * nsgs
+    # kernel-only times (HIP events on the library stream around the launches)
     kern = {
-        "residual_2nd_fp64": dict(ms=pp["residual"], gbs=gbs(ab64["residual_2nd"], pp["residual"])),
-        "sgs_sweep_fp64": dict(ms=pp["sgs"] / sweeps, gbs=gbs(ab64["sgs_sweep"], pp["sgs"] / sweeps)),
-        "sgs_sweep_fp32": dict(ms=pc["sgs"] / sweeps, gbs=gbs(ab32["sgs_sweep"], pc["sgs"] / sweeps)),
-        "jacobian_fp64": dict(ms=pp["jacobian"], gbs=gbs(ab64["jacobian"], pp["jacobian"])),
-        "jacobian_fp32": dict(ms=pc["jacobian"], gbs=gbs(ab32["jacobian"], pc["jacobian"])),
-        "gradient": dict(ms=pp["gradient"], gbs=gbs(ab64["gradient5"], pp["gradient"])),
+        "residual_2nd_fp64": dict(ms=pp["k_residual"], gbs=gbs(ab64["residual_2nd"], pp["k_residual"])),
+        "sgs_sweep_fp64": dict(ms=pp["k_sgs_sweeps"] / sweeps, gbs=gbs(ab64["sgs_sweep"], pp["k_sgs_sweeps"] / sweeps)),
+        "sgs_sweep_fp32": dict(ms=pc["k_sgs_sweeps"] / sweeps, gbs=gbs(ab32["sgs_sweep"], pc["k_sgs_sweeps"] / sweeps)),
+        "jacobian_fp64": dict(ms=pp["k_jacobian"], gbs=gbs(ab64["jacobian"], pp["k_jacobian"])),
+        "jacobian_fp32": dict(ms=pc["k_jacobian"], gbs=gbs(ab32["jacobian"], pc["k_jacobian"])),
+        "gradient9": dict(ms=pp["gradient"], gbs=gbs(ab64["gradient9"], pp["gradient"])),
         "limiter": dict(ms=pp["limiter"], gbs=gbs(ab64["limiter"], pp["limiter"])),
     }
     for v in kern.values():
         v["frac"] = v["gbs"] / HBM_PEAK_GBS
-    per_iter_ms = {k: sum(v.values()) for k, v in [("plain", pp), ("chosen", pc)]}
+    phase_keys = gpu.PHASES[:9]
+    per_iter_ms = {k: sum(v[x] for x in phase_keys) for k, v in [("plain", pp), ("chosen", pc)]}
     dominant = max(["sgs", "jacobian", "residual"], key=lambda k: pc[k])
     dom_key = {"sgs": "sgs_sweep_fp32", "jacobian": "jacobian_fp32", "residual": "residual_2nd_fp64"}[dominant]
     dom = kern[dom_key]
-    edge_flux = (m.nedge + m.nbedge) / (pp["residual"] * 1e-3)
+    traffic = measured_traffic(dom_key)
+    edge_flux = (m.nedge + m.nbedge) / (pp["k_residual"] * 1e-3)
     # converged-field error of the chosen form against the plain FP64 path
     qp, qc = plain["q"][: m.nnode, :5], chosen["q"][: m.nnode, :5]
     field_err = {v: float(np.abs(qc[:, k] - qp[:, k]).max() / np.abs(qp[:, k]).max())
@@ -214,7 +253,8 @@ def main():
         "config": {
             "workload": f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
                         f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
-                        f"M=0.5 farfield, weighted LSQ, modified Venkat K=1, first-order FD Jacobian, "
+                        f"M=0.5 farfield, weighted LSQ (9 terms), modified Venkat K={args.venkat_k:g}, "
+                        f"first-order FD Jacobian, "
                         f"8 SGS, CFL 1->50 over 20 iterations",
             "step": "one full steady solve from freestream to an 8-order residual drop",
             "parallelism": "single GPU",
@@ -231,7 +271,7 @@ def main():
         "phase_ms": {"plain": pp, "chosen": pc},
         "kernels": kern,
         "roofline": {"bound": "hbm", "kernel": dom_key, "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["frac"], "traffic": None},
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic},
         "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true>", "achieved": kern["residual_2nd_fp64"]["gbs"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern["residual_2nd_fp64"]["frac"]},
         "cpu_baseline": cpu,

@@ -138,8 +138,12 @@ int pcfd_solve(pcfd_ctx* ctx, int max_iter, double drop, double* history, int* n
 
 /* Per-phase device time (ms) accumulated since the last reset, measured with
  * HIP events on the context stream: [0] bcs [1] gradient [2] limiter
- * [3] residual [4] timestep [5] jacobian [6] prepare_sgs [7] sgs [8] update. */
-int pcfd_phase_times(pcfd_ctx* ctx, double ms[9], int reset);
+ * [3] residual [4] timestep [5] jacobian [6] prepare_sgs [7] sgs [8] update,
+ * and kernel-only times inside phases: [9] SGS colour sweeps (no packing or
+ * norms) [10] residual flux kernel (no norm reduction) [11] Jacobian fill
+ * kernels (boundary part, rows, Dirichlet rows). */
+#define PCFD_NPHASE 12
+int pcfd_phase_times(pcfd_ctx* ctx, double ms[PCFD_NPHASE], int reset);
 int pcfd_enable_phase_timing(pcfd_ctx* ctx, int on);
 int pcfd_synchronize(pcfd_ctx* ctx);
 

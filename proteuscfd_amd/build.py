@@ -55,9 +55,20 @@ def build_hip() -> str:
     return out
 
 
+def build_examples() -> str:
+    """C++ host program on include/pcfd_ucs.hpp, linked against libpcfd.so."""
+    src = os.path.join(HERE, "examples", "ucs_newton.cpp")
+    out = os.path.join(HERE, "examples", "ucs_newton")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, src, "-o", out + ".tmp", "-L", HERE, "-lpcfd",
+          "-Wl,-rpath,$ORIGIN/.."])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_all() -> None:
     build_mesh()
     build_hip()
+    build_examples()
 
 
 if __name__ == "__main__":

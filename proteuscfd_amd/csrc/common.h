@@ -99,8 +99,9 @@ struct Ctx {
   Dev d;
   // timing
   bool timing = false;
-  double phase_ms[9] = {0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double phase_ms[PCFD_NPHASE] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // phase timer
+  hipEvent_t ev2 = nullptr, ev3 = nullptr;  // kernel sub-timer inside a phase
   long long device_bytes = 0;
 };
 
@@ -116,6 +117,12 @@ inline int nblk(long long n, int bs = kBlock) { return (int)((n + bs - 1) / bs);
 __host__ __device__ inline long long gell_index(const int* gbase, int i, int s) {
   return ((long long)gbase[i >> 6] + s) * kW + (i & 63);
 }
+
+// timers: phase k (events ev0/ev1) and kernel-only sub-timers (ev2/ev3)
+void phase_begin(Ctx& c);
+void phase_end(Ctx& c, int k);
+void sub_begin(Ctx& c);
+void sub_end(Ctx& c, int k);
 
 // launchers (each enqueues on ctx.stream; throw std::runtime_error on failure)
 void launch_lsq_moments(Ctx& c);

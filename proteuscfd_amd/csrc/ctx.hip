@@ -48,8 +48,10 @@ void free_all(Ctx& c) {
   dfree(d.red_out); dfree(d.icount);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
+  if (c.ev2) (void)hipEventDestroy(c.ev2);
+  if (c.ev3) (void)hipEventDestroy(c.ev3);
   if (c.stream) (void)hipStreamDestroy(c.stream);
-  c.ev0 = c.ev1 = nullptr;
+  c.ev0 = c.ev1 = c.ev2 = c.ev3 = nullptr;
   c.stream = nullptr;
 }
 
@@ -116,6 +118,17 @@ void phase_end(Ctx& c, int k) {
   PCFD_HIP(hipEventElapsedTime(&ms, c.ev0, c.ev1));
   c.phase_ms[k] += ms;
 }
+void sub_begin(Ctx& c) {
+  if (c.timing) PCFD_HIP(hipEventRecord(c.ev2, c.stream));
+}
+void sub_end(Ctx& c, int k) {
+  if (!c.timing) return;
+  PCFD_HIP(hipEventRecord(c.ev3, c.stream));
+  PCFD_HIP(hipEventSynchronize(c.ev3));
+  float ms = 0.f;
+  PCFD_HIP(hipEventElapsedTime(&ms, c.ev2, c.ev3));
+  c.phase_ms[k] += ms;
+}
 void ensure_matrix_alloc(Ctx& c) { ensure_matrix(c); }
 }  // namespace pcfd
 
@@ -164,6 +177,8 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     PCFD_HIP(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     PCFD_HIP(hipEventCreate(&c.ev0));
     PCFD_HIP(hipEventCreate(&c.ev1));
+    PCFD_HIP(hipEventCreate(&c.ev2));
+    PCFD_HIP(hipEventCreate(&c.ev3));
     c.nnode = nn;
     c.nbnode = nb;
     c.nedge = ne;
@@ -653,9 +668,9 @@ int pcfd_explicit_update(pcfd_ctx* ctx, int* nzeroed) {
 int pcfd_phase_times(pcfd_ctx* ctx, double ms[9], int reset) {
   if (!ctx) return PCFD_ERR_ARG;
   if (ms)
-    for (int k = 0; k < 9; ++k) ms[k] = ctx->c.phase_ms[k];
+    for (int k = 0; k < PCFD_NPHASE; ++k) ms[k] = ctx->c.phase_ms[k];
   if (reset)
-    for (int k = 0; k < 9; ++k) ctx->c.phase_ms[k] = 0.0;
+    for (int k = 0; k < PCFD_NPHASE; ++k) ctx->c.phase_ms[k] = 0.0;
   return PCFD_OK;
 }
 

@@ -512,6 +512,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce6(int nparts, const double* __
 void launch_residual(Ctx& c, int ho) {
   const pcfd_params& p = c.prm;
   int nb = nblk(c.nnode);
+  sub_begin(c);
   if (ho) {
     hipLaunchKernelGGL(k_residual<true>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
                        p.symmetry2d, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a,
@@ -522,6 +523,7 @@ void launch_residual(Ctx& c, int ho) {
                        c.nbnode, c.d.b, c.d.red);
   }
   PCFD_HIP(hipGetLastError());
+  sub_end(c, 10);
   c.nred = nb;
 }
 

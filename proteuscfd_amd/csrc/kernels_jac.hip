@@ -217,6 +217,7 @@ void launch_jacobian(Ctx& c) {
   JacArgs A{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase, c.d.gcol,
             c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type, c.d.qref, c.d.vol,
             c.d.dt, c.d.q, c.d.srow, c.d.sbase};
+  sub_begin(c);
   if (c.nbnd > 0) {
     const long long bw = (c.nbnd + 11) / 12;
     hipLaunchKernelGGL(k_jac_boundary, dim3(nblk(bw * kW)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd,
@@ -237,6 +238,7 @@ void launch_jacobian(Ctx& c) {
                        c.d.bnd_nodes, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.bc_type, c.d.qref, c.d.q);
     PCFD_HIP(hipGetLastError());
   }
+  sub_end(c, 11);
 }
 
 // In-place LU with partial pivoting of each diagonal block (LU,

@@ -168,6 +168,7 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
   PCFD_HIP(hipGetLastError());
   double xold = 0.0, xn = 0.0;
   for (int it = 0; it < nsgs; ++it) {
+    sub_begin(c);
     for (int dir = 0; dir < 2; ++dir) {
       for (int cc = 0; cc < c.ncolor; ++cc) {
         const int col = dir == 0 ? cc : c.ncolor - 1 - cc;
@@ -179,6 +180,7 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
         PCFD_HIP(hipGetLastError());
       }
     }
+    sub_end(c, 9);
     if (delta && it >= nsgs - 2) {
       xold = xn;
       xn = xnorm(c, x8);

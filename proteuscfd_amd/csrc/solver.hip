@@ -13,8 +13,6 @@
 #include "common.h"
 
 namespace pcfd {
-void phase_begin(Ctx& c);
-void phase_end(Ctx& c, int k);
 void ensure_matrix_alloc(Ctx& c);
 }  // namespace pcfd
 

@@ -19,6 +19,10 @@ ip = C.POINTER(C.c_int)
 
 PLAIN_FP64, CHOSEN_FP32 = 0, 1
 
+# pcfd_phase_times slots (PCFD_NPHASE); the last three are kernel-only times
+PHASES = ["bcs", "gradient", "limiter", "residual", "timestep", "jacobian", "prepare_sgs", "sgs", "update",
+          "k_sgs_sweeps", "k_residual", "k_jacobian"]
+
 
 class MeshDesc(C.Structure):
     _fields_ = [("nnode", C.c_int), ("nbnode", C.c_int), ("nedge", C.c_int), ("edges", ip), ("edge_a", dp),
@@ -255,10 +259,9 @@ class Context:
         return hist[: n.value], n.value, secs.value
 
     def phase_times(self, reset=False):
-        ms = np.zeros(9)
+        ms = np.zeros(len(PHASES))
         self._chk(lib().pcfd_phase_times(self.h, _d(ms), int(reset)), "pcfd_phase_times")
-        return dict(zip(["bcs", "gradient", "limiter", "residual", "timestep", "jacobian", "prepare_sgs", "sgs",
-                         "update"], ms))
+        return dict(zip(PHASES, ms))
 
     def enable_phase_timing(self, on=True):
         self._chk(lib().pcfd_enable_phase_timing(self.h, int(on)), "pcfd_enable_phase_timing")
