@@ -59,6 +59,8 @@ typedef struct {
   int nfactag;              /* face tags are 0..nfactag-1 */
   const int* bc_type;       /* [nfactag] BCTypes ids (ucs/bc_defines.h:4-30) */
   const double* qref;       /* [nfactag*5] conservative reference state per tag (BCObj Qref) */
+  const double* twall;      /* [nfactag] non-dimensional wall temperature of noSlip tags, <0 adiabatic
+                               (BCObj twall / ref_temperature, ucs/bc.tcc:1284-1290); NULL: adiabatic */
 } pcfd_mesh_desc;
 
 /* Solver parameters (ucs/param.tcc keywords) */
@@ -78,6 +80,11 @@ typedef struct {
   int jac_freq;          /* jacobianUpdateFrequency */
   int precision;         /* PCFD_PLAIN_FP64 | PCFD_CHOSEN_FP32 */
   int gradient_terms;    /* 9: full reference set; 5: conservative only (inviscid fast path) */
+  int viscous;           /* 1: laminar Navier-Stokes terms (CompressibleNS, Laminar model, mut = 0) */
+  double re_tilde;       /* Re / Mach (ucs/compressible.tcc:753-755) */
+  double prandtl;        /* prandtlNumber (0.72) */
+  double prandtl_t;      /* PrT (0.85, ucs/param.tcc:89) */
+  double sutherland_s;   /* 110.4 K / ref_temperature (ucs/eqnset.h:251-268) */
 } pcfd_params;
 
 const char* pcfd_last_error(const pcfd_ctx* ctx);

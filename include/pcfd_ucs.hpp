@@ -44,6 +44,7 @@ inline void Check(const pcfd_ctx* ctx, int rc, const char* what) {
 struct BoundaryConditions {
   std::vector<int> type;     /* [nfactag] */
   std::vector<double> qref;  /* [nfactag*5] */
+  std::vector<double> twall; /* [nfactag] noSlip wall temperature / T_ref, <0 adiabatic; empty: adiabatic */
 };
 
 /* Mesh arrays as CalcMetrics leaves them (ucs/mesh.h:199-254): edges with
@@ -66,6 +67,9 @@ inline pcfd_mesh_desc Describe(const pcfd_mesh_data& mesh, const BoundaryConditi
   d.nfactag = mesh.nfactag;
   d.bc_type = bc.type.data();
   d.qref = bc.qref.data();
+  d.twall = bc.twall.empty() ? nullptr : bc.twall.data();
+  if (!bc.twall.empty() && (int)bc.twall.size() != mesh.nfactag)
+    throw Error(PCFD_ERR_ARG, "BoundaryConditions: twall needs one value per face tag");
   return d;
 }
 

@@ -21,7 +21,8 @@ ip = C.POINTER(C.c_int)
 class OMesh(C.Structure):
     _fields_ = [("nnode", C.c_int), ("nbnode", C.c_int), ("nedge", C.c_int),
                 ("edges", ip), ("edge_a", dp), ("bedges", ip), ("bedge_a", dp), ("bfactag", ip),
-                ("xyz", dp), ("vol", dp), ("nfactag", C.c_int), ("bc_type", ip), ("qref", dp)]
+                ("xyz", dp), ("vol", dp), ("nfactag", C.c_int), ("bc_type", ip), ("qref", dp),
+                ("twall", dp), ("bnormal", ip)]
 
 
 class OParams(C.Structure):
@@ -29,7 +30,8 @@ class OParams(C.Structure):
                 ("limiter", C.c_int), ("venkat_K", C.c_double), ("limiter_refresh", C.c_int),
                 ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double),
                 ("cfl_ramp", C.c_int), ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int),
-                ("clip_jacobi", C.c_int)]
+                ("clip_jacobi", C.c_int), ("viscous", C.c_int), ("re_tilde", C.c_double), ("pr", C.c_double),
+                ("prt", C.c_double), ("sutherland_s", C.c_double)]
 
 
 class OCrs(C.Structure):
@@ -56,6 +58,10 @@ def lib():
         L.o_roe_flux.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_int, dp]
         L.o_roe_flux.restype = C.c_int
         L.o_bc_state.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double]
+        L.o_bc_state_ex.argtypes = [C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, dp]
+        L.o_wall_normal_nodes.argtypes = [C.POINTER(OMesh), ip]
+        L.o_viscous_flux.argtypes = [C.POINTER(OParams), dp, dp, dp, dp]
+        L.o_viscous_jacobian.argtypes = [C.POINTER(OParams), dp, dp, dp, C.c_double, dp, dp, dp]
         L.o_update_bcs.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp]
         L.o_lsq_moments.argtypes = [C.POINTER(OMesh), dp]
         L.o_gradient.argtypes = [C.POINTER(OMesh), dp, dp, dp]
@@ -102,23 +108,44 @@ def _i32(a):
 class Problem:
     """Mesh + BCs + parameters in oracle form (keeps the numpy buffers alive)."""
 
-    def __init__(self, mesh, bc_type, qref, params, clip_jacobi=False):
+    def __init__(self, mesh, bc_type, qref, params, clip_jacobi=False, twall=None):
+        nt = len(bc_type)
+        tw = np.full(nt, -1.0) if twall is None else _f64(twall)
         self.keep = dict(edges=_i32(mesh.edges), edge_a=_f64(mesh.edge_a), bedges=_i32(mesh.bedges),
                          bedge_a=_f64(mesh.bedge_a), bfactag=_i32(mesh.bfactag), xyz=_f64(mesh.xyz),
-                         vol=_f64(mesh.vol), bc_type=_i32(bc_type), qref=_f64(qref))
+                         vol=_f64(mesh.vol), bc_type=_i32(bc_type), qref=_f64(qref), twall=_f64(tw),
+                         bnormal=np.full(mesh.nbedge, -1, dtype=np.int32))
         k = self.keep
         self.nnode, self.nbnode, self.nedge = mesh.nnode, mesh.nbedge, mesh.nedge
         self.m = OMesh(mesh.nnode, mesh.nbedge, mesh.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
-                       _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), len(k["bc_type"]),
-                       _i(k["bc_type"]), _d(k["qref"]))
+                       _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), nt,
+                       _i(k["bc_type"]), _d(k["qref"]), _d(k["twall"]), _i(k["bnormal"]))
+        lib().o_wall_normal_nodes(C.byref(self.m), _i(k["bnormal"]))
         self.params = params
         self.p = OParams(params.gamma, params.sorder, params.nfirst, params.chi, params.limiter, params.venkat_K,
                          params.limiter_refresh, params.limiter_freeze, params.cfl, params.cfl_start,
-                         params.cfl_ramp, params.nsgs, params.symmetry2d, params.jac_freq, int(clip_jacobi))
+                         params.cfl_ramp, params.nsgs, params.symmetry2d, params.jac_freq, int(clip_jacobi),
+                         int(params.viscous), params.re_tilde, params.prandtl, params.prandtl_t,
+                         params.sutherland_s)
+
+    @property
+    def bnormal(self):
+        return self.keep["bnormal"]
 
     @classmethod
     def from_case(cls, case, clip_jacobi=False):
-        return cls(case.mesh, case.bc_type, case.qref, case.params, clip_jacobi)
+        return cls(case.mesh, case.bc_type, case.qref, case.params, clip_jacobi, twall=case.twall)
+
+    def viscous_flux(self, Q, grad, avec):
+        F = np.zeros(5)
+        lib().o_viscous_flux(C.byref(self.p), _d(_f64(Q)), _d(_f64(grad)), _d(_f64(avec)), _d(F))
+        return F
+
+    def viscous_jacobian(self, QL, QR, dx, s2, avec):
+        aL, aR = np.zeros(25), np.zeros(25)
+        lib().o_viscous_jacobian(C.byref(self.p), _d(_f64(QL)), _d(_f64(QR)), _d(_f64(dx)), float(s2),
+                                 _d(_f64(avec)), _d(aL), _d(aR))
+        return aL.reshape(5, 5), aR.reshape(5, 5)
 
     # --- single phases (arrays in reference AoS layout) ---
     def update_bcs(self, q):

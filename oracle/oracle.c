@@ -24,6 +24,8 @@
 static inline double omax(double x, double y) { return x > y ? x : y; }  /* ucs/macros.h:32-36 */
 static inline double omin(double x, double y) { return x < y ? x : y; }  /* ucs/macros.h:38-42 */
 
+static int cmp_int(const void* a, const void* b);
+
 static double now_s(void) {
   struct timespec ts;
   clock_gettime(CLOCK_MONOTONIC, &ts);
@@ -298,10 +300,43 @@ static void wall_state(const double* QL, double* QR, const double* avec, double 
   }
 }
 
-/* CalculateBoundaryVariables: ucs/bc.tcc:1058-1398 (types on this path) */
+/* GetViscousWallBoundaryVariables (ucs/compressible.tcc:1544-1575) for a
+ * wall at rest (vel = 0, no bleed steps): isothermal walls set the energy
+ * from Twall, adiabatic walls take density and energy from the wall-normal
+ * neighbour; both sides hard-set the interior node too. */
+static void noslip_state(double* QL, double* QR, double twall, const double* normalQ, double gamma) {
+  const double vel[3] = {0.0, 0.0, 0.0};
+  if (twall < 0.0) {
+    QR[0] = QL[0] = normalQ[0];
+    QR[4] = QL[4] = normalQ[4];
+  } else {
+    double v2 = vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2];
+    double gm1 = gamma - 1.0;
+    double rhoEt = (twall * QL[0] / (gamma * gm1) + 0.5 * QR[0] * v2);
+    QR[0] = QL[0];
+    QR[4] = QL[4] = rhoEt;
+  }
+  QR[1] = QL[1] = QL[0] * vel[0];
+  QR[2] = QL[2] = QL[0] * vel[1];
+  QR[3] = QL[3] = QL[0] * vel[2];
+}
+
 void o_bc_state(int bctype, double* QL, double* QR, const double* Qref, const double* avec, double gamma) {
+  o_bc_state_ex(bctype, QL, QR, Qref, avec, gamma, -1.0, NULL);
+}
+
+/* CalculateBoundaryVariables: ucs/bc.tcc:1058-1398 (types on this path) */
+void o_bc_state_ex(int bctype, double* QL, double* QR, const double* Qref, const double* avec, double gamma,
+                   double twall, const double* normalQ) {
   const double vdotn = 0.0;
   if (bctype == O_BC_PARALLEL) return;
+  if (bctype == O_BC_NOSLIP) {
+    if (twall < 0.0 && !normalQ) return;
+    noslip_state(QL, QR, twall, normalQ, gamma);
+    o_aux(QR, gamma);
+    o_aux(QL, gamma);
+    return;
+  }
   if (bctype == O_BC_SONIC_INFLOW || bctype == O_BC_DIRICHLET) {
     for (int i = 0; i < NV; i++) QR[i] = QL[i] = Qref[i];
   } else if (bctype == O_BC_SONIC_OUTFLOW || bctype == O_BC_NEUMANN) {
@@ -317,13 +352,70 @@ void o_bc_state(int bctype, double* QL, double* QR, const double* Qref, const do
   o_aux(QL, gamma);
 }
 
+static double tag_twall(const o_mesh* m, int tag) { return m->twall ? m->twall[tag] : -1.0; }
+
+static const double* normal_state(const o_mesh* m, const double* q, int e) {
+  int nn = m->bnormal ? m->bnormal[e] : -1;
+  return nn >= 0 ? &q[NV * nn] : NULL;
+}
+
 /* UpdateBCs -> BdriverNoScatter(BC_Kernel): ucs/bc.tcc:1399-1457, 723-745 */
 void o_update_bcs(const o_mesh* m, const o_params* p, double* q) {
   for (int e = 0; e < m->nbnode; e++) {
     int L = m->bedges[2 * e], R = m->bedges[2 * e + 1];
     int tag = m->bfactag[e];
-    o_bc_state(m->bc_type[tag], &q[L * NV], &q[R * NV], &m->qref[tag * NV], &m->bedge_a[4 * e], p->gamma);
+    double nQ[NV];
+    const double* ns = normal_state(m, q, e);
+    if (ns) memcpy(nQ, ns, sizeof(nQ));
+    o_bc_state_ex(m->bc_type[tag], &q[L * NV], &q[R * NV], &m->qref[tag * NV], &m->bedge_a[4 * e], p->gamma,
+                  tag_twall(m, tag), ns ? nQ : NULL);
   }
+}
+
+/* Wall-normal neighbour: over the node's neighbours in ascending id, the
+ * last one maximising -unit(x_pt - x_wall).a with a start of 0
+ * (ucs/bc.tcc:1182-1210). */
+void o_wall_normal_nodes(const o_mesh* m, int* bnormal) {
+  int nn = m->nnode;
+  int* cnt = (int*)calloc((size_t)nn + 1, sizeof(int));
+  for (int e = 0; e < m->nedge; e++) {
+    cnt[m->edges[2 * e] + 1]++;
+    cnt[m->edges[2 * e + 1] + 1]++;
+  }
+  for (int i = 0; i < nn; i++) cnt[i + 1] += cnt[i];
+  int* adj = (int*)malloc(sizeof(int) * (size_t)cnt[nn]);
+  int* fill = (int*)malloc(sizeof(int) * (size_t)nn);
+  memcpy(fill, cnt, sizeof(int) * (size_t)nn);
+  for (int e = 0; e < m->nedge; e++) {
+    int a = m->edges[2 * e], b = m->edges[2 * e + 1];
+    adj[fill[a]++] = b;
+    adj[fill[b]++] = a;
+  }
+  for (int i = 0; i < nn; i++) qsort(&adj[cnt[i]], (size_t)(cnt[i + 1] - cnt[i]), sizeof(int), cmp_int);
+  for (int e = 0; e < m->nbnode; e++) {
+    bnormal[e] = -1;
+    if (m->bc_type[m->bfactag[e]] != O_BC_NOSLIP) continue;
+    int L = m->bedges[2 * e];
+    const double* wx = &m->xyz[3 * L];
+    const double* a = &m->bedge_a[4 * e];
+    double dotmax = 0.0;
+    for (int k = cnt[L]; k < cnt[L + 1]; k++) {
+      int pt = adj[k];
+      double dx[3];
+      for (int i = 0; i < 3; i++) dx[i] = m->xyz[3 * pt + i] - wx[i];
+      double mag = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+      for (int i = 0; i < 3; i++) dx[i] = dx[i] / mag;
+      double dot = 0.0;
+      for (int i = 0; i < 3; i++) dot -= dx[i] * a[i];
+      if (dot >= dotmax) {
+        bnormal[e] = pt;
+        dotmax = dot;
+      }
+    }
+  }
+  free(cnt);
+  free(adj);
+  free(fill);
 }
 
 /* ComputeNodeLSQCoefficients (weighted set sw): ucs/gradient.tcc:114-138, 460-503 */
@@ -561,6 +653,208 @@ void o_limiter(const o_mesh* m, const o_params* p, const double* q, const double
   free(qmax);
 }
 
+/* ComputeViscosity, Sutherland's law on the non-dimensional T = Q[5]
+ * (ucs/eqnset.h:251-268) */
+static double viscosity(const o_params* p, const double* Q) {
+  double T = Q[5];
+  double S = p->sutherland_s;
+  double mu = (1.0 + S) * pow(T, 1.5) / (T + S);
+  return mu;
+}
+
+/* ViscousFlux: ucs/compressible.tcc:712-795 (written on the right-hand side) */
+void o_viscous_flux(const o_params* p, const double* Q, const double* grad, const double* avec, double* flux) {
+  const double mut = 0.0;
+  double Tx = grad[15], Ty = grad[16], Tz = grad[17];
+  double ux = grad[18], uy = grad[19], uz = grad[20];
+  double vx = grad[21], vy = grad[22], vz = grad[23];
+  double wx = grad[24], wy = grad[25], wz = grad[26];
+  double rho = Q[0];
+  double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+  double mu = viscosity(p, Q);
+  double tmut = (mu + mut);
+  double fact = -2.0 / 3.0 * (ux + vy + wz);
+  double tauxx = 2.0 * ux + fact;
+  double tauyy = 2.0 * vy + fact;
+  double tauzz = 2.0 * wz + fact;
+  double tauxy = uy + vx;
+  double tauxz = uz + wx;
+  double tauyz = vz + wy;
+  double RK = avec[3] / p->re_tilde;
+  double RKT = RK * tmut;
+  double cp = 1.0 / (p->gamma - 1.0);
+  double k = mu / p->pr * cp;
+  double kT = mut / p->prt * cp;
+  double c1 = -(k + kT);
+  double Tn = Tx * avec[0] + Ty * avec[1] + Tz * avec[2];
+  double tauxn = tauxx * avec[0] + tauxy * avec[1] + tauxz * avec[2];
+  double tauyn = tauxy * avec[0] + tauyy * avec[1] + tauyz * avec[2];
+  double tauzn = tauxz * avec[0] + tauyz * avec[1] + tauzz * avec[2];
+  flux[0] = 0.0;
+  flux[1] = -RKT * (tauxn);
+  flux[2] = -RKT * (tauyn);
+  flux[3] = -RKT * (tauzn);
+  flux[4] = -RKT * (tauxn * u + tauyn * v + tauzn * w) + RK * c1 * Tn;
+  if (p->symmetry2d == 1) flux[1] = 0.0;
+  else if (p->symmetry2d == 2) flux[2] = 0.0;
+  else if (p->symmetry2d == 3) flux[3] = 0.0;
+}
+
+/* one side (Dx = the directional-derivative weights of that side) of
+ * ViscousJacobian: ucs/compressible.tcc:1662-1857 */
+static void viscous_jac_side(const double* Dx, const double* avec, double u, double v, double w, double RK,
+                             double RKT, double Tn, double dP_dr, double P_over_rho, double dP_dru, double dP_drv,
+                             double dP_drw, double dP_dret, double* a) {
+  const double c43 = 4.0 / 3.0, mc23 = -2.0 / 3.0;
+  double dux = -u * Dx[0], duy = -u * Dx[1], duz = -u * Dx[2];
+  double dvx = -v * Dx[0], dvy = -v * Dx[1], dvz = -v * Dx[2];
+  double dwx = -w * Dx[0], dwy = -w * Dx[1], dwz = -w * Dx[2];
+  double dfact = -2.0 / 3.0 * (dux + dvy + dwz);
+  double dtauxx = (2.0 * dux + dfact);
+  double dtauyy = (2.0 * dvy + dfact);
+  double dtauzz = (2.0 * dwz + dfact);
+  double dtauxy = duy + dvx;
+  double dtauxz = duz + dwx;
+  double dtauyz = dvz + dwy;
+  double dtauxn = dtauxx * avec[0] + dtauxy * avec[1] + dtauxz * avec[2];
+  double dtauyn = dtauxy * avec[0] + dtauyy * avec[1] + dtauyz * avec[2];
+  double dtauzn = dtauxz * avec[0] + dtauyz * avec[1] + dtauzz * avec[2];
+  double d2u = (c43 * Dx[0] * avec[0] + Dx[1] * avec[1] + Dx[2] * avec[2]);
+  double d2v = (mc23 * Dx[1] * avec[0] + Dx[0] * avec[1]);
+  double d2w = (mc23 * Dx[2] * avec[0] + Dx[0] * avec[2]);
+  double d3u = (mc23 * Dx[0] * avec[1] + Dx[1] * avec[0]);
+  double d3v = (Dx[0] * avec[0] + c43 * Dx[1] * avec[1] + Dx[2] * avec[2]);
+  double d3w = (mc23 * Dx[2] * avec[1] + Dx[1] * avec[2]);
+  double d4u = (mc23 * Dx[0] * avec[2] + Dx[2] * avec[0]);
+  double d4v = (mc23 * Dx[1] * avec[2] + Dx[2] * avec[1]);
+  double d4w = (Dx[0] * avec[0] + Dx[1] * avec[1] + c43 * Dx[2] * avec[2]);
+  for (int i = 0; i < 5; i++) a[i] = 0.0;
+  a[5] = -RKT * (dtauxn);
+  a[6] = -RKT * d2u;
+  a[7] = -RKT * d2v;
+  a[8] = -RKT * d2w;
+  a[9] = 0.0;
+  a[10] = -RKT * (dtauyn);
+  a[11] = -RKT * d3u;
+  a[12] = -RKT * d3v;
+  a[13] = -RKT * d3w;
+  a[14] = 0.0;
+  a[15] = -RKT * (dtauzn);
+  a[16] = -RKT * d4u;
+  a[17] = -RKT * d4v;
+  a[18] = -RKT * d4w;
+  a[19] = 0.0;
+  a[20] = -RKT * (dtauxn * u + dtauyn * v + dtauzn * w) + RK * Tn * (dP_dr - P_over_rho);
+  a[21] = -RKT * (d2u * u + d3u * v + d4u * w) + RK * Tn * dP_dru;
+  a[22] = -RKT * (d2v * u + d3v * v + d4v * w) + RK * Tn * dP_drv;
+  a[23] = -RKT * (d2w * u + d3w * v + d4w * w) + RK * Tn * dP_drw;
+  a[24] = RK * Tn * dP_dret;
+}
+
+/* ViscousJacobian: ucs/compressible.tcc:1632-1893.  aL = dR/dQL with the
+ * sign flipped for the left node, aR = dR/dQR. */
+void o_viscous_jacobian(const o_params* p, const double* QL, const double* QR, const double* dx, double s2,
+                        const double* avec, double* aL, double* aR) {
+  const double mut = 0.0;
+  double Qavg[NV];
+  for (int i = 0; i < NE; i++) Qavg[i] = 0.5 * (QL[i] + QR[i]);
+  o_aux(Qavg, p->gamma);
+  double mu = viscosity(p, Qavg);
+  double tmut = (mu + mut);
+  double RK = avec[3] / p->re_tilde;
+  double RKT = RK * tmut;
+  double rhoL = QL[0], uL = QL[1] / rhoL, vL = QL[2] / rhoL, wL = QL[3] / rhoL, PL = QL[6];
+  double rhoR = QR[0], uR = QR[1] / rhoR, vR = QR[2] / rhoR, wR = QR[3] / rhoR, PR = QR[6];
+  double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
+  double DxL[3], DxR[3];
+  for (int i = 0; i < 3; i++) {
+    DxL[i] = -dx[i] / (rhoL * s2);
+    DxR[i] = dx[i] / (rhoR * s2);
+  }
+  double gamma = p->gamma, gm1 = (gamma - 1.0);
+  double v2L = (uL * uL + vL * vL + wL * wL);
+  double v2R = (uR * uR + vR * vR + wR * wR);
+  double dT_dPL = gamma / rhoL, dT_dPR = gamma / rhoR;
+  double cp = 1.0 / (p->gamma - 1.0);
+  double k = mu / p->pr * cp;
+  double kT = mut / p->prt * cp;
+  double c1 = -(k + kT);
+  double TnL = (DxL[0] * avec[0] + DxL[1] * avec[1] + DxL[2] * avec[2]) * dT_dPL * c1;
+  double TnR = (DxR[0] * avec[0] + DxR[1] * avec[1] + DxR[2] * avec[2]) * dT_dPR * c1;
+  viscous_jac_side(DxR, avec, u, v, w, RK, RKT, TnR, +gm1 * 0.5 * v2R, PR / rhoR, -gm1 * uR, -gm1 * vR, -gm1 * wR,
+                   +gm1, aR);
+  viscous_jac_side(DxL, avec, u, v, w, RK, RKT, TnL, +gm1 * 0.5 * v2L, PL / rhoL, -gm1 * uL, -gm1 * vL, -gm1 * wL,
+                   +gm1, aL);
+  for (int i = 0; i < 25; i++) aL[i] = -aL[i];
+  if (p->symmetry2d >= 1 && p->symmetry2d <= 3) {
+    int r = p->symmetry2d;
+    for (int i = 0; i < NE; i++) aL[r * NE + i] = aR[r * NE + i] = 0.0;
+  }
+}
+
+/* Kernel_Viscous_Flux / Bkernel_Viscous_Flux (ucs/residual.tcc:389-562):
+ * averaged state, averaged gradient with the directional correction along
+ * the edge at second order (terms at gradientsLoc {0..5,7,8,9},
+ * ucs/compressible.tcc:1009-1027); the boundary side uses the node's own
+ * gradient.  b[R] += F, b[L] -= F; boundary b[L] -= F. */
+static void viscous_residual(const o_mesh* m, const o_params* p, int iter, const double* q, const double* grad,
+                             double* b) {
+  static const int gloc[O_NTERMS] = {0, 1, 2, 3, 4, 5, 7, 8, 9};
+  int ho = (p->sorder > 1 && iter > p->nfirst);
+  for (int e = 0; e < m->nedge; e++) {
+    int L = m->edges[2 * e], R = m->edges[2 * e + 1];
+    const double* qL = &q[NV * L];
+    const double* qR = &q[NV * R];
+    double Qavg[NV], g[NT * 3], F[5];
+    for (int i = 0; i < NE; i++) Qavg[i] = (qL[i] + qR[i]) / 2.0;
+    o_aux(Qavg, p->gamma);
+    const double* gL = &grad[NT * 3 * L];
+    const double* gR = &grad[NT * 3 * R];
+    for (int i = 0; i < NT * 3; i++) g[i] = 0.5 * (gL[i] + gR[i]);
+    if (ho) {
+      double dx[3], s2 = 0.0;
+      for (int i = 0; i < 3; i++) {
+        dx[i] = (m->xyz[3 * R + i] - m->xyz[3 * L + i]);
+        s2 += dx[i] * dx[i];
+      }
+      for (int j = 0; j < NT; j++) {
+        double qdots = dx[0] * g[j * 3] + dx[1] * g[j * 3 + 1] + dx[2] * g[j * 3 + 2];
+        int loc = gloc[j];
+        double dq = (qR[loc] - qL[loc] - qdots) / s2;
+        for (int i = 0; i < 3; i++) g[j * 3 + i] += dq * dx[i];
+      }
+    }
+    o_viscous_flux(p, Qavg, g, &m->edge_a[4 * e], F);
+    for (int i = 0; i < NE; i++) {
+      b[NE * L + i] += -F[i];
+      b[NE * R + i] += F[i];
+    }
+  }
+  for (int e = 0; e < m->nbnode; e++) {
+    int L = m->bedges[2 * e], R = m->bedges[2 * e + 1];
+    double Qavg[NV], F[5];
+    for (int i = 0; i < NE; i++) Qavg[i] = 0.5 * (q[NV * L + i] + q[NV * R + i]);
+    o_aux(Qavg, p->gamma);
+    o_viscous_flux(p, Qavg, &grad[NT * 3 * L], &m->bedge_a[4 * e], F);
+    for (int i = 0; i < NE; i++) b[NE * L + i] += -F[i];
+  }
+}
+
+/* Bkernel_BC_Res_Modify -> ModifyViscousWallResidual (ucs/bc.tcc:902-1030,
+ * ucs/compressible.tcc:1608-1630): hard-set rows of noSlip nodes */
+static void noslip_residual(const o_mesh* m, double* b) {
+  for (int e = 0; e < m->nbnode; e++) {
+    int tag = m->bfactag[e];
+    if (m->bc_type[tag] != O_BC_NOSLIP) continue;
+    double* r = &b[NE * m->bedges[2 * e]];
+    r[1] = 0.0;
+    r[2] = 0.0;
+    r[3] = 0.0;
+    r[4] = 0.0;
+    if (tag_twall(m, tag) < 0.0) r[0] = 0.0;
+  }
+}
+
 /* ComputeResiduals/SpatialResidual: ucs/residual.tcc:12-122, interior kernel
  * :191-296 (b[L] -= F, b[R] += F), boundary kernel :298-387 (b[L] -= F with the
  * unextrapolated state against the phantom), residGlobal/strided norms
@@ -604,6 +898,8 @@ void o_residual(const o_mesh* m, const o_params* p, int iter, const double* q, c
     o_roe_flux(&q[NV * L], &q[NV * R], &m->bedge_a[4 * e], 0.0, p->gamma, p->symmetry2d, F);
     for (int i = 0; i < NE; i++) b[NE * L + i] += -F[i];
   }
+  if (p->viscous) viscous_residual(m, p, iter, q, grad, b);
+  noslip_residual(m, b);
   if (norms) {
     double s = 0.0, se[5] = {0, 0, 0, 0, 0};
     for (int i = 0; i < nn * NE; i++) s += b[i] * b[i];
@@ -721,6 +1017,13 @@ void o_crs_free(o_crs* A) {
   memset(A, 0, sizeof(*A));
 }
 
+/* CRSMatrix::BlankSubRow: ucs/crsmatrix.tcc:525-541 */
+static void blank_subrow(o_crs* A, int row, int sub) {
+  for (int k = A->ia[row]; k < A->ia[row + 1]; k++)
+    for (int i = 0; i < NE; i++) A->M[25 * (size_t)k + sub * NE + i] = 0.0;
+  A->M[25 * (size_t)A->iau[row] + sub * NE + sub] = 1.0;
+}
+
 /* ComputeJacobians: ComputeSpatialJacobian (ucs/jacobian.tcc:129-210) with
  * Kernel_NumJac (:253-304, one-sided h=1e-8 on the unextrapolated states),
  * Bkernel_NumJac (:458-535, boundary states re-evaluated under perturbation,
@@ -763,7 +1066,9 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
     double* QL = &q[NV * L];
     double* QR = &q[NV * R];
     double fS[5], fL[5], QPL[NV], QPR[NV];
-    o_bc_state(bct, QL, QR, Qref, a, g);
+    double tw = tag_twall(m, tag);
+    const double* ns = normal_state(m, q, e);
+    o_bc_state_ex(bct, QL, QR, Qref, a, g, tw, ns);
     o_roe_flux(QL, QR, a, 0.0, g, p->symmetry2d, fS);
     double* D = &A->M[25 * (size_t)A->iau[L]];
     for (int i = 0; i < NE; i++) {
@@ -772,9 +1077,28 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
       o_aux(QPL, g);
       memcpy(QPR, QR, sizeof(QPR));
       o_aux(QPR, g);
-      o_bc_state(bct, QPL, QPR, Qref, a, g);
+      o_bc_state_ex(bct, QPL, QPR, Qref, a, g, tw, ns);
       o_roe_flux(QPL, QPR, a, 0.0, g, p->symmetry2d, fL);
       for (int j = 0; j < NE; j++) D[j * NE + i] += (fL[j] - fS[j]) / h;
+    }
+  }
+  if (p->viscous) {
+    /* Driver(Kernel_Viscous_Jac): ucs/jacobian.tcc:734-767; the boundary
+     * variant (:771-816) ends with size 0 and contributes nothing */
+    for (int e = 0; e < m->nedge; e++) {
+      int L = m->edges[2 * e], R = m->edges[2 * e + 1];
+      double dx[3], s2 = 0.0, aL[25], aR[25];
+      for (int i = 0; i < 3; i++) {
+        dx[i] = (m->xyz[3 * R + i] - m->xyz[3 * L + i]);
+        s2 += dx[i] * dx[i];
+      }
+      o_viscous_jacobian(p, &q[NV * L], &q[NV * R], dx, s2, &m->edge_a[4 * e], aL, aR);
+      double* bRL = &A->M[25 * (size_t)A->e_rl[e]];
+      double* bLR = &A->M[25 * (size_t)A->e_lr[e]];
+      for (int k = 0; k < 25; k++) {
+        bRL[k] += aL[k];
+        bLR[k] += aR[k];
+      }
     }
   }
   for (int e = 0; e < m->nedge; e++) {
@@ -791,6 +1115,32 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
   for (int i = 0; i < m->nnode; i++) {
     double* D = &A->M[25 * (size_t)A->iau[i]];
     for (int k = 0; k < NE; k++) D[k * NE + k] += 1.0 * m->vol[i] / dt[i];
+  }
+  /* Bkernel_BC_Jac_Modify -> ModifyViscousWallJacobian (ucs/bc.tcc:747-900,
+   * ucs/compressible.tcc:1577-1606), per noSlip half-edge in order */
+  for (int e = 0; e < m->nbnode; e++) {
+    int tag = m->bfactag[e];
+    if (m->bc_type[tag] != O_BC_NOSLIP) continue;
+    int row = m->bedges[2 * e];
+    double tw = tag_twall(m, tag);
+    blank_subrow(A, row, 1);
+    blank_subrow(A, row, 2);
+    blank_subrow(A, row, 3);
+    double* diag = &A->M[25 * (size_t)A->iau[row]];
+    if (tw < 0.0) {
+      blank_subrow(A, row, 0);
+      blank_subrow(A, row, 4);
+      int nn = m->bnormal ? m->bnormal[e] : -1;
+      for (int k = A->ia[row]; nn >= 0 && k < A->ia[row + 1]; k++)
+        if (A->ja[k] == nn) {
+          A->M[25 * (size_t)k + 0] = -1.0;
+          A->M[25 * (size_t)k + 24] = -1.0;
+        }
+    } else {
+      double v2 = 0.0;
+      blank_subrow(A, row, 4);
+      diag[4 * NE + 0] = -(tw / (g * (g - 1.0)) + 0.5 * v2);
+    }
   }
 }
 
@@ -930,7 +1280,7 @@ double o_solver_step(o_solver* s, double* norms, int* nzeroed) {
   o_timestep(m, p, s->q, s->cfl, s->dt);
   o_update_bcs(m, p, s->q);
   double t2 = now_s();
-  if (p->sorder > 1 && s->iter > p->nfirst) o_gradient(m, s->q, s->sw, s->grad);
+  if ((p->sorder > 1 && s->iter > p->nfirst) || p->viscous) o_gradient(m, s->q, s->sw, s->grad);
   int lr = p->limiter_refresh > 0 ? p->limiter_refresh : 1;
   if ((s->iter % lr == 0) && p->sorder > 1 && p->limiter && s->iter > p->nfirst && s->iter < p->limiter_freeze)
     o_limiter(m, p, s->q, s->grad, s->l);

@@ -40,6 +40,8 @@ typedef struct {
   int nfactag;
   const int* bc_type;    /* [nfactag] */
   const double* qref;    /* [nfactag*10] reference state (with aux vars) */
+  const double* twall;   /* [nfactag] non-dimensional wall temperature of noSlip tags (<0: adiabatic); may be NULL */
+  const int* bnormal;    /* [nbnode] most wall-normal neighbour of a noSlip half-edge's node (o_wall_normal_nodes) */
 } o_mesh;
 
 typedef struct {
@@ -55,6 +57,10 @@ typedef struct {
   int symmetry2d;
   int jac_freq;
   int clip_jacobi;              /* 0: reference sequential pressure clip; 1: all edges see the pre-clip limiter */
+  int viscous;                  /* laminar Navier-Stokes terms (CompressibleNS, mut = 0) */
+  double re_tilde;              /* Re / Mach (ReTilde, ucs/compressible.tcc:753-755) */
+  double pr, prt;               /* Prandtl and turbulent Prandtl numbers */
+  double sutherland_s;          /* 110.4 K / T_ref (ucs/eqnset.h:251-268) */
 } o_params;
 
 typedef struct {
@@ -72,6 +78,18 @@ void o_aux(double* Q, double gamma);
 int o_roe_flux(const double* QL, const double* QR, const double* avec, double vdotn, double gamma,
                int symmetry2d, double* flux);
 void o_bc_state(int bctype, double* QL, double* QR, const double* Qref, const double* avec, double gamma);
+/* as o_bc_state, plus noSlip walls: twall (non-dimensional, <0 adiabatic) and
+ * the state of the wall-normal neighbour (adiabatic walls only) */
+void o_bc_state_ex(int bctype, double* QL, double* QR, const double* Qref, const double* avec, double gamma,
+                   double twall, const double* normalQ);
+/* wall-normal neighbour of each noSlip half-edge's node (-1 elsewhere):
+ * ucs/bc.tcc:1182-1210 */
+void o_wall_normal_nodes(const o_mesh* m, int* bnormal);
+/* CompressibleEqnSet::ViscousFlux (ucs/compressible.tcc:712-795), mut = 0 */
+void o_viscous_flux(const o_params* p, const double* Q, const double* grad, const double* avec, double* flux);
+/* CompressibleEqnSet::ViscousJacobian (ucs/compressible.tcc:1632-1893), mut = 0 */
+void o_viscous_jacobian(const o_params* p, const double* QL, const double* QR, const double* dx, double s2,
+                        const double* avec, double* aL, double* aR);
 
 void o_update_bcs(const o_mesh* m, const o_params* p, double* q);
 void o_lsq_moments(const o_mesh* m, double* sw);

@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
 MESH_SRC = ["mesh.cpp"]
-HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_jac.hip", "kernels_sgs.hip",
+HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_visc.hip", "kernels_jac.hip", "kernels_sgs.hip",
            "kernels_misc.hip", "solver.hip"]
 
 # gfx950 only.  Packed-FP32 instructions are disabled for all device code.

@@ -72,6 +72,17 @@ class Params:
     symmetry2d: int = -1
     jac_freq: int = 1
     precision: int = 0            # 0 plain FP64, 1 FP32 Jacobian + SGS
+    # laminar Navier-Stokes (CompressibleNS with the Laminar model, mut = 0)
+    viscous: int = 0
+    reynolds: float = 1.0         # Re (ucs/compressible.tcc:85-89)
+    mach_ref: float = 1.0         # freestream velocity / c_inf (Param::GetVelocity)
+    prandtl: float = 0.72         # prandtlNumber
+    prandtl_t: float = 0.85       # PrT (ucs/param.tcc:89)
+    sutherland_s: float = 110.4 / 300.0   # 110.4 K / T_ref (ucs/eqnset.h:251-268)
+
+    @property
+    def re_tilde(self) -> float:  # Re / Mach (ucs/compressible.tcc:753-755)
+        return self.reynolds / self.mach_ref if self.viscous else 1.0
 
     @property
     def chi(self) -> float:       # ucs/param.tcc:410-417
@@ -87,6 +98,7 @@ class Case:
     qinf: np.ndarray              # [10]
     params: Params
     extra: dict = field(default_factory=dict)
+    twall: np.ndarray | None = None   # [nfactag] non-dimensional noSlip wall temperature (<0 adiabatic)
 
     def initial_state(self) -> np.ndarray:
         """Uniform freestream on interior and phantom nodes (AoS, 10 vars)."""
@@ -120,6 +132,26 @@ def bump_case(n=100, seed=2, jitter=0.03, mach=0.5, gamma=1.4, rcm=True,
         params = Params(gamma=gamma, sorder=2, limiter=3, venkat_K=1.0, cfl=50.0, cfl_start=1.0,
                         cfl_ramp=50, nsgs=8)
     return Case("bump", m, bct, qref, q0, params)
+
+
+def plate_case(nx=100, ny=20, nz=40, nlayers=20, first_h=1e-4, growth=1.15, seed=3, jitter=0.0, mach=0.3,
+               reynolds=1.0e4, twall=1.0, gamma=1.4, rcm=True, params: Params | None = None) -> Case:
+    """Laminar flat-plate channel: prism layers on a noSlip isothermal floor
+    (twall = T_wall / T_ref, 1 = freestream temperature) under a Kuhn-tet
+    core; farfield inflow, outflow and top, symmetry side planes.  Re is
+    per unit length; T_ref = 300 K sets Sutherland's constant."""
+    m = M.Mesh.build(M.PLATE, nx=nx, ny=ny, nz=nz, nlayers=nlayers, first_h=first_h, growth=growth, seed=seed,
+                     jitter=jitter, rcm=int(rcm))
+    q0 = qinf(gamma, mach)
+    bct = np.array([BC_PARALLEL, BC_FARFIELD, BC_FARFIELD, BC_SYMMETRY, BC_SYMMETRY, BC_NOSLIP, BC_FARFIELD],
+                   dtype=np.int32)
+    qref = np.tile(q0, (M.NFACTAG, 1))
+    tw = np.full(M.NFACTAG, -1.0)
+    tw[5] = twall
+    if params is None:
+        params = Params(gamma=gamma, sorder=2, limiter=0, cfl=20.0, cfl_start=1.0, cfl_ramp=50, nsgs=8,
+                        viscous=1, reynolds=reynolds, mach_ref=mach)
+    return Case("plate", m, bct, qref, q0, params, twall=tw)
 
 
 def perturbed_state(case: Case, amp: float = 0.05, seed: int = 7) -> np.ndarray:

@@ -49,6 +49,9 @@ struct Dev {
   // boundary-condition tables per face tag
   int* bc_type = nullptr;   // ntag
   double* qref = nullptr;   // 5 x ntag (conservative), aux pressure at [5*ntag + tag]
+  double* twall = nullptr;  // ntag non-dimensional noSlip wall temperature (<0 adiabatic)
+  int* he_nrm = nullptr;    // nbnode: wall-normal neighbour of a noSlip half-edge's node, else -1
+  int* spos = nullptr;      // nnode: SELL row of each node
   // state
   double* q = nullptr;      // 5 x ntot (interior then phantoms)
   double* grad = nullptr;   // 27 x nnode (term*3+dir)
@@ -85,6 +88,7 @@ struct Ctx {
   long long nslot_g = 0, nslot_s = 0;
   long long nblocks = 0;  // nnode + 2*nedge (reference count)
   int ncolor = 0;
+  bool has_noslip = false;
   std::vector<int> color_chunk;  // [ncolor+1] chunk range of each colour
   std::vector<int> h_bc_type;
   // host copies used by downloads
@@ -113,6 +117,56 @@ struct Ctx {
 
 inline int nblk(long long n, int bs = kBlock) { return (int)((n + bs - 1) / bs); }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b runs on XCD
+// b%8).  xcd_block() renumbers them so XCD x runs one contiguous range of
+// logical blocks: neighbouring nodes (which share gathered neighbour data)
+// then sit in one XCD's L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+// gELL neighbour lists as kernel arguments
+struct Gell {
+  const int* width;
+  const int* base;
+  const int* col;
+  const double* nrm;
+  long long stride;  // nslot_g*64, component stride of nrm
+};
+
+__device__ __forceinline__ long long gbase_of(const Gell& g, int i) {
+  return (long long)g.base[i >> 6] * kW + (i & 63);
+}
+
+// Per-block partial sums of b^2 for the residual norms: total and per
+// equation, reduced with a fixed-shape tree (deterministic), written to
+// red[k*gridDim.x + blk].  Every thread of the block must call it.
+__device__ __forceinline__ void block_norm_partials(const double* bi, double* red, int blk) {
+  __shared__ double sh[6][kBlock];
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    double v = bi[k] * bi[k];
+    sh[1 + k][threadIdx.x] = v;
+    tot += v;
+  }
+  sh[0][threadIdx.x] = tot;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k * gridDim.x + blk] = sh[k][0];
+  }
+}
+
 // gELL traversal helpers: slot s of node i
 __host__ __device__ inline long long gell_index(const int* gbase, int i, int s) {
   return ((long long)gbase[i >> 6] + s) * kW + (i & 63);
@@ -129,7 +183,8 @@ void launch_lsq_moments(Ctx& c);
 void launch_update_bcs(Ctx& c);
 void launch_gradient(Ctx& c, int nterms);
 void launch_limiter(Ctx& c);
-void launch_residual(Ctx& c, int higher_order);
+void launch_residual(Ctx& c, int higher_order);  // inviscid, then viscous / noSlip edits
+void launch_viscous(Ctx& c, int higher_order);
 void launch_norms(Ctx& c, double* host_norms6);
 void launch_timestep(Ctx& c, double cfl);
 double launch_dtmin(Ctx& c);

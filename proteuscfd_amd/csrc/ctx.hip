@@ -41,7 +41,7 @@ void free_all(Ctx& c) {
   Dev& d = c.d;
   dfree(d.xyz); dfree(d.vol); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
   dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
-  dfree(d.jdb); dfree(d.bc_type);
+  dfree(d.jdb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
   dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
   dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.x8d); dfree(d.x8f); dfree(d.red);
@@ -265,6 +265,33 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     std::vector<double> qref(6 * (size_t)nt, 0.0);
     for (int t = 0; t < nt; ++t)
       for (int k = 0; k < 5; ++k) qref[(size_t)k * nt + t] = m->qref[5 * (size_t)t + k];
+    std::vector<double> twall(nt, -1.0);
+    if (m->twall) twall.assign(m->twall, m->twall + nt);
+    // wall-normal neighbour of each noSlip half-edge's node: over the
+    // neighbours in ascending id, the last one maximising -unit(x_pt - x_wall).a
+    // from a start of 0 (ucs/bc.tcc:1182-1210)
+    std::vector<int> he_nrm(nb, -1);
+    for (int h = 0; h < nb; ++h) {
+      if (c.h_bc_type[m->bfactag[h]] != 4 /* noSlip */) continue;
+      c.has_noslip = true;
+      const int L = m->bedges[2 * h];
+      const double* wx = &m->xyz[3 * (size_t)L];
+      const double* a = &m->bedge_a[4 * (size_t)h];
+      double dotmax = 0.0;
+      for (int p = row_ptr[L]; p < row_ptr[L + 1]; ++p) {
+        const int pt = c.h_col[p];
+        double dx[3];
+        for (int k = 0; k < 3; ++k) dx[k] = m->xyz[3 * (size_t)pt + k] - wx[k];
+        const double mag = std::sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        for (int k = 0; k < 3; ++k) dx[k] = dx[k] / mag;
+        double dot = 0.0;
+        for (int k = 0; k < 3; ++k) dot -= dx[k] * a[k];
+        if (dot >= dotmax) {
+          he_nrm[h] = pt;
+          dotmax = dot;
+        }
+      }
+    }
     c.nblocks = (long long)nn + c.nnz;
     // ---- colouring and the SELL row order
     std::vector<int> colour(nn);
@@ -330,6 +357,9 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     dalloc(c, &d.jdb, 25 * (size_t)std::max(c.nbnd, 1));
     dput(c, &d.bc_type, c.h_bc_type);
     dput(c, &d.qref, qref);
+    dput(c, &d.twall, twall);
+    dput(c, &d.he_nrm, he_nrm);
+    dput(c, &d.spos, spos);
     dalloc(c, &d.q, 5 * (size_t)c.ntot);
     dalloc(c, &d.grad, 27 * (size_t)nn);
     dalloc(c, &d.lim, 5 * (size_t)nn);
@@ -373,8 +403,10 @@ int pcfd_set_params(pcfd_ctx* ctx, const pcfd_params* p) {
     }
     if (p->gamma <= 1.0 || p->sorder < 1 || p->sorder > 3 || p->limiter < 0 || p->limiter > 3 || p->nsgs < 0 ||
         (p->precision != PCFD_PLAIN_FP64 && p->precision != PCFD_CHOSEN_FP32) ||
-        (p->gradient_terms != 5 && p->gradient_terms != 9)) {
-      c.err = "pcfd_set_params: parameter out of range";
+        (p->gradient_terms != 5 && p->gradient_terms != 9) ||
+        (p->viscous && (p->gradient_terms != 9 || !(p->re_tilde > 0.0) || !(p->prandtl > 0.0) ||
+                        !(p->prandtl_t > 0.0) || !(p->sutherland_s >= 0.0)))) {
+      c.err = "pcfd_set_params: parameter out of range (viscous runs need gradient_terms 9, Re/M, Pr, PrT > 0)";
       return PCFD_ERR_ARG;
     }
     c.prm = *p;

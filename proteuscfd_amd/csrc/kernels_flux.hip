@@ -16,17 +16,7 @@
 
 namespace pcfd {
 
-struct Gell {
-  const int* width;
-  const int* base;
-  const int* col;
-  const double* nrm;
-  long long stride;  // nslot_g*64, component stride of nrm
-};
 
-__device__ __forceinline__ long long gbase_of(const Gell& g, int i) {
-  return (long long)g.base[i >> 6] * kW + (i & 63);
-}
 
 static Gell gell(const Ctx& c) { return Gell{c.d.gwidth, c.d.gbase, c.d.gcol, c.d.gnrm, c.nslot_g * kW}; }
 
@@ -76,7 +66,9 @@ __global__ void __launch_bounds__(kBlock) k_update_bcs(int nbnd, int nnode, int 
                                                       const int* __restrict__ he_tag,
                                                       const double* __restrict__ he_a, int nbnode,
                                                       const int* __restrict__ bc_type,
-                                                      const double* __restrict__ qref, double gamma,
+                                                      const double* __restrict__ qref,
+                                                      const double* __restrict__ twall,
+                                                      const int* __restrict__ he_nrm, double gamma,
                                                       double* __restrict__ q) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
@@ -98,7 +90,21 @@ __global__ void __launch_bounds__(kBlock) k_update_bcs(int nbnd, int nnode, int 
     }
     double pref = qref[5 * ntag + tag];
     const int type = bc_type[tag];
-    if (is_dirichlet(type)) {
+    if (type == BC_NOSLIP) {
+      // hard-set wall node (adiabatic walls read the wall-normal neighbour
+      // as it stands before this update)
+      const double tw = twall[tag];
+      const int nn = he_nrm[h];
+      if (tw < 0.0 && nn < 0) continue;
+      double nQ[5] = {0, 0, 0, 0, 0};
+      if (tw < 0.0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) nQ[k] = q[k * ntot + nn];
+      }
+      noslip_state(QL, QR, tw, nQ, gamma);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) q[k * ntot + ph] = QR[k];
+    } else if (is_dirichlet(type)) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
         QL[k] = Qr[k];
@@ -118,7 +124,7 @@ void launch_update_bcs(Ctx& c) {
   if (c.nbnd == 0) return;
   hipLaunchKernelGGL(k_update_bcs, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.nnode, c.ntot,
                      c.ntag, c.d.bnd_nodes, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
-                     c.d.qref, c.prm.gamma, c.d.q);
+                     c.d.qref, c.d.twall, c.d.he_nrm, c.prm.gamma, c.d.q);
   PCFD_HIP(hipGetLastError());
 }
 
@@ -150,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g
                                                     const double* __restrict__ he_a, int nbnode,
                                                     const int* __restrict__ bc_type, double gamma,
                                                     double* __restrict__ grad) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   const double gm1 = gamma - 1.0;
   double gr[NT * 3];
@@ -244,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
                                                    const double* __restrict__ xyz, const double* __restrict__ vol,
                                                    const double* __restrict__ q, const double* __restrict__ grad,
                                                    double* __restrict__ lim2) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double qi[5], qmin[5], qmax[5], phi[5], gi[15];
 #pragma unroll
@@ -326,7 +332,7 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
                                                          const double* __restrict__ q,
                                                          const double* __restrict__ grad,
                                                          const double* __restrict__ lim2, double* __restrict__ lim) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   bool zero = false;
   double qi[5], li[5];
@@ -395,7 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
                                                     const int* __restrict__ he_ptr, const int* __restrict__ he_id,
                                                     const double* __restrict__ he_a, int nbnode,
                                                     double* __restrict__ b, double* __restrict__ red) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int blk = xcd_block();
+  int i = blk * blockDim.x + threadIdx.x;
   double bi[5] = {0, 0, 0, 0, 0};
   if (i < nnode) {
     double qi[5], gi[15], li[5], xi[3];
@@ -467,28 +474,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
 #pragma unroll
     for (int k = 0; k < 5; ++k) b[k * nnode + i] = bi[k];
   }
-  // block partial sums of b^2, fixed-shape tree -> deterministic
-  __shared__ double sh[6][kBlock];
-  double tot = 0.0;
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    double v = bi[k] * bi[k];
-    sh[1 + k][threadIdx.x] = v;
-    tot += v;
-  }
-  sh[0][threadIdx.x] = tot;
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-#pragma unroll
-      for (int k = 0; k < 6; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k) red[k * gridDim.x + blockIdx.x] = sh[k][0];
-  }
+  block_norm_partials(bi, red, blk);
 }
 
 // final reduction of the per-block partial sums (single block, fixed order)
@@ -523,8 +509,9 @@ void launch_residual(Ctx& c, int ho) {
                        c.nbnode, c.d.b, c.d.red);
   }
   PCFD_HIP(hipGetLastError());
-  sub_end(c, 10);
   c.nred = nb;
+  launch_viscous(c, ho);
+  sub_end(c, 10);
 }
 
 // residGlobal = sqrt(sum b^2)/(nnode*neqn), per-equation sqrt(sum b_k^2)/nnode

@@ -33,6 +33,11 @@ struct JacArgs {
   const double *vol, *dt;
   double* q;
   const int *srow, *sbase;
+  const double* xyz;       // SoA, for the viscous directional weights
+  const double* twall;     // per tag, noSlip walls
+  const int* he_nrm;       // per half-edge, wall-normal neighbour (noSlip)
+  int viscous;
+  ViscParams vp;
 };
 
 // Five lanes per block row: lane c of a row's group owns perturbation
@@ -78,6 +83,32 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
         Qr[k] = A.qref[k * A.ntag + tag];
       }
       const double pref = A.qref[5 * A.ntag + tag];
+      if (type == BC_NOSLIP) {
+        // CalculateBoundaryVariables hard-sets the wall node for the base
+        // and for every perturbed state (ucs/jacobian.tcc:484-512)
+        const double tw = A.twall[tag];
+        const int nn = A.he_nrm[hh];
+        if (tw < 0.0 && nn < 0) continue;
+        double nQ[5] = {0, 0, 0, 0, 0};
+        if (tw < 0.0) {
+#pragma unroll
+          for (int k = 0; k < 5; ++k) nQ[k] = A.q[k * A.ntot + nn];
+        }
+        noslip_state(QL, QR, tw, nQ, gamma);
+        if (c == 0) {
+#pragma unroll
+          for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
+        }
+        double fS[5], fL[5], QPL[5];
+        roe_flux_fast(QL, QR, a, gamma, A.sym2d, fS);
+        perturb(QL, c, h, QPL);
+        double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
+        noslip_state(QPL, QPR, tw, nQ, gamma);
+        roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) / h;
+        continue;
+      }
       if (is_dirichlet(type)) {
         // QL = QR = Qref and every perturbed evaluation resets both to Qref,
         // so the reference's contribution is exactly zero
@@ -108,7 +139,7 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
   for (int rr = 0; rr < 5; ++rr) Db[(long long)t * 25 + rr * 5 + c] = Dc[rr];
 }
 
-template <typename T>
+template <typename T, bool VISC>
 __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __restrict__ bidx,
                                                     const double* __restrict__ Db, T* __restrict__ Moff,
                                                     T* __restrict__ Dg) {
@@ -129,9 +160,14 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) Dc[rr] = Db[(long long)bt * 25 + rr * 5 + c];
   }
-  double qi[5];
+  double qi[5], xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int k = 0; k < 5; ++k) qi[k] = A.q[k * A.ntot + i];
+  if (VISC) {
+    xi[0] = A.xyz[i];
+    xi[1] = A.xyz[A.nnode + i];
+    xi[2] = A.xyz[2 * A.nnode + i];
+  }
   // --- interior edges (Kernel_NumJac): row blocks + diagonal column sums ---
   const long long gb = (long long)A.gbase[i >> 6] * kW + (i & 63);
   const int w = A.gwidth[i >> 6];
@@ -156,6 +192,9 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     // the three evaluations are retired one after another (scheduler
     // fences keep them from being interleaved, which would triple the
     // live registers)
+    // offc: column c of this row's block; colc: column c of the other
+    // row's block (this row's diagonal gets its negative)
+    double offc[5], colc[5];
     __builtin_amdgcn_sched_barrier(0);
     perturb(QL, c, h, QP);
     roe_side(QP, a, gamma, SP);
@@ -163,8 +202,8 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) {
       const double aRL = (fS[rr] - fP[rr]) / h;  // A[R][L] = -dF/dQL
-      if (left) Dc[rr] += -aRL;
-      else blk[rr * 5 * kW] = (T)aRL;
+      if (left) colc[rr] = aRL;
+      else offc[rr] = aRL;
     }
     __builtin_amdgcn_sched_barrier(0);
     perturb(QR, c, h, QP);
@@ -173,10 +212,28 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) {
       const double aLR = (fP[rr] - fS[rr]) / h;  // A[L][R] = +dF/dQR
-      if (left) blk[rr * 5 * kW] = (T)aLR;
-      else Dc[rr] += -aLR;
+      if (left) offc[rr] = aLR;
+      else colc[rr] = aLR;
     }
     __builtin_amdgcn_sched_barrier(0);
+    if (VISC) {
+      // Kernel_Viscous_Jac (ucs/jacobian.tcc:734-767): A[R][L] += aL,
+      // A[L][R] += aR, added before the column sums
+      const double xj[3] = {A.xyz[j], A.xyz[A.nnode + j], A.xyz[2 * A.nnode + j]};
+      double vL[5], vR[5];
+      visc_jac_cols(c, QL, QR, left ? xi : xj, left ? xj : xi, a, A.vp, vL, vR);
+#pragma unroll
+      for (int rr = 0; rr < 5; ++rr) {
+        offc[rr] += left ? vR[rr] : vL[rr];
+        colc[rr] += left ? vL[rr] : vR[rr];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 5; ++rr) {
+      blk[rr * 5 * kW] = (T)offc[rr];
+      Dc[rr] += -colc[rr];
+    }
   }
   // --- temporal term, steady pseudo-time: A_ii += 1*vol/dtau ---
   const double vt = 1.0 * A.vol[i] / A.dt[i];
@@ -188,27 +245,105 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
   for (int rr = 0; rr < 5; ++rr) dg[rr * 5 * kW] = (T)Dc[rr];
 }
 
-// Dirichlet / sonic-inflow overwrite of boundary node states, deferred until
-// all rows have read the pre-update states (the reference applies it inside
-// Bkernel_NumJac after Kernel_NumJac has run).
-__global__ void __launch_bounds__(kBlock) k_commit_dirichlet(int nbnd, int ntot, int ntag,
-                                                            const int* __restrict__ bnd_nodes,
-                                                            const int* __restrict__ he_ptr,
-                                                            const int* __restrict__ he_id,
-                                                            const int* __restrict__ he_tag,
-                                                            const int* __restrict__ bc_type,
-                                                            const double* __restrict__ qref, double* __restrict__ q) {
+// Dirichlet / sonic-inflow and noSlip overwrites of boundary node states,
+// deferred until all rows have read the pre-update states (the reference
+// applies them inside Bkernel_NumJac after Kernel_NumJac has run).  The
+// half-edges of a node are replayed in order, as in k_jac_boundary.
+__global__ void __launch_bounds__(kBlock) k_commit_hardset(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
   int i = bnd_nodes[t];
-  int last = -1;
-  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
-    int tag = he_tag[he_id[p]];
-    if (is_dirichlet(bc_type[tag])) last = tag;
-  }
-  if (last >= 0) {
+  double QL[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q[k * ntot + i] = qref[k * ntag + last];
+  for (int k = 0; k < 5; ++k) QL[k] = A.q[k * A.ntot + i];
+  bool set = false;
+  for (int p = A.he_ptr[i]; p < A.he_ptr[i + 1]; ++p) {
+    const int hh = A.he_id[p];
+    const int tag = A.he_tag[hh];
+    const int type = A.bc_type[tag];
+    if (is_dirichlet(type)) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) QL[k] = A.qref[k * A.ntag + tag];
+      set = true;
+    } else if (type == BC_NOSLIP) {
+      const double tw = A.twall[tag];
+      const int nn = A.he_nrm[hh];
+      if (tw < 0.0 && nn < 0) continue;
+      double nQ[5] = {0, 0, 0, 0, 0}, QR[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        QR[k] = A.q[k * A.ntot + A.nnode + hh];
+        if (tw < 0.0) nQ[k] = A.q[k * A.ntot + nn];
+      }
+      noslip_state(QL, QR, tw, nQ, A.gamma);
+      set = true;
+    }
+  }
+  if (set) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) A.q[k * A.ntot + i] = QL[k];
+  }
+}
+
+// Bkernel_BC_Jac_Modify -> ModifyViscousWallJacobian (ucs/bc.tcc:747-900,
+// ucs/compressible.tcc:1577-1606) after the temporal terms: per noSlip
+// half-edge, BlankSubRow (ucs/crsmatrix.tcc:525-541) zeroes sub-row r of
+// every block of the row and puts 1 on the diagonal; isothermal walls then
+// tie the energy change to the density change, adiabatic walls tie density
+// and energy to the wall-normal neighbour.  One thread per boundary node.
+template <typename T>
+__device__ __forceinline__ void blank_subrow(T* Moff, T* Dg, const int* scol, long long sb, int w, int chunk,
+                                             int lane, int sub) {
+  for (int s = 0; s < w; ++s) {
+    if (scol[(sb + s) * kW + lane] < 0) break;
+    T* blk = Moff + (sb + s) * 25 * kW + lane;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) blk[(sub * 5 + k) * kW] = (T)0;
+  }
+  T* dg = Dg + (long long)chunk * 25 * kW + lane;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) dg[(sub * 5 + k) * kW] = (T)0;
+  dg[(sub * 5 + sub) * kW] = (T)1;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_noslip_rows(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd,
+                                                       const int* __restrict__ spos, const int* __restrict__ swidth,
+                                                       const int* __restrict__ scol, T* __restrict__ Moff,
+                                                       T* __restrict__ Dg) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbnd) return;
+  const int i = bnd_nodes[t];
+  const int r = spos[i];
+  const int chunk = r >> 6, lane = r & 63;
+  const long long sb = A.sbase[chunk];
+  const int w = swidth[chunk];
+  const double g = A.gamma;
+  for (int p = A.he_ptr[i]; p < A.he_ptr[i + 1]; ++p) {
+    const int hh = A.he_id[p];
+    const int tag = A.he_tag[hh];
+    if (A.bc_type[tag] != BC_NOSLIP) continue;
+    const double tw = A.twall[tag];
+    blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 1);
+    blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 2);
+    blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 3);
+    if (tw < 0.0) {
+      blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 0);
+      blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 4);
+      const int nn = A.he_nrm[hh];
+      for (int s = 0; nn >= 0 && s < w; ++s) {
+        const int j = scol[(sb + s) * kW + lane];
+        if (j < 0) break;
+        if (j != nn) continue;
+        T* blk = Moff + (sb + s) * 25 * kW + lane;
+        blk[0 * kW] = (T)-1.0;
+        blk[24 * kW] = (T)-1.0;
+      }
+    } else {
+      const double v2 = 0.0;
+      blank_subrow(Moff, Dg, scol, sb, w, chunk, lane, 4);
+      Dg[(long long)chunk * 25 * kW + lane + 20 * kW] = (T)(-(tw / (g * (g - 1.0)) + 0.5 * v2));
+    }
   }
 }
 
@@ -216,7 +351,8 @@ void launch_jacobian(Ctx& c) {
   const pcfd_params& p = c.prm;
   JacArgs A{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase, c.d.gcol,
             c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type, c.d.qref, c.d.vol,
-            c.d.dt, c.d.q, c.d.srow, c.d.sbase};
+            c.d.dt, c.d.q, c.d.srow, c.d.sbase, c.d.xyz, c.d.twall, c.d.he_nrm, p.viscous,
+            ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d}};
   sub_begin(c);
   if (c.nbnd > 0) {
     const long long bw = (c.nbnd + 11) / 12;
@@ -226,16 +362,31 @@ void launch_jacobian(Ctx& c) {
   }
   const long long waves = (c.nrow_pad + 11) / 12;  // 12 rows x 5 lanes per wavefront
   const int grid = nblk(waves * kW);
-  if (p.precision == PCFD_CHOSEN_FP32)
-    hipLaunchKernelGGL(k_jacobian<float>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.Mf,
-                       c.d.Df);
+  const bool f32 = p.precision == PCFD_CHOSEN_FP32;
+  if (f32 && p.viscous)
+    hipLaunchKernelGGL((k_jacobian<float, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
+                       c.d.Mf, c.d.Df);
+  else if (f32)
+    hipLaunchKernelGGL((k_jacobian<float, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
+                       c.d.Mf, c.d.Df);
+  else if (p.viscous)
+    hipLaunchKernelGGL((k_jacobian<double, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
+                       c.d.Md, c.d.Dd);
   else
-    hipLaunchKernelGGL(k_jacobian<double>, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.Md,
-                       c.d.Dd);
+    hipLaunchKernelGGL((k_jacobian<double, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
+                       c.d.Md, c.d.Dd);
   PCFD_HIP(hipGetLastError());
   if (c.nbnd > 0) {
-    hipLaunchKernelGGL(k_commit_dirichlet, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.ntot, c.ntag,
-                       c.d.bnd_nodes, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.bc_type, c.d.qref, c.d.q);
+    hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd);
+    PCFD_HIP(hipGetLastError());
+  }
+  if (c.has_noslip) {
+    if (p.precision == PCFD_CHOSEN_FP32)
+      hipLaunchKernelGGL(k_noslip_rows<float>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes,
+                         c.nbnd, c.d.spos, c.d.swidth, c.d.scol, c.d.Mf, c.d.Df);
+    else
+      hipLaunchKernelGGL(k_noslip_rows<double>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes,
+                         c.nbnd, c.d.spos, c.d.swidth, c.d.scol, c.d.Md, c.d.Dd);
     PCFD_HIP(hipGetLastError());
   }
   sub_end(c, 11);

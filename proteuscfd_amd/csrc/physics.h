@@ -351,6 +351,195 @@ PCFD_DEV void wall_state(const double* QL, double pL, double* QR, const double* 
 // phantom store picked up the pre-update interior state).
 PCFD_DEV bool is_dirichlet(int type) { return type == BC_SONIC_INFLOW || type == BC_DIRICHLET; }
 
+// noSlip wall at rest (GetViscousWallBoundaryVariables, ucs/compressible.tcc:
+// 1544-1575, vel = 0, no bleed steps): the wall node itself is hard-set.
+// Isothermal (twall >= 0): energy from the wall temperature; adiabatic
+// (twall < 0): density and energy of the wall-normal neighbour nQ.
+PCFD_DEV void noslip_state(double* QL, double* QR, double twall, const double* nQ, double gamma) {
+  const double vel = 0.0;
+  if (twall < 0.0) {
+    QR[0] = QL[0] = nQ[0];
+    QR[4] = QL[4] = nQ[4];
+  } else {
+    const double v2 = vel * vel + vel * vel + vel * vel;
+    const double gm1 = gamma - 1.0;
+    const double rhoEt = (twall * QL[0] / (gamma * gm1) + 0.5 * QR[0] * v2);
+    QR[0] = QL[0];
+    QR[4] = QL[4] = rhoEt;
+  }
+  QR[1] = QL[1] = QL[0] * vel;
+  QR[2] = QL[2] = QL[0] * vel;
+  QR[3] = QL[3] = QL[0] * vel;
+}
+
+// T, u, v, w of ComputeAuxiliaryVariables (ucs/compressible.tcc:1229-1243,
+// T = gamma*P/rho through ComputePressure, :1090-1110), same operation order
+PCFD_DEV void aux_tuvw(const double* Q, double gamma, double* t) {
+  const double gm1 = gamma - 1.0;
+  const double r = Q[0];
+  const double uu = Q[1] / r, vv = Q[2] / r, ww = Q[3] / r;
+  const double v2h = 0.5 * (uu * uu + vv * vv + ww * ww);
+  const double p = gm1 * (Q[4] - r * v2h);
+  t[0] = gamma * p / r;
+  t[1] = uu;
+  t[2] = vv;
+  t[3] = ww;
+}
+
+// Sutherland's law, non-dimensional (ComputeViscosity, ucs/eqnset.h:251-268)
+PCFD_DEV double sutherland(double T, double S) { return (1.0 + S) * pow(T, 1.5) / (T + S); }
+
+struct ViscParams {
+  double gamma, re_tilde, pr, prt, suth;
+  int sym2d;
+};
+
+// ViscousFlux (ucs/compressible.tcc:712-795), laminar (mut = 0), written on
+// the right-hand side.  Q: averaged conservative state, T its temperature,
+// g[12] = gradients of T, u, v, w (x, y, z each).
+PCFD_DEV void viscous_flux(const double* Q, double T, const double* g, const double* a, const ViscParams& vp,
+                           double* F) {
+  const double mut = 0.0;
+  const double Tx = g[0], Ty = g[1], Tz = g[2];
+  const double ux = g[3], uy = g[4], uz = g[5];
+  const double vx = g[6], vy = g[7], vz = g[8];
+  const double wx = g[9], wy = g[10], wz = g[11];
+  const double rho = Q[0];
+  const double u = Q[1] / rho, v = Q[2] / rho, w = Q[3] / rho;
+  const double mu = sutherland(T, vp.suth);
+  const double tmut = (mu + mut);
+  const double fact = -2.0 / 3.0 * (ux + vy + wz);
+  const double tauxx = 2.0 * ux + fact;
+  const double tauyy = 2.0 * vy + fact;
+  const double tauzz = 2.0 * wz + fact;
+  const double tauxy = uy + vx;
+  const double tauxz = uz + wx;
+  const double tauyz = vz + wy;
+  const double RK = a[3] / vp.re_tilde;
+  const double RKT = RK * tmut;
+  const double cp = 1.0 / (vp.gamma - 1.0);
+  const double k = mu / vp.pr * cp;
+  const double kT = mut / vp.prt * cp;
+  const double c1 = -(k + kT);
+  const double Tn = Tx * a[0] + Ty * a[1] + Tz * a[2];
+  const double tauxn = tauxx * a[0] + tauxy * a[1] + tauxz * a[2];
+  const double tauyn = tauxy * a[0] + tauyy * a[1] + tauyz * a[2];
+  const double tauzn = tauxz * a[0] + tauyz * a[1] + tauzz * a[2];
+  F[0] = 0.0;
+  F[1] = -RKT * (tauxn);
+  F[2] = -RKT * (tauyn);
+  F[3] = -RKT * (tauzn);
+  F[4] = -RKT * (tauxn * u + tauyn * v + tauzn * w) + RK * c1 * Tn;
+  if (vp.sym2d == 1) F[1] = 0.0;
+  else if (vp.sym2d == 2) F[2] = 0.0;
+  else if (vp.sym2d == 3) F[3] = 0.0;
+}
+
+// Column c of one side of ViscousJacobian (ucs/compressible.tcc:1632-1893)
+// before the left side's sign flip.  Dx: that side's directional weights
+// (+-dx/(rho s2)); u, v, w: edge-averaged velocity; Tn: that side's
+// temperature-gradient weight; rho, uS, vS, wS, P: that side's state.
+PCFD_DEV void visc_jac_col(int c, const double* Dx, const double* a, double u, double v, double w, double RK,
+                           double RKT, double Tn, double gm1, double rhoS, double uS, double vS, double wS,
+                           double PS, double* col) {
+  const double c43 = 4.0 / 3.0, mc23 = -2.0 / 3.0;
+  col[0] = 0.0;
+  if (c == 0) {
+    const double dux = -u * Dx[0], duy = -u * Dx[1], duz = -u * Dx[2];
+    const double dvx = -v * Dx[0], dvy = -v * Dx[1], dvz = -v * Dx[2];
+    const double dwx = -w * Dx[0], dwy = -w * Dx[1], dwz = -w * Dx[2];
+    const double dfact = -2.0 / 3.0 * (dux + dvy + dwz);
+    const double dtauxx = (2.0 * dux + dfact);
+    const double dtauyy = (2.0 * dvy + dfact);
+    const double dtauzz = (2.0 * dwz + dfact);
+    const double dtauxy = duy + dvx;
+    const double dtauxz = duz + dwx;
+    const double dtauyz = dvz + dwy;
+    const double dtauxn = dtauxx * a[0] + dtauxy * a[1] + dtauxz * a[2];
+    const double dtauyn = dtauxy * a[0] + dtauyy * a[1] + dtauyz * a[2];
+    const double dtauzn = dtauxz * a[0] + dtauyz * a[1] + dtauzz * a[2];
+    const double v2 = (uS * uS + vS * vS + wS * wS);
+    const double dP_dr = +gm1 * 0.5 * v2;
+    col[1] = -RKT * (dtauxn);
+    col[2] = -RKT * (dtauyn);
+    col[3] = -RKT * (dtauzn);
+    col[4] = -RKT * (dtauxn * u + dtauyn * v + dtauzn * w) + RK * Tn * (dP_dr - PS / rhoS);
+  } else if (c == 4) {
+    col[1] = 0.0;
+    col[2] = 0.0;
+    col[3] = 0.0;
+    col[4] = RK * Tn * (+gm1);
+  } else {
+    double d2, d3, d4, dP;
+    if (c == 1) {
+      d2 = (c43 * Dx[0] * a[0] + Dx[1] * a[1] + Dx[2] * a[2]);
+      d3 = (mc23 * Dx[0] * a[1] + Dx[1] * a[0]);
+      d4 = (mc23 * Dx[0] * a[2] + Dx[2] * a[0]);
+      dP = -gm1 * uS;
+    } else if (c == 2) {
+      d2 = (mc23 * Dx[1] * a[0] + Dx[0] * a[1]);
+      d3 = (Dx[0] * a[0] + c43 * Dx[1] * a[1] + Dx[2] * a[2]);
+      d4 = (mc23 * Dx[1] * a[2] + Dx[2] * a[1]);
+      dP = -gm1 * vS;
+    } else {
+      d2 = (mc23 * Dx[2] * a[0] + Dx[0] * a[2]);
+      d3 = (mc23 * Dx[2] * a[1] + Dx[1] * a[2]);
+      d4 = (Dx[0] * a[0] + Dx[1] * a[1] + c43 * Dx[2] * a[2]);
+      dP = -gm1 * wS;
+    }
+    col[1] = -RKT * d2;
+    col[2] = -RKT * d3;
+    col[3] = -RKT * d4;
+    col[4] = -RKT * (d2 * u + d3 * v + d4 * w) + RK * Tn * dP;
+  }
+}
+
+// Columns c of both sides of ViscousJacobian for the edge (QL, QR), left
+// side already sign-flipped; symmetry2D rows zeroed.
+PCFD_DEV void visc_jac_cols(int c, const double* QL, const double* QR, const double* xL, const double* xR,
+                            const double* a, const ViscParams& vp, double* colL, double* colR) {
+  const double gamma = vp.gamma, gm1 = gamma - 1.0;
+  double Qavg[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Qavg[k] = 0.5 * (QL[k] + QR[k]);
+  double ta[4];
+  aux_tuvw(Qavg, gamma, ta);
+  const double mu = sutherland(ta[0], vp.suth);
+  const double tmut = (mu + 0.0);
+  const double RK = a[3] / vp.re_tilde;
+  const double RKT = RK * tmut;
+  const double rhoL = QL[0], uL = QL[1] / rhoL, vL = QL[2] / rhoL, wL = QL[3] / rhoL;
+  const double rhoR = QR[0], uR = QR[1] / rhoR, vR = QR[2] / rhoR, wR = QR[3] / rhoR;
+  const double PL = aux_pressure(QL, gm1), PR = aux_pressure(QR, gm1);
+  const double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
+  double dx[3], s2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    dx[k] = (xR[k] - xL[k]);
+    s2 += dx[k] * dx[k];
+  }
+  double DxL[3], DxR[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    DxL[k] = -dx[k] / (rhoL * s2);
+    DxR[k] = dx[k] / (rhoR * s2);
+  }
+  const double cp = 1.0 / (vp.gamma - 1.0);
+  const double kk = mu / vp.pr * cp;
+  const double kT = 0.0 / vp.prt * cp;
+  const double c1 = -(kk + kT);
+  const double TnL = (DxL[0] * a[0] + DxL[1] * a[1] + DxL[2] * a[2]) * (gamma / rhoL) * c1;
+  const double TnR = (DxR[0] * a[0] + DxR[1] * a[1] + DxR[2] * a[2]) * (gamma / rhoR) * c1;
+  visc_jac_col(c, DxR, a, u, v, w, RK, RKT, TnR, gm1, rhoR, uR, vR, wR, PR, colR);
+  visc_jac_col(c, DxL, a, u, v, w, RK, RKT, TnL, gm1, rhoL, uL, vL, wL, PL, colL);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) colL[k] = -colL[k];
+  if (vp.sym2d >= 1 && vp.sym2d <= 3) {
+    colL[vp.sym2d] = 0.0;
+    colR[vp.sym2d] = 0.0;
+  }
+}
+
 // CalculateBoundaryVariables for the non-Dirichlet BC types on this path
 // (ucs/bc.tcc:1058-1398).  QL is read only.  Returns false for types it does
 // not handle (the phantom is then left untouched).

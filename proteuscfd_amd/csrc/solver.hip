@@ -50,7 +50,7 @@ void iterate(Ctx& c, double* resid) {
   launch_update_bcs(c);
   phase_end(c, 0);
   const bool ho = p.sorder > 1 && c.iter > p.nfirst;
-  if (ho) {
+  if (ho || p.viscous) {  // ucs/solutionSpace.tcc:671-677
     phase_begin(c);
     launch_gradient(c, p.gradient_terms);
     phase_end(c, 1);

@@ -27,7 +27,7 @@ PHASES = ["bcs", "gradient", "limiter", "residual", "timestep", "jacobian", "pre
 class MeshDesc(C.Structure):
     _fields_ = [("nnode", C.c_int), ("nbnode", C.c_int), ("nedge", C.c_int), ("edges", ip), ("edge_a", dp),
                 ("bedges", ip), ("bedge_a", dp), ("bfactag", ip), ("xyz", dp), ("vol", dp), ("nfactag", C.c_int),
-                ("bc_type", ip), ("qref", dp)]
+                ("bc_type", ip), ("qref", dp), ("twall", dp)]
 
 
 class CParams(C.Structure):
@@ -35,7 +35,8 @@ class CParams(C.Structure):
                 ("limiter", C.c_int), ("venkat_K", C.c_double), ("limiter_refresh", C.c_int),
                 ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_ramp", C.c_int),
                 ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int),
-                ("gradient_terms", C.c_int)]
+                ("gradient_terms", C.c_int), ("viscous", C.c_int), ("re_tilde", C.c_double),
+                ("prandtl", C.c_double), ("prandtl_t", C.c_double), ("sutherland_s", C.c_double)]
 
 
 SYMBOLS = [
@@ -114,7 +115,9 @@ def _i32(a):
 
 def to_cparams(p: Params, gradient_terms: int = 9) -> CParams:
     return CParams(p.gamma, p.sorder, p.nfirst, p.chi, p.limiter, p.venkat_K, p.limiter_refresh, p.limiter_freeze,
-                   p.cfl, p.cfl_start, p.cfl_ramp, p.nsgs, p.symmetry2d, p.jac_freq, p.precision, gradient_terms)
+                   p.cfl, p.cfl_start, p.cfl_ramp, p.nsgs, p.symmetry2d, p.jac_freq, p.precision,
+                   9 if p.viscous else gradient_terms, int(p.viscous), p.re_tilde, p.prandtl, p.prandtl_t,
+                   p.sutherland_s)
 
 
 class Context:
@@ -126,11 +129,12 @@ class Context:
         self.nnode, self.nbnode = m.nnode, m.nbedge
         self._keep = dict(edges=_i32(m.edges), edge_a=_f64(m.edge_a), bedges=_i32(m.bedges),
                           bedge_a=_f64(m.bedge_a), bfactag=_i32(m.bfactag), xyz=_f64(m.xyz), vol=_f64(m.vol),
-                          bc_type=_i32(case.bc_type), qref=_f64(case.qref[:, :5]))
+                          bc_type=_i32(case.bc_type), qref=_f64(case.qref[:, :5]),
+                          twall=_f64(case.twall if case.twall is not None else np.full(len(case.bc_type), -1.0)))
         k = self._keep
         desc = MeshDesc(m.nnode, m.nbedge, m.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
                         _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), len(k["bc_type"]),
-                        _i(k["bc_type"]), _d(k["qref"]))
+                        _i(k["bc_type"]), _d(k["qref"]), _d(k["twall"]))
         h = C.c_void_p()
         rc = lib().pcfd_create(C.byref(desc), C.byref(h))
         if rc != 0:
