@@ -1,6 +1,6 @@
 """Build recipe for the CPU oracle (test infrastructure only).
 
-Compiles oracle.c into oracle/liboracle.so with gcc.  FP contraction is off
+Compiles oracle.c and oracle_chem.c into oracle/liboracle.so with gcc.  FP contraction is off
 so the oracle evaluates every product and sum exactly as written, like the
 reference's separate multiplies and adds.
 
@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 def build() -> str:
     out = os.path.join(HERE, "liboracle.so")
     cmd = ["gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-std=c11", "-fPIC", "-shared",
-           os.path.join(HERE, "oracle.c"), "-o", out + ".tmp", "-lm"]
+           os.path.join(HERE, "oracle.c"), os.path.join(HERE, "oracle_chem.c"), "-o", out + ".tmp", "-lm"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)

@@ -109,6 +109,35 @@ int o_lu(double* a, int* piv, int n);
 void o_lu_solve(const double* a, double* b, const int* piv, double* x, int n);
 void o_apply_dq(const double* dq, double* Q, double gamma);
 
+/* Finite-rate chemistry model (ucs/chem.tcc, ucs/reaction.tcc,
+ * ucs/species.tcc): per reaction the local species list in the reaction
+ * parser's order with stoichiometric coefficients and third-body
+ * efficiencies. */
+#define O_CHEM_MAXS 8
+#define O_CHEM_MAXR 16
+typedef struct {
+  int ns, nr;
+  double mw[O_CHEM_MAXS];                       /* kg/mol */
+  double lo[O_CHEM_MAXS][7], hi[O_CHEM_MAXS][7];  /* NASA-7, T <= 1000 K / T > 1000 K */
+  int type[O_CHEM_MAXR];                        /* 0 Arrhenius 1 ModArrhenius 2 GuptaModArrhenius 3 Power */
+  int tb[O_CHEM_MAXR];                          /* third bodies present */
+  double A[O_CHEM_MAXR], EA[O_CHEM_MAXR], n[O_CHEM_MAXR];
+  int nloc[O_CHEM_MAXR];
+  int loc[O_CHEM_MAXR][O_CHEM_MAXS];            /* global species of local slot */
+  double nup[O_CHEM_MAXR][O_CHEM_MAXS], nupp[O_CHEM_MAXR][O_CHEM_MAXS], eff[O_CHEM_MAXR][O_CHEM_MAXS];
+  double ref_density, ref_time, ref_temperature;
+} o_chem;
+
+double o_chem_kf(const o_chem* c, int r, double T);
+double o_chem_kc(const o_chem* c, int r, double T);
+/* GetMassProductionRates: rhoi (kg/m^3), T (K) -> wdot (kg/m^3/s) */
+void o_chem_wdot(const o_chem* c, const double* rhoi, double T, double* wdot);
+/* SourceTerm on the FR state (rho_i.., u, v, w, T; non-dimensional) */
+void o_chem_source(const o_chem* c, const double* Q, double vol, double* source);
+/* SourceTermJacobian by one-sided differences, h = 1e-8 */
+void o_chem_source_jacobian(const o_chem* c, const double* Q, double vol, double* A);
+double o_chem_cp(const o_chem* c, int s, double T);
+
 /* Full solver state for the steady Newton / explicit loop. */
 typedef struct {
   const o_mesh* m;

@@ -46,6 +46,7 @@ struct Dev {
   int* bnd_nodes = nullptr; // nodes with >= 1 half-edge
   int* bidx = nullptr;      // nnode: index into bnd_nodes, -1 for nodes without half-edges
   double* jdb = nullptr;    // 25 x nbnd boundary part of the Jacobian diagonal
+  double* jqb = nullptr;    // 5 x nbnd boundary-node states after the Jacobian's BC re-evaluation
   // boundary-condition tables per face tag
   int* bc_type = nullptr;   // ntag
   double* qref = nullptr;   // 5 x ntag (conservative), aux pressure at [5*ntag + tag]

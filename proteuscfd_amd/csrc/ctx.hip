@@ -41,7 +41,7 @@ void free_all(Ctx& c) {
   Dev& d = c.d;
   dfree(d.xyz); dfree(d.vol); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
   dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
-  dfree(d.jdb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
+  dfree(d.jdb); dfree(d.jqb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
   dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
   dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.x8d); dfree(d.x8f); dfree(d.red);
@@ -355,6 +355,7 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     dput(c, &d.bnd_nodes, bnd);
     dput(c, &d.bidx, bidx);
     dalloc(c, &d.jdb, 25 * (size_t)std::max(c.nbnd, 1));
+    dalloc(c, &d.jqb, 5 * (size_t)std::max(c.nbnd, 1));
     dput(c, &d.bc_type, c.h_bc_type);
     dput(c, &d.qref, qref);
     dput(c, &d.twall, twall);

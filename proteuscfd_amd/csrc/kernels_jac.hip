@@ -55,7 +55,7 @@ __device__ __forceinline__ void perturb(const double* Q, int c, double h, double
 // diagonal block to Db[t*25 + r*5 + c] (t = boundary-node index) and the
 // re-evaluated phantom states (lane 0).
 __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd,
-                                                        double* __restrict__ Db) {
+                                                        double* __restrict__ Db, double* __restrict__ Qb) {
   const int lane64 = threadIdx.x & 63;
   if (lane64 >= 60) return;
   const int t = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 12 + lane64 / 5;
@@ -134,6 +134,12 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) / h;
     }
+    // the node state after the re-evaluation: the viscous Jacobian, which
+    // the reference runs after Bkernel_NumJac, sees it
+    if (c == 0) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) Qb[(long long)t * 5 + k] = QL[k];
+    }
   }
 #pragma unroll
   for (int rr = 0; rr < 5; ++rr) Db[(long long)t * 25 + rr * 5 + c] = Dc[rr];
@@ -141,8 +147,8 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
 
 template <typename T, bool VISC>
 __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __restrict__ bidx,
-                                                    const double* __restrict__ Db, T* __restrict__ Moff,
-                                                    T* __restrict__ Dg) {
+                                                    const double* __restrict__ Db, const double* __restrict__ Qb,
+                                                    T* __restrict__ Moff, T* __restrict__ Dg) {
   const int lane64 = threadIdx.x & 63;
   if (lane64 >= 60) return;
   const int r = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * 12 + lane64 / 5;
@@ -218,10 +224,19 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     __builtin_amdgcn_sched_barrier(0);
     if (VISC) {
       // Kernel_Viscous_Jac (ucs/jacobian.tcc:734-767): A[R][L] += aL,
-      // A[L][R] += aR, added before the column sums
+      // A[L][R] += aR, added before the column sums; it runs after the
+      // boundary Jacobian has hard-set wall / Dirichlet nodes, so boundary
+      // nodes enter with their re-evaluated states
       const double xj[3] = {A.xyz[j], A.xyz[A.nnode + j], A.xyz[2 * A.nnode + j]};
+      double vi[5], vj[5];
+      const int bj = bidx[j];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        vi[k] = bt >= 0 ? Qb[(long long)bt * 5 + k] : qi[k];
+        vj[k] = bj >= 0 ? Qb[(long long)bj * 5 + k] : qj[k];
+      }
       double vL[5], vR[5];
-      visc_jac_cols(c, QL, QR, left ? xi : xj, left ? xj : xi, a, A.vp, vL, vR);
+      visc_jac_cols(c, left ? vi : vj, left ? vj : vi, left ? xi : xj, left ? xj : xi, a, A.vp, vL, vR);
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) {
         offc[rr] += left ? vR[rr] : vL[rr];
@@ -247,42 +262,15 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 
 // Dirichlet / sonic-inflow and noSlip overwrites of boundary node states,
 // deferred until all rows have read the pre-update states (the reference
-// applies them inside Bkernel_NumJac after Kernel_NumJac has run).  The
-// half-edges of a node are replayed in order, as in k_jac_boundary.
-__global__ void __launch_bounds__(kBlock) k_commit_hardset(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd) {
+// applies them inside Bkernel_NumJac after Kernel_NumJac has run): the
+// states k_jac_boundary ended with are copied into q.
+__global__ void __launch_bounds__(kBlock) k_commit_hardset(int nbnd, int ntot, const int* __restrict__ bnd_nodes,
+                                                          const double* __restrict__ Qb, double* __restrict__ q) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
-  int i = bnd_nodes[t];
-  double QL[5];
+  const int i = bnd_nodes[t];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) QL[k] = A.q[k * A.ntot + i];
-  bool set = false;
-  for (int p = A.he_ptr[i]; p < A.he_ptr[i + 1]; ++p) {
-    const int hh = A.he_id[p];
-    const int tag = A.he_tag[hh];
-    const int type = A.bc_type[tag];
-    if (is_dirichlet(type)) {
-#pragma unroll
-      for (int k = 0; k < 5; ++k) QL[k] = A.qref[k * A.ntag + tag];
-      set = true;
-    } else if (type == BC_NOSLIP) {
-      const double tw = A.twall[tag];
-      const int nn = A.he_nrm[hh];
-      if (tw < 0.0 && nn < 0) continue;
-      double nQ[5] = {0, 0, 0, 0, 0}, QR[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        QR[k] = A.q[k * A.ntot + A.nnode + hh];
-        if (tw < 0.0) nQ[k] = A.q[k * A.ntot + nn];
-      }
-      noslip_state(QL, QR, tw, nQ, A.gamma);
-      set = true;
-    }
-  }
-  if (set) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) A.q[k * A.ntot + i] = QL[k];
-  }
+  for (int k = 0; k < 5; ++k) q[k * ntot + i] = Qb[(long long)t * 5 + k];
 }
 
 // Bkernel_BC_Jac_Modify -> ModifyViscousWallJacobian (ucs/bc.tcc:747-900,
@@ -357,7 +345,7 @@ void launch_jacobian(Ctx& c) {
   if (c.nbnd > 0) {
     const long long bw = (c.nbnd + 11) / 12;
     hipLaunchKernelGGL(k_jac_boundary, dim3(nblk(bw * kW)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd,
-                       c.d.jdb);
+                       c.d.jdb, c.d.jqb);
     PCFD_HIP(hipGetLastError());
   }
   const long long waves = (c.nrow_pad + 11) / 12;  // 12 rows x 5 lanes per wavefront
@@ -365,19 +353,20 @@ void launch_jacobian(Ctx& c) {
   const bool f32 = p.precision == PCFD_CHOSEN_FP32;
   if (f32 && p.viscous)
     hipLaunchKernelGGL((k_jacobian<float, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.Mf, c.d.Df);
+                       c.d.jqb, c.d.Mf, c.d.Df);
   else if (f32)
     hipLaunchKernelGGL((k_jacobian<float, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.Mf, c.d.Df);
+                       c.d.jqb, c.d.Mf, c.d.Df);
   else if (p.viscous)
     hipLaunchKernelGGL((k_jacobian<double, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.Md, c.d.Dd);
+                       c.d.jqb, c.d.Md, c.d.Dd);
   else
     hipLaunchKernelGGL((k_jacobian<double, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.Md, c.d.Dd);
+                       c.d.jqb, c.d.Md, c.d.Dd);
   PCFD_HIP(hipGetLastError());
   if (c.nbnd > 0) {
-    hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd);
+    hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.ntot,
+                       c.d.bnd_nodes, c.d.jqb, c.d.q);
     PCFD_HIP(hipGetLastError());
   }
   if (c.has_noslip) {

@@ -272,3 +272,95 @@ class Context:
 
     def synchronize(self):
         self._chk(lib().pcfd_synchronize(self.h), "pcfd_synchronize")
+
+
+# ---------------------------------------------------------------------------
+# finite-rate chemistry (include/pcfd_chem.h)
+
+class ChemDesc(C.Structure):
+    _fields_ = [("nspecies", C.c_int), ("nreactions", C.c_int), ("mw", dp), ("nasa_lo", dp), ("nasa_hi", dp),
+                ("rxn_type", ip), ("third_body", ip), ("A", dp), ("EA", dp), ("n", dp), ("nloc", ip), ("loc", ip),
+                ("nup", dp), ("nupp", dp), ("tb_eff", dp), ("ref_density", C.c_double), ("ref_time", C.c_double),
+                ("ref_temperature", C.c_double)]
+
+
+CHEM_SYMBOLS = ["pcfd_chem_create", "pcfd_chem_destroy", "pcfd_chem_last_error", "pcfd_chem_wdot",
+                "pcfd_chem_source", "pcfd_chem_source_device", "pcfd_chem_last_kernel_ms"]
+
+
+def _chem_lib():
+    L = lib()
+    if not getattr(L, "_chem_ready", False):
+        vp = C.c_void_p
+        L.pcfd_chem_create.argtypes = [C.POINTER(ChemDesc), C.POINTER(vp)]
+        L.pcfd_chem_destroy.argtypes = [vp]
+        L.pcfd_chem_last_error.argtypes = [vp]
+        L.pcfd_chem_last_error.restype = C.c_char_p
+        L.pcfd_chem_wdot.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.pcfd_chem_source.argtypes = [vp, C.c_int, dp, dp, dp, dp]
+        L.pcfd_chem_source_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.pcfd_chem_last_kernel_ms.argtypes = [vp]
+        L.pcfd_chem_last_kernel_ms.restype = C.c_double
+        L._chem_ready = True
+    return L
+
+
+class ChemContext:
+    """A chemistry model on the current HIP device (proteuscfd_amd.chem.ChemModel)."""
+
+    def __init__(self, model):
+        L = _chem_lib()
+        self.model = model
+        MAXLOC = 8
+        k = dict(mw=_f64(model.mw), lo=_f64(model.lo.reshape(-1)), hi=_f64(model.hi.reshape(-1)),
+                 rt=_i32(model.rtype), tb=_i32(model.tb), A=_f64(model.A), EA=_f64(model.EA), n=_f64(model.n),
+                 nloc=_i32(model.nloc), loc=_i32(model.loc[:, :MAXLOC].reshape(-1)),
+                 nup=_f64(model.nup[:, :MAXLOC].reshape(-1)), nupp=_f64(model.nupp[:, :MAXLOC].reshape(-1)),
+                 eff=_f64(model.eff[:, :MAXLOC].reshape(-1)))
+        self._keep = k
+        d = ChemDesc(model.ns, model.nr, _d(k["mw"]), _d(k["lo"]), _d(k["hi"]), _i(k["rt"]), _i(k["tb"]),
+                     _d(k["A"]), _d(k["EA"]), _d(k["n"]), _i(k["nloc"]), _i(k["loc"]), _d(k["nup"]), _d(k["nupp"]),
+                     _d(k["eff"]), model.ref_density, model.ref_time, model.ref_temperature)
+        h = C.c_void_p()
+        rc = L.pcfd_chem_create(C.byref(d), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pcfd_chem_create failed ({rc}): {L.pcfd_chem_last_error(None).decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {_chem_lib().pcfd_chem_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _chem_lib().pcfd_chem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def wdot(self, rhoi, T):
+        rhoi = _f64(rhoi)
+        T = _f64(T)
+        w = np.zeros_like(rhoi)
+        self._chk(_chem_lib().pcfd_chem_wdot(self.h, len(T), _d(rhoi), _d(T), _d(w)), "pcfd_chem_wdot")
+        return w
+
+    def source(self, q, vol, jacobian=True):
+        q = _f64(q)
+        vol = _f64(vol)
+        n, neqn = q.shape
+        s = np.zeros((n, neqn))
+        J = np.zeros((n, neqn, neqn)) if jacobian else None
+        self._chk(_chem_lib().pcfd_chem_source(self.h, n, _d(q), _d(vol), _d(s), _d(J) if jacobian else None),
+                  "pcfd_chem_source")
+        return (s, J) if jacobian else s
+
+    def source_device(self, q_ptr, vol_ptr, src_ptr, jac_ptr, n):
+        """Device pointers (e.g. torch tensor data_ptr()); returns kernel ms."""
+        self._chk(_chem_lib().pcfd_chem_source_device(self.h, n, q_ptr, vol_ptr, src_ptr, jac_ptr),
+                  "pcfd_chem_source_device")
+        return _chem_lib().pcfd_chem_last_kernel_ms(self.h)

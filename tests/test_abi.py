@@ -16,7 +16,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(pcfd_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("lib,headers", [("libpcfd.so", ["pcfd.h", "pcfd_mesh.h"]),
+@pytest.mark.parametrize("lib,headers", [("libpcfd.so", ["pcfd.h", "pcfd_mesh.h", "pcfd_chem.h"]),
                                          ("libpcfd_mesh.so", ["pcfd_mesh.h"])])
 def test_library_exports_declared_symbols(lib, headers):
     path = os.path.join(ROOT, "proteuscfd_amd", lib)
@@ -31,6 +31,7 @@ def test_library_exports_declared_symbols(lib, headers):
 def test_python_binding_covers_the_abi():
     from proteuscfd_amd import gpu
     assert sorted(gpu.SYMBOLS) == declared("pcfd.h")
+    assert sorted(gpu.CHEM_SYMBOLS) == declared("pcfd_chem.h")
 
 
 def test_version_string_without_gpu():

@@ -1,0 +1,113 @@
+"""Pins for the oracle's finite-rate chemistry (5-species air).
+
+Known answers from the reference's own unit test (N2 Cp at 300 K and
+5500 K, R and MW: unitTest/chemTest.h:74-85); element conservation of the
+molar production rates (exact for every reaction); the equilibrium
+constant against the Gibbs energies H/RT - S/R of the NASA-7 fits
+written out independently; the forward rate law at a point; the source
+term's non-dimensionalisation and its Jacobian against a central
+difference.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import chem
+
+UNIV_R = 8.31447215
+MODEL = chem.load_air5()
+N, N2, O2, NO, O_ = range(5)
+
+
+def test_species_thermo_known_answers():
+    c = O.Chem(MODEL)
+    assert abs(c.cp(N2, 300.0) - 1039.6) < 1e-1
+    assert abs(c.cp(N2, 5500.0) - 1363.6) < 1e-1
+    assert abs(UNIV_R / MODEL.mw[N2] - 296.8) < 1e-1
+    assert abs(MODEL.mw[N2] - 0.0280134) < 1e-7
+
+
+def rand_mix(rng):
+    rho = np.exp(rng.uniform(np.log(1e-5), np.log(1.0)))
+    Y = rng.dirichlet(np.ones(5))
+    T = np.exp(rng.uniform(np.log(300.0), np.log(12000.0)))
+    return rho * Y, T
+
+
+def test_element_conservation():
+    """Molar production N: w_N/M_N + 2 w_N2/M_N2 + w_NO/M_NO = 0, O likewise."""
+    c = O.Chem(MODEL)
+    rng = np.random.default_rng(3)
+    mw = MODEL.mw
+    for _ in range(200):
+        rhoi, T = rand_mix(rng)
+        w = c.wdot(rhoi, T)
+        m = w / mw
+        scale = np.abs(m).max()
+        if scale == 0.0:
+            continue
+        assert abs(m[N] + 2 * m[N2] + m[NO]) <= 1e-13 * scale
+        assert abs(m[O_] + 2 * m[O2] + m[NO]) <= 1e-13 * scale
+
+
+def gibbs_rt(s, T):
+    """G/RT = H/RT - S/R of the NASA-7 fit (textbook forms)."""
+    a = MODEL.hi[s] if T > 1000.0 else MODEL.lo[s]
+    h = a[0] + a[1] * T / 2 + a[2] * T ** 2 / 3 + a[3] * T ** 3 / 4 + a[4] * T ** 4 / 5 + a[5] / T
+    sr = a[0] * np.log(T) + a[1] * T + a[2] * T ** 2 / 2 + a[3] * T ** 3 / 3 + a[4] * T ** 4 / 4 + a[6]
+    return h - sr
+
+
+@pytest.mark.parametrize("T", [400.0, 900.0, 1500.0, 4000.0, 9000.0])
+def test_equilibrium_constant_from_gibbs(T):
+    c = O.Chem(MODEL)
+    for r in range(MODEL.nr):
+        dg, dnu = 0.0, 0.0
+        for k in range(MODEL.nloc[r]):
+            s = MODEL.loc[r, k]
+            nu = MODEL.nupp[r, k] - MODEL.nup[r, k]
+            dg += nu * gibbs_rt(s, T)
+            dnu += nu
+        kp = np.exp(-dg)
+        kc = kp * (101325.0 / (UNIV_R * T)) ** dnu
+        np.testing.assert_allclose(c.kc(r, T), kc, rtol=1e-9)
+
+
+def test_forward_rate_law():
+    c = O.Chem(MODEL)
+    T = 5000.0
+    for r in range(MODEL.nr):
+        assert MODEL.rtype[r] == 2
+        np.testing.assert_allclose(c.kf(r, T), MODEL.A[r] * T ** MODEL.n[r] * np.exp(-MODEL.EA[r] / T), rtol=1e-14)
+
+
+def test_dissociation_direction():
+    """Hot pure O2 dissociates (w_O2 < 0, w_O > 0); cold O atoms recombine."""
+    c = O.Chem(MODEL)
+    w = c.wdot(np.array([0, 0, 1e-2, 0, 0.0]), 8000.0)
+    assert w[O2] < 0 < w[O_]
+    w = c.wdot(np.array([0, 0, 0, 0, 1e-2]), 1500.0)
+    assert w[O_] < 0 < w[O2]
+
+
+def test_source_term_scaling_and_jacobian():
+    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, [0, 0.767, 0.233, 0, 0])
+    m = MODEL.with_reference(rho_ref, t_ref, T_ref)
+    c = O.Chem(m)
+    rng = np.random.default_rng(5)
+    rhoi, T = rand_mix(rng)
+    T = 6500.0
+    Q = np.concatenate([rhoi / rho_ref, [0.3, -0.1, 0.2], [T / T_ref]])
+    vol = 0.37
+    s = c.source(Q, vol)
+    np.testing.assert_allclose(s[:5], vol * c.wdot(rhoi, T) / (rho_ref / t_ref), rtol=1e-13)
+    assert np.all(s[5:] == 0.0)
+    A = c.source_jacobian(Q, vol)
+    assert np.all(A[:, 5:8] == 0.0) and np.all(A[5:, :] == 0.0)
+    for i in list(range(5)) + [8]:
+        h = 1e-5 * max(abs(Q[i]), 1e-3)
+        qp, qm = Q.copy(), Q.copy()
+        qp[i] += h
+        qm[i] -= h
+        fd = (c.source(qp, vol) - c.source(qm, vol)) / (2 * h)
+        np.testing.assert_allclose(A[:5, i], fd[:5], rtol=2e-3, atol=1e-6 * np.abs(fd).max())
