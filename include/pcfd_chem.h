@@ -71,6 +71,60 @@ int pcfd_chem_source_device(pcfd_chem* c, int n, const double* q, const double* 
  * chemistry stream. */
 double pcfd_chem_last_kernel_ms(const pcfd_chem* c);
 
+/* ---- learned surrogate (no counterpart in the reference) -------------
+ * MLP 7 -> 64 -> 64 -> 5, tanh hidden layers, linear output; BF16 MFMA
+ * products with FP32 accumulation; FP32 master parameters, Adam.
+ * Parameter vector (float32, PCFD_MLP_NPARAM), tile-padded rows:
+ *   W1[64][16] (cols 7..15 zero), b1[64], W2[64][64], b2[64],
+ *   W3[16][64] (rows 5..15 zero), b3[16] (5..15 zero). */
+#define PCFD_MLP_NPARAM 6288
+typedef struct pcfd_mlp pcfd_mlp;
+
+int pcfd_mlp_create(unsigned long long seed, pcfd_mlp** out);  /* Glorot-uniform weights, zero biases */
+int pcfd_mlp_destroy(pcfd_mlp* m);
+const char* pcfd_mlp_last_error(const pcfd_mlp* m);
+double pcfd_mlp_last_kernel_ms(const pcfd_mlp* m);
+int pcfd_mlp_set_params(pcfd_mlp* m, const float* params);  /* also resets the Adam state */
+int pcfd_mlp_get_params(pcfd_mlp* m, float* params);
+/* y[n*5] = net(x[n*7]); host arrays / device arrays */
+int pcfd_mlp_forward(pcfd_mlp* m, int n, const float* x, float* y);
+int pcfd_mlp_forward_device(pcfd_mlp* m, int n, const float* x, float* y);
+/* mean squared error over n samples x 5 outputs and its gradient[NPARAM] */
+int pcfd_mlp_loss_grad(pcfd_mlp* m, int n, const float* x, const float* t, float* loss, float* grad);
+/* Adam on mini-batches drawn with replacement (splitmix64 from seed),
+ * cosine-decayed step size; the data set is copied to the device once.
+ * loss_history[steps] (may be NULL) receives each batch loss. */
+int pcfd_mlp_train(pcfd_mlp* m, int n, const float* x, const float* t, int steps, int batch, float lr,
+                   unsigned long long seed, float* loss_history);
+
+/* Feature/target transforms of the production-rate surrogate and its
+ * fallback rule (see pcfd_chem_surrogate_wdot_device). */
+typedef struct {
+  double lnT_mid, lnT_half;      /* f0 = (ln T - mid) / half */
+  double lnrho_mid, lnrho_half;  /* f1 = (ln rho - mid) / half */
+  double y_mid, y_scale;         /* f2..f6 = (g(Y_i) - y_mid) / y_scale, */
+  double y_eps;                  /*   g(Y) = ln(Y + y_eps) when y_eps > 0, else Y */
+  double rate_scale;             /* s0 [1/s]: target_i = asinh(wdot_i / (rho s0)) / sigma_i */
+  double sigma[5];
+  double T_min, T_max, rho_min, rho_max;  /* training box; outside it the exact rates are used */
+  double tol, floor;             /* element-balance bound: fall back when
+                                    imbalance > tol * (sum|w_i/M_i| + rho*floor) */
+} pcfd_surrogate_desc;
+
+/* Production rates by the surrogate for n cells (device arrays rhoi[n*5]
+ * kg/m^3, T[n] K -> wdot[n*5] kg/(m^3 s)), with the exact rates of `chem`
+ * for every cell the rule sends back; *nfallback counts those cells and
+ * *ms the device time of both passes. */
+int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n,
+                                    const double* rhoi, const double* T, double* wdot, int* nfallback, double* ms);
+
+/* the same on host arrays (copied in and out; *ms is device time only) */
+int pcfd_chem_surrogate_wdot(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n, const double* rhoi,
+                             const double* T, double* wdot, int* nfallback, double* ms);
+
+/* exact production rates on device arrays (the method without the surrogate) */
+int pcfd_chem_wdot_device(pcfd_chem* c, int n, const double* rhoi, const double* T, double* wdot);
+
 #ifdef __cplusplus
 }
 #endif

@@ -364,3 +364,116 @@ class ChemContext:
         self._chk(_chem_lib().pcfd_chem_source_device(self.h, n, q_ptr, vol_ptr, src_ptr, jac_ptr),
                   "pcfd_chem_source_device")
         return _chem_lib().pcfd_chem_last_kernel_ms(self.h)
+
+
+# ---------------------------------------------------------------------------
+# chemistry surrogate MLP (include/pcfd_chem.h, pcfd_mlp_*)
+
+MLP_NPARAM = 6288
+MLP_SYMBOLS = ["pcfd_mlp_create", "pcfd_mlp_destroy", "pcfd_mlp_last_error", "pcfd_mlp_last_kernel_ms",
+               "pcfd_mlp_set_params", "pcfd_mlp_get_params", "pcfd_mlp_forward", "pcfd_mlp_forward_device",
+               "pcfd_mlp_loss_grad", "pcfd_mlp_train", "pcfd_chem_surrogate_wdot_device", "pcfd_chem_wdot_device",
+               "pcfd_chem_surrogate_wdot"]
+CHEM_SYMBOLS += MLP_SYMBOLS
+
+fp = C.POINTER(C.c_float)
+
+
+class SurrogateDesc(C.Structure):
+    _fields_ = [("lnT_mid", C.c_double), ("lnT_half", C.c_double), ("lnrho_mid", C.c_double),
+                ("lnrho_half", C.c_double), ("y_mid", C.c_double), ("y_scale", C.c_double), ("y_eps", C.c_double),
+                ("rate_scale", C.c_double), ("sigma", C.c_double * 5), ("T_min", C.c_double), ("T_max", C.c_double),
+                ("rho_min", C.c_double), ("rho_max", C.c_double), ("tol", C.c_double), ("floor", C.c_double)]
+
+
+def _mlp_lib():
+    L = _chem_lib()
+    if not getattr(L, "_mlp_ready", False):
+        vp = C.c_void_p
+        L.pcfd_mlp_create.argtypes = [C.c_ulonglong, C.POINTER(vp)]
+        L.pcfd_mlp_destroy.argtypes = [vp]
+        L.pcfd_mlp_last_error.argtypes = [vp]
+        L.pcfd_mlp_last_error.restype = C.c_char_p
+        L.pcfd_mlp_last_kernel_ms.argtypes = [vp]
+        L.pcfd_mlp_last_kernel_ms.restype = C.c_double
+        L.pcfd_mlp_set_params.argtypes = [vp, fp]
+        L.pcfd_mlp_get_params.argtypes = [vp, fp]
+        L.pcfd_mlp_forward.argtypes = [vp, C.c_int, fp, fp]
+        L.pcfd_mlp_forward_device.argtypes = [vp, C.c_int, vp, vp]
+        L.pcfd_mlp_loss_grad.argtypes = [vp, C.c_int, fp, fp, fp, fp]
+        L.pcfd_mlp_train.argtypes = [vp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
+        L.pcfd_chem_surrogate_wdot_device.argtypes = [vp, vp, C.POINTER(SurrogateDesc), C.c_int, vp, vp, vp, ip,
+                                                      dp]
+        L.pcfd_chem_wdot_device.argtypes = [vp, C.c_int, vp, vp, vp]
+        L.pcfd_chem_surrogate_wdot.argtypes = [vp, vp, C.POINTER(SurrogateDesc), C.c_int, dp, dp, dp, ip, dp]
+        L._mlp_ready = True
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class MLP:
+    """The surrogate network on the current HIP device."""
+
+    def __init__(self, seed=0):
+        L = _mlp_lib()
+        h = C.c_void_p()
+        rc = L.pcfd_mlp_create(seed, C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pcfd_mlp_create failed ({rc}): {L.pcfd_mlp_last_error(None).decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {_mlp_lib().pcfd_mlp_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _mlp_lib().pcfd_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._chk(_mlp_lib().pcfd_mlp_get_params(self.h, p.ctypes.data_as(fp)), "pcfd_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = _f32(p)
+        self._chk(_mlp_lib().pcfd_mlp_set_params(self.h, p.ctypes.data_as(fp)), "pcfd_mlp_set_params")
+
+    def forward(self, x):
+        x = _f32(x)
+        y = np.zeros((x.shape[0], 5), dtype=np.float32)
+        self._chk(_mlp_lib().pcfd_mlp_forward(self.h, x.shape[0], x.ctypes.data_as(fp), y.ctypes.data_as(fp)),
+                  "pcfd_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x), _f32(t)
+        loss = C.c_float()
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._chk(_mlp_lib().pcfd_mlp_loss_grad(self.h, x.shape[0], x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                C.byref(loss), g.ctypes.data_as(fp)), "pcfd_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, steps, batch, lr, seed=1, history=True):
+        x, t = _f32(x), _f32(t)
+        hist = np.zeros(steps, dtype=np.float32) if history else None
+        self._chk(_mlp_lib().pcfd_mlp_train(self.h, x.shape[0], x.ctypes.data_as(fp), t.ctypes.data_as(fp), steps,
+                                            batch, lr, seed, hist.ctypes.data_as(fp) if history else None),
+                  "pcfd_mlp_train")
+        return hist
+
+    @property
+    def last_ms(self):
+        return _mlp_lib().pcfd_mlp_last_kernel_ms(self.h)

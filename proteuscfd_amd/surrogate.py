@@ -1,0 +1,138 @@
+"""Production-rate surrogate for 5-species air: data, transforms, training.
+
+The training set is made by the repository's own exact chemistry on the
+GPU (pcfd_chem_wdot): T log-uniform in [300, 12000] K, rho log-uniform in
+[1e-5, 1] kg/m^3, mass fractions from a symmetric Dirichlet(1).  Inputs
+are (ln T, ln rho, Y_1..Y_5) scaled to about [-1, 1]; targets are
+asinh(wdot_i / (rho s0)) / sigma_i, which compresses the ~90 decades the
+rates span across that box.  The network is trained on the GPU by the
+library's own MFMA forward/backward kernels (pcfd_mlp_train).  At run
+time pcfd_chem_surrogate_wdot_device evaluates it and sends every cell
+outside the box, or whose predicted rates break N/O element conservation
+by more than the bound, to the exact rates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+T_MIN, T_MAX = 300.0, 12000.0
+RHO_MIN, RHO_MAX = 1e-5, 1.0
+
+
+def sample_states(n, seed):
+    rng = np.random.default_rng(seed)
+    T = np.exp(rng.uniform(np.log(T_MIN), np.log(T_MAX), n))
+    rho = np.exp(rng.uniform(np.log(RHO_MIN), np.log(RHO_MAX), n))
+    Y = rng.dirichlet(np.ones(5), n)
+    return rho[:, None] * Y, T
+
+
+@dataclass
+class Transforms:
+    lnT_mid: float = 0.5 * (np.log(T_MIN) + np.log(T_MAX))
+    lnT_half: float = 0.5 * (np.log(T_MAX) - np.log(T_MIN))
+    lnrho_mid: float = 0.5 * (np.log(RHO_MIN) + np.log(RHO_MAX))
+    lnrho_half: float = 0.5 * (np.log(RHO_MAX) - np.log(RHO_MIN))
+    # mass fractions enter through ln(Y + 1e-4), mapped to about [-1, 1]:
+    # the rates are products of concentration powers
+    y_eps: float = 1e-4
+    y_mid: float = 0.5 * np.log(1e-4)
+    y_scale: float = -0.5 * np.log(1e-4)
+    rate_scale: float = 1.0
+    sigma: np.ndarray = field(default_factory=lambda: np.ones(5))
+
+    def features(self, rhoi, T):
+        rho = rhoi.sum(axis=1)
+        x = np.empty((len(T), 7), dtype=np.float32)
+        x[:, 0] = (np.log(T) - self.lnT_mid) / self.lnT_half
+        x[:, 1] = (np.log(rho) - self.lnrho_mid) / self.lnrho_half
+        Y = rhoi / rho[:, None]
+        g = np.log(Y + self.y_eps) if self.y_eps > 0 else Y
+        x[:, 2:] = (g - self.y_mid) / self.y_scale
+        return x
+
+    def targets(self, wdot, rhoi):
+        rho = rhoi.sum(axis=1)
+        return (np.arcsinh(wdot / (rho[:, None] * self.rate_scale)) / self.sigma).astype(np.float32)
+
+    def rates(self, y, rhoi):
+        rho = rhoi.sum(axis=1)
+        return rho[:, None] * self.rate_scale * np.sinh(y * self.sigma)
+
+    def fit_sigma(self, wdot, rhoi):
+        rho = rhoi.sum(axis=1)
+        self.sigma = np.maximum(np.arcsinh(wdot / (rho[:, None] * self.rate_scale)).std(axis=0), 1e-3)
+
+
+def desc(tr: Transforms, tol=0.05, floor=10.0):
+    from . import gpu
+    d = gpu.SurrogateDesc()
+    d.lnT_mid, d.lnT_half, d.lnrho_mid, d.lnrho_half = tr.lnT_mid, tr.lnT_half, tr.lnrho_mid, tr.lnrho_half
+    d.y_mid, d.y_scale, d.y_eps, d.rate_scale = tr.y_mid, tr.y_scale, tr.y_eps, tr.rate_scale
+    for k in range(5):
+        d.sigma[k] = float(tr.sigma[k])
+    d.T_min, d.T_max, d.rho_min, d.rho_max = T_MIN, T_MAX, RHO_MIN, RHO_MAX
+    d.tol, d.floor = tol, floor
+    return d
+
+
+def train_surrogate(chem_ctx, n=2_000_000, steps=4000, batch=4096, lr=3e-3, seed=40):
+    """Exact rates on the GPU -> transforms -> MLP trained on the GPU."""
+    from . import gpu
+    rhoi, T = sample_states(n, seed)
+    wdot = chem_ctx.wdot(rhoi, T)
+    tr = Transforms()
+    tr.fit_sigma(wdot, rhoi)
+    x, t = tr.features(rhoi, T), tr.targets(wdot, rhoi)
+    net = gpu.MLP(seed=seed)
+    hist = net.train(x, t, steps, batch, lr, seed=seed + 1)
+    return net, tr, hist
+
+
+def surrogate_wdot_device(chem_ctx, net, d, rhoi_ptr, T_ptr, wdot_ptr, n):
+    """Device pointers in and out; returns (fallback count, device ms)."""
+    from . import gpu
+    nf = C.c_int()
+    ms = C.c_double()
+    L = gpu._mlp_lib()
+    rc = L.pcfd_chem_surrogate_wdot_device(chem_ctx.h, net.h, C.byref(d), n, rhoi_ptr, T_ptr, wdot_ptr,
+                                           C.byref(nf), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"pcfd_chem_surrogate_wdot_device failed ({rc}): {L.pcfd_mlp_last_error(net.h).decode()}")
+    return nf.value, ms.value
+
+
+def surrogate_wdot(chem_ctx, net, d, rhoi, T):
+    """Host arrays; returns (wdot, fallback count, device ms of the kernels)."""
+    from . import gpu
+    rhoi = np.ascontiguousarray(rhoi, dtype=np.float64)
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    w = np.zeros_like(rhoi)
+    nf = C.c_int()
+    ms = C.c_double()
+    L = gpu._mlp_lib()
+    dp = C.POINTER(C.c_double)
+    rc = L.pcfd_chem_surrogate_wdot(chem_ctx.h, net.h, C.byref(d), len(T), rhoi.ctypes.data_as(dp),
+                                    T.ctypes.data_as(dp), w.ctypes.data_as(dp), C.byref(nf), C.byref(ms))
+    if rc != 0:
+        raise RuntimeError(f"pcfd_chem_surrogate_wdot failed ({rc}): {L.pcfd_mlp_last_error(net.h).decode()}")
+    return w, nf.value, ms.value
+
+
+def exact_wdot(chem_ctx, rhoi, T):
+    """Host arrays; returns (wdot, device ms of the exact kernel)."""
+    from . import gpu
+    w = chem_ctx.wdot(rhoi, T)
+    return w, gpu._mlp_lib().pcfd_chem_last_kernel_ms(chem_ctx.h)
+
+
+def exact_wdot_device(chem_ctx, rhoi_ptr, T_ptr, wdot_ptr, n):
+    from . import gpu
+    L = gpu._mlp_lib()
+    rc = L.pcfd_chem_wdot_device(chem_ctx.h, n, rhoi_ptr, T_ptr, wdot_ptr)
+    if rc != 0:
+        raise RuntimeError("pcfd_chem_wdot_device failed")
+    return L.pcfd_chem_last_kernel_ms(chem_ctx.h)
