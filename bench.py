@@ -1,6 +1,7 @@
 """Benchmark: time to an 8-order residual drop, chosen form (FP32 Jacobian +
 SGS, FP64 residual/gradients/norms) vs the plain FP64 form, on the 1.03M-node
-second-order Euler bump case, on one MI355X.
+second-order Euler bump case (default) or the laminar flat plate (--case
+plate, 8M cells by default), on one MI355X.
 
 A "step" here is one complete steady solve from freestream until
 residGlobal <= 1e-8 * residGlobal(iteration 1); --steps K times K solves of
@@ -33,7 +34,7 @@ def log(*a):
 def algorithmic_bytes(m, nblocks, prec_bytes):
     nn, ne, nb = m.nnode, m.nedge, m.nbedge
     return {
-        # SURVEY-defined compulsory bytes (indices int32, state FP64)
+        # compulsory bytes per launch (indices int32, state FP64)
         "residual_2nd": 264 * nn + 40 * ne + 84 * nb,
         "residual_1st": 80 * nn + 40 * ne + 84 * nb,
         "sgs_sweep": nblocks * 25 * prec_bytes + 8 * 15 * nn,
@@ -41,6 +42,26 @@ def algorithmic_bytes(m, nblocks, prec_bytes):
         "gradient9": nn * 8 * (10 + 6 + 3 + 27) + 8 * ne,
         "limiter": nn * 8 * (5 + 15 + 3 + 1 + 5) + 8 * ne,
     }
+
+
+def sgs_pass_bytes(m, nsgs, prec_bytes):
+    """Bytes the colour passes of one pcfd_sgs call must move, and the pass
+    counts.  A pass over colour c reads its rows' blocks (diagonal + one per
+    neighbour, 25 values each) and moves 15 values per row (b, the
+    neighbours' x, the new x), all in the storage precision.  The library
+    skips a pass that directly follows a pass over the same colour (it would
+    rewrite the same x), so nsgs symmetric sweeps over C colours run
+    2*nsgs*C - (2*nsgs - 1) passes."""
+    col, nc = m.colour()
+    deg = np.bincount(np.asarray(m.edges).ravel(), minlength=m.nnode)
+    blocks = np.bincount(col, weights=deg + 1.0, minlength=nc)
+    rows = np.bincount(col, minlength=nc).astype(float)
+    seq = []
+    for _ in range(nsgs):
+        seq += list(range(nc)) + list(range(nc - 1, -1, -1))
+    run = [c for k, c in enumerate(seq) if k == 0 or seq[k - 1] != c]
+    nbytes = sum(blocks[c] * 25 * prec_bytes + rows[c] * 15 * prec_bytes for c in run)
+    return float(nbytes), len(run), len(seq)
 
 
 def phase_profile(ctx, case, iters):
@@ -133,6 +154,10 @@ def main():
                     help="run a device copy of BYTES first (for counter scaling)")
     ap.add_argument("--venkat-k", type=float, default=5.0, help="modified-Venkat K (the workload's K=5)")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--case", choices=["bump", "plate"], default="bump",
+                    help="bump: second-order Euler bump (the headline workload); plate: laminar flat plate")
+    ap.add_argument("--plate", default="200,40,180", help="plate hexes nx,ny,nz (the lowest 20 become prism layers)")
+    ap.add_argument("--cfl", type=float, default=0.0, help="final CFL (default 50 bump, 100 plate)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -147,10 +172,30 @@ def main():
     from proteuscfd_amd.cases import Params
 
     t0 = time.time()
-    base = dict(sorder=2, limiter=3, venkat_K=args.venkat_k, cfl=50.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
-    case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
-    m = case.mesh
-    log(f"[bench] mesh bump {args.n}^3: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
+    if args.case == "bump":
+        base = dict(sorder=2, limiter=3, venkat_K=args.venkat_k, cfl=args.cfl or 50.0, cfl_start=1.0, cfl_ramp=20,
+                    nsgs=8)
+        case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
+        m = case.mesh
+        workload = (f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
+                    f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
+                    f"M=0.5 farfield, weighted LSQ (9 terms), modified Venkat K={args.venkat_k:g}, "
+                    f"first-order FD Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 20 iterations")
+    else:
+        nx, ny, nz = (int(v) for v in args.plate.split(","))
+        base = dict(sorder=2, limiter=0, cfl=args.cfl or 100.0, cfl_start=1.0, cfl_ramp=100, nsgs=8, viscous=1,
+                    reynolds=1.0e4, mach_ref=0.3)
+        case = cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=20, first_h=1e-4, growth=1.15, twall=1.0,
+                                top_bc=cases.BC_SYMMETRY, params=Params(**base, precision=0))
+        m = case.mesh
+        ncell = nx * ny * (20 * 2 + (nz - 20) * 6)
+        workload = (f"second-order laminar Navier-Stokes flat-plate channel, Re=1e4 per unit length, Pr=0.72, "
+                    f"Sutherland, M=0.3, isothermal noSlip floor T=1; {nx}x{ny}x{nz} hexes: 20 prism "
+                    f"layers (first 1e-4, growth 1.15) under a Kuhn-tet core = {ncell} cells, {m.nnode} nodes, "
+                    f"{m.nedge} edges; farfield inflow/outflow, symmetry top and sides; unlimited MUSCL, "
+                    f"first-order FD Jacobian + analytic viscous Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 100 "
+                    f"iterations")
+    log(f"[bench] mesh {args.case}: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
         f"({time.time() - t0:.1f}s)")
 
     results = {}
@@ -197,11 +242,14 @@ def main():
     pp, pc = plain["prof"], chosen["prof"]
     nsgs = base["nsgs"]
     sweeps = 2 * nsgs
+    sgs64, npass, npass_full = sgs_pass_bytes(m, nsgs, 8)
+    sgs32, _, _ = sgs_pass_bytes(m, nsgs, 4)
     # kernel-only times (HIP events on the library stream around the launches)
     kern = {
         "residual_2nd_fp64": dict(ms=pp["k_residual"], gbs=gbs(ab64["residual_2nd"], pp["k_residual"])),
-        "sgs_sweep_fp64": dict(ms=pp["k_sgs_sweeps"] / sweeps, gbs=gbs(ab64["sgs_sweep"], pp["k_sgs_sweeps"] / sweeps)),
-        "sgs_sweep_fp32": dict(ms=pc["k_sgs_sweeps"] / sweeps, gbs=gbs(ab32["sgs_sweep"], pc["k_sgs_sweeps"] / sweeps)),
+        # per symmetric sweep (2 per SGS iteration); bytes of the passes actually run
+        "sgs_sweep_fp64": dict(ms=pp["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs64, pp["k_sgs_sweeps"])),
+        "sgs_sweep_fp32": dict(ms=pc["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs32, pc["k_sgs_sweeps"])),
         "jacobian_fp64": dict(ms=pp["k_jacobian"], gbs=gbs(ab64["jacobian"], pp["k_jacobian"])),
         "jacobian_fp32": dict(ms=pc["k_jacobian"], gbs=gbs(ab32["jacobian"], pc["k_jacobian"])),
         "gradient9": dict(ms=pp["gradient"], gbs=gbs(ab64["gradient9"], pp["gradient"])),
@@ -229,7 +277,7 @@ def main():
         cpu = {
             "value": it_s * n_iter, "unit": "s (estimated: measured s/iteration x GPU plain iteration count)",
             "cores": 1, "kind": "port",
-            "sample": f"1 implicit iteration of the same 1.03M-node case ({it_s:.1f} s: jacobian "
+            "sample": f"1 implicit iteration of the same {m.nnode}-node case ({it_s:.1f} s: jacobian "
                       f"{tm['jacobian']:.1f} s, residual {tm['residual']:.2f} s, gradient+limiter "
                       f"{tm['gradient']:.2f} s, SGS {tm['linear']:.1f} s) x {n_iter} iterations",
             "edge_flux_per_s": (m.nedge + m.nbedge) / tm["residual"] if tm["residual"] > 0 else None,
@@ -251,11 +299,7 @@ def main():
         "dtype": "f64 residual/gradients/norms, f32 Jacobian+SGS (plain path all f64)",
         "data": "synthetic",
         "config": {
-            "workload": f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
-                        f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
-                        f"M=0.5 farfield, weighted LSQ (9 terms), modified Venkat K={args.venkat_k:g}, "
-                        f"first-order FD Jacobian, "
-                        f"8 SGS, CFL 1->50 over 20 iterations",
+            "workload": workload,
             "step": "one full steady solve from freestream to an 8-order residual drop",
             "parallelism": "single GPU",
         },
@@ -270,6 +314,7 @@ def main():
         "per_iteration_ms": per_iter_ms,
         "phase_ms": {"plain": pp, "chosen": pc},
         "kernels": kern,
+        "sgs_colour_passes": {"run": npass, "of": npass_full, "ncolor": plain["info"]["ncolor"]},
         "roofline": {"bound": "hbm", "kernel": dom_key, "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["frac"], "traffic": traffic},
         "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true>", "achieved": kern["residual_2nd_fp64"]["gbs"],

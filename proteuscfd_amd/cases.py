@@ -136,7 +136,7 @@ def bump_case(n=100, seed=2, jitter=0.03, mach=0.5, gamma=1.4, rcm=True,
 
 def plate_case(nx=100, ny=20, nz=40, nlayers=20, first_h=1e-4, growth=1.15, seed=3, jitter=0.0, mach=0.3,
                reynolds=1.0e4, twall=1.0, gamma=1.4, rcm=True, incidence_deg=0.0, outflow_bc=BC_FARFIELD,
-               params: Params | None = None) -> Case:
+               top_bc=BC_FARFIELD, params: Params | None = None) -> Case:
     """Laminar flat-plate channel: prism layers on a noSlip isothermal floor
     (twall = T_wall / T_ref, 1 = freestream temperature) under a Kuhn-tet
     core; farfield inflow, outflow and top, symmetry side planes.  Re is
@@ -144,7 +144,7 @@ def plate_case(nx=100, ny=20, nz=40, nlayers=20, first_h=1e-4, growth=1.15, seed
     incidence_deg tilts the freestream towards +z: with 0 the flow is exactly
     tangential to the top farfield face, where the characteristic branch
     (inflow / outflow, theta == 0) is decided by the sign of round-off.
-    outflow_bc sets the x-max face; the reference farfield returns early
+    outflow_bc sets the x-max face and top_bc the z-max face; the reference farfield returns early
     when the averaged normal velocity is exactly 0, which happens at the
     outflow corner of the no-slip floor (zero wall velocity against a
     phantom whose velocity the fixed point drove to 0) unless round-off
@@ -153,7 +153,7 @@ def plate_case(nx=100, ny=20, nz=40, nlayers=20, first_h=1e-4, growth=1.15, seed
                      jitter=jitter, rcm=int(rcm))
     a = np.deg2rad(incidence_deg)
     q0 = qinf(gamma, mach, direction=(np.cos(a), 0.0, np.sin(a)))
-    bct = np.array([BC_PARALLEL, BC_FARFIELD, outflow_bc, BC_SYMMETRY, BC_SYMMETRY, BC_NOSLIP, BC_FARFIELD],
+    bct = np.array([BC_PARALLEL, BC_FARFIELD, outflow_bc, BC_SYMMETRY, BC_SYMMETRY, BC_NOSLIP, top_bc],
                    dtype=np.int32)
     qref = np.tile(q0, (M.NFACTAG, 1))
     tw = np.full(M.NFACTAG, -1.0)

@@ -335,39 +335,40 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   bool zero = false;
-  double qi[5], li[5];
+  // The node's own state, gradient and limiter stay in registers; only the
+  // neighbour side is gathered.  Both extrapolations are written relative to
+  // node i: dQ = qj - qi and dx = (xj - xi)/2 give bit-identical states to
+  // the reference's (L,R) form, because a - b == -(b - a) and 0.5*(-d) ==
+  // -(0.5*d) exactly in IEEE arithmetic.
+  double qi[5], li[5], gi[15];
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     qi[k] = q[k * ntot + i];
     li[k] = lim2[k * nnode + i];
   }
-  double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
+#pragma unroll
+  for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
+  const double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
   const long long b = gbase_of(g, i);
   const int w = g.width[i >> 6];
-  for (int s = 0; s < w; ++s) {
+  for (int s = 0; s < w && !zero; ++s) {
     const int j = g.col[b + (long long)s * kW];
     if (j < 0) break;
-    const bool left = i < j;
-    double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
-    const int L = left ? i : j, R = left ? j : i;
-    const double* xL = left ? xi : xj;
-    const double* xR = left ? xj : xi;
-    double dx[3] = {0.5 * (xR[0] - xL[0]), 0.5 * (xR[1] - xL[1]), 0.5 * (xR[2] - xL[2])};
-    double mdx[3] = {-dx[0], -dx[1], -dx[2]};
-    double QL[5], QR[5];
+    const double dx[3] = {0.5 * (xyz[j] - xi[0]), 0.5 * (xyz[nnode + j] - xi[1]), 0.5 * (xyz[2 * nnode + j] - xi[2])};
+    const double mdx[3] = {-dx[0], -dx[1], -dx[2]};
+    double Qi[5], Qj[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      double qL = q[k * ntot + L], qR = q[k * ntot + R];
-      double dQ = qR - qL;
-      QL[k] = extrap1(qL, dQ, grad[(3 * k) * nnode + L], grad[(3 * k + 1) * nnode + L],
-                      grad[(3 * k + 2) * nnode + L], dx, chi, lim2[k * nnode + L]);
-      QR[k] = extrap1(qR, -dQ, grad[(3 * k) * nnode + R], grad[(3 * k + 1) * nnode + R],
-                      grad[(3 * k + 2) * nnode + R], mdx, chi, lim2[k * nnode + R]);
+      const double qj = q[k * ntot + j];
+      const double dQ = qj - qi[k];
+      Qi[k] = extrap1(qi[k], dQ, gi[3 * k], gi[3 * k + 1], gi[3 * k + 2], dx, chi, li[k]);
+      Qj[k] = extrap1(qj, -dQ, grad[(3 * k) * nnode + j], grad[(3 * k + 1) * nnode + j],
+                      grad[(3 * k + 2) * nnode + j], mdx, chi, lim2[k * nnode + j]);
     }
-    if (left && bad_state_r(QL, gamma)) zero = true;
-    if (!left && bad_state_r(QR, gamma)) zero = true;
+    if (bad_state_r(Qi, gamma)) zero = true;
     double Qr[5];
-    roe_state(QL, QR, gamma, Qr);
+    if (i < j) roe_state(Qi, Qj, gamma, Qr);
+    else roe_state(Qj, Qi, gamma, Qr);
     if (bad_state_r(Qr, gamma)) zero = true;
   }
 #pragma unroll
@@ -428,25 +429,21 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
       for (int k = 0; k < 5; ++k) qj[k] = q[k * ntot + j];
       double QL[5], QR[5];
       if (HO) {
-        double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
-        const double* xL = left ? xi : xj;
-        const double* xR = left ? xj : xi;
-        double dx[3] = {0.5 * (xR[0] - xL[0]), 0.5 * (xR[1] - xL[1]), 0.5 * (xR[2] - xL[2])};
-        // own end: extrapolate toward the edge midpoint; neighbour end: same
+        // both ends extrapolated relative to node i (dQ = qj - qi, dx =
+        // (xj - xi)/2), then assigned to L/R: the same bits as the (L,R)
+        // form because a - b == -(b - a) and 0.5*(-d) == -(0.5*d) in IEEE
+        // arithmetic, with two selects per variable instead of eight
+        const double dx[3] = {0.5 * (xyz[j] - xi[0]), 0.5 * (xyz[nnode + j] - xi[1]),
+                              0.5 * (xyz[2 * nnode + j] - xi[2])};
+        const double mdx[3] = {-dx[0], -dx[1], -dx[2]};
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-          const double qL = left ? qi[k] : qj[k];
-          const double qR = left ? qj[k] : qi[k];
-          const double dQ = qR - qL;
-          const double gj0 = grad[(3 * k) * nnode + j], gj1 = grad[(3 * k + 1) * nnode + j],
-                       gj2 = grad[(3 * k + 2) * nnode + j];
-          const double lj = lim[k * nnode + j];
-          const double gL0 = left ? gi[3 * k] : gj0, gL1 = left ? gi[3 * k + 1] : gj1, gL2 = left ? gi[3 * k + 2] : gj2;
-          const double gR0 = left ? gj0 : gi[3 * k], gR1 = left ? gj1 : gi[3 * k + 1], gR2 = left ? gj2 : gi[3 * k + 2];
-          const double lL = left ? li[k] : lj, lR = left ? lj : li[k];
-          double mdx[3] = {-dx[0], -dx[1], -dx[2]};
-          QL[k] = extrap1(qL, dQ, gL0, gL1, gL2, dx, chi, lL);
-          QR[k] = extrap1(qR, -dQ, gR0, gR1, gR2, mdx, chi, lR);
+          const double dQ = qj[k] - qi[k];
+          const double Qi = extrap1(qi[k], dQ, gi[3 * k], gi[3 * k + 1], gi[3 * k + 2], dx, chi, li[k]);
+          const double Qj = extrap1(qj[k], -dQ, grad[(3 * k) * nnode + j], grad[(3 * k + 1) * nnode + j],
+                                    grad[(3 * k + 2) * nnode + j], mdx, chi, lim[k * nnode + j]);
+          QL[k] = left ? Qi : Qj;
+          QR[k] = left ? Qj : Qi;
         }
       } else {
 #pragma unroll

@@ -57,12 +57,15 @@ __global__ void __launch_bounds__(kBlock) k_pack_b(int nrow_pad, int nnode, cons
   for (int k = 0; k < 5; ++k) o[k * kW] = (i >= 0) ? (T)b[k * nnode + i] : (T)0;
 }
 
-// Four lanes per row: lanes {l, l+16, l+32, l+48} of a wavefront share a row
-// and take every fourth slot, so a colour launch has four times as many
-// waves in flight; their partial sums are combined with a fixed butterfly
-// (every lane ends with the same value) before lane group 0 solves the
-// diagonal block.  A wavefront covers 16 consecutive SELL rows.
-constexpr int kTPR = 4;
+// Two lanes per row: lanes l and l+32 of a wavefront share a row and take
+// alternate slots, so a colour launch has twice as many waves in flight;
+// their partial sums are combined with one fixed exchange (both lanes end
+// with the same value) before lane l solves the diagonal block.  A
+// wavefront covers 32 consecutive SELL rows, so each block-entry load of a
+// lane half reads 32 consecutive rows' values: a full 128-byte line in FP32.
+// (Measured on the 1.03M-node bump: 2 lanes/row beats 4 by 4% in FP32 and
+// 1.4% in FP64, and 8 lanes/row is 25% slower.)
+constexpr int kTPR = 2;
 
 // Block-matrix entries are read once per sweep.  FP64 blocks are loaded
 // non-temporally (they then do not evict the gathered x lines from L2); for
@@ -80,8 +83,8 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
                                                      const T* __restrict__ Dg, const int* __restrict__ piv,
                                                      const T* __restrict__ bs, T* __restrict__ x8) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int sub = (threadIdx.x & 63) >> 4;
-  const int local = (t >> 6) * 16 + (threadIdx.x & 15);
+  const int sub = (threadIdx.x & 63) >> 5;
+  const int local = (t >> 6) * 32 + (threadIdx.x & 31);
   const bool inr = local < nrows;
   const int r = row0 + (inr ? local : 0);
   const int i = inr ? srow[r] : -1;
@@ -110,7 +113,6 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
   }
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
-    acc[k] += __shfl_xor(acc[k], 16);
     acc[k] += __shfl_xor(acc[k], 32);
   }
   if (i < 0 || sub != 0) return;
@@ -176,6 +178,11 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
                      c.d.b, bs);
   PCFD_HIP(hipGetLastError());
   double xold = 0.0, xn = 0.0;
+  // A colour pass directly after a pass over the same colour (the turn of
+  // each symmetric sweep, and the start of the next forward sweep) reads the
+  // same neighbour x and writes the same values bit for bit: the rows of a
+  // colour never read each other.  Those passes are skipped.
+  int last = -1;
   for (int it = 0; it < nsgs; ++it) {
     sub_begin(c);
     for (int dir = 0; dir < 2; ++dir) {
@@ -183,7 +190,8 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
         const int col = dir == 0 ? cc : c.ncolor - 1 - cc;
         const int row0 = c.color_chunk[col] * kW;
         const int nrows = (c.color_chunk[col + 1] - c.color_chunk[col]) * kW;
-        if (nrows == 0) continue;
+        if (nrows == 0 || col == last) continue;
+        last = col;
         hipLaunchKernelGGL(k_sgs_color<T>, dim3(nblk((long long)nrows * kTPR)), dim3(kBlock), 0, c.stream, row0,
                            nrows, c.d.srow, c.d.swidth, c.d.sbase, c.d.scol, Moff, Dg, c.d.piv, bs, x8);
         PCFD_HIP(hipGetLastError());
