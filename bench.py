@@ -93,16 +93,16 @@ def timed_solves(ctx, case, k, max_iter, drop):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sgs_color_bump100_traffic.txt")
 
 
-def measured_traffic(kernel_key):
-    """HBM bytes per unit of `kernel_key` measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size) and
-    recorded in the profile summary; None when no measurement is recorded."""
+def measured_traffic(kernel):
+    """Average HBM bytes per dispatch of `kernel` on the 100^3 bump, measured
+    by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of
+    known size) and recorded in the profile summary; None when absent."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         for line in f:
             parts = line.split()
-            if len(parts) >= 2 and parts[0] == f"traffic_bytes[{kernel_key}]":
+            if len(parts) >= 2 and parts[0] == f"traffic_bytes_per_launch[{kernel}]":
                 return float(parts[1])
     return None
 
@@ -259,10 +259,13 @@ def main():
         v["frac"] = v["gbs"] / HBM_PEAK_GBS
     phase_keys = gpu.PHASES[:9]
     per_iter_ms = {k: sum(v[x] for x in phase_keys) for k, v in [("plain", pp), ("chosen", pc)]}
-    dominant = max(["sgs", "jacobian", "residual"], key=lambda k: pc[k])
-    dom_key = {"sgs": "sgs_sweep_fp32", "jacobian": "jacobian_fp32", "residual": "residual_2nd_fp64"}[dominant]
-    dom = kern[dom_key]
-    traffic = measured_traffic(dom_key)
+    # dominant kernel of the chosen form: the FP32 SGS colour pass.  Per
+    # launch: the passes' algorithmic bytes / passes run, over the average
+    # pass duration (both from the same HIP-event window).
+    dom = kern["sgs_sweep_fp32"]
+    sgs_launch_bytes = sgs32 / npass
+    sgs_launch_us = pc["k_sgs_sweeps"] * 1e3 / npass
+    traffic = measured_traffic("k_sgs_color<float>") if (args.case == "bump" and args.n == 100) else None
     edge_flux = (m.nedge + m.nbedge) / (pp["k_residual"] * 1e-3)
     # converged-field error of the chosen form against the plain FP64 path
     qp, qc = plain["q"][: m.nnode, :5], chosen["q"][: m.nnode, :5]
@@ -315,8 +318,10 @@ def main():
         "phase_ms": {"plain": pp, "chosen": pc},
         "kernels": kern,
         "sgs_colour_passes": {"run": npass, "of": npass_full, "ncolor": plain["info"]["ncolor"]},
-        "roofline": {"bound": "hbm", "kernel": dom_key, "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic},
+        "roofline": {"bound": "hbm", "kernel": "k_sgs_color<float>", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
+                     "algorithmic_bytes_per_launch": sgs_launch_bytes, "avg_launch_us": sgs_launch_us,
+                     "launches_per_sgs_call": npass},
         "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true>", "achieved": kern["residual_2nd_fp64"]["gbs"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern["residual_2nd_fp64"]["frac"]},
         "cpu_baseline": cpu,

@@ -41,17 +41,14 @@ static_assert(kNparam == PCFD_MLP_NPARAM, "parameter layout");
 constexpr int kMlpBlock = 256;  // 4 wavefronts
 constexpr int kWaves = kMlpBlock / 64;
 
-__device__ __forceinline__ short f2bf(float f) {
-  uint32_t u = __float_as_uint(f);
-  u = (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
-  return (short)u;
-}
+// FP32 -> BF16, round to nearest even (the gfx950 conversion instruction)
+__device__ __forceinline__ short f2bf(float f) { return __builtin_bit_cast(short, (__bf16)f); }
 
-// tanh(x) = 1 - 2 / (exp(2x) + 1) on the hardware exp2 / reciprocal
+// tanh(x) = 1 - 2 / (exp(2x) + 1) on the hardware exp2 and reciprocal
 // (relative error ~1e-6, far below the BF16 rounding of the activations)
 __device__ __forceinline__ float tanh_fast(float x) {
   const float e = __expf(2.f * x);
-  return 1.f - 2.f * __frcp_rn(e + 1.f);
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(e + 1.f);
 }
 
 __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -176,75 +173,70 @@ __device__ __forceinline__ void load_x_tile(WaveTiles& T, const float* __restric
 }
 
 // ---- inference path: weights held in registers, 64 samples per pass ----
-// Every lane keeps its B-operand fragments of W1 (4), W2 (16) and W3 (4)
-// and its biases for the whole kernel; a pass runs 64 samples as four
-// 16-sample MFMA tiles with the activations staged in LDS as BF16, so each
-// lane builds (and later post-processes) exactly one sample.
+// Products are taken transposed, Z^T = W H^T: the weight tile is the A
+// operand and the activations the B operand.  The MFMA result then holds,
+// in lane (r, q), features 16mt+4q..+3 of sample r -- exactly the B-operand
+// fragment of k-tile mt for the next layer -- so the activations stay in
+// registers from layer to layer.  Every lane keeps its A fragments of W1
+// (4), W2 (16) and W3 (4) and the biases of the features it produces; a
+// pass runs 64 samples as four 16-sample tiles, with LDS used only to
+// hand features in and outputs out (each lane builds and post-processes
+// one sample).
 struct WRegs {
   bf16x4 w1[4], w2[4][4], w3[4];
-  float b1[4], b2[4], b3;
+  float b1[4][4], b2[4][4], b3[4];
 };
 
 __device__ __forceinline__ void load_wregs(const MlpWeights& w, int r, int q, WRegs& R) {
 #pragma unroll
-  for (int nt = 0; nt < 4; ++nt) {
-    R.w1[nt] = wfrag(w.W1, kKin, nt * 16 + r, 4 * q);
-    R.b1[nt] = w.p[oB1 + nt * 16 + r];
-    R.b2[nt] = w.p[oB2 + nt * 16 + r];
+  for (int mt = 0; mt < 4; ++mt) {
+    R.w1[mt] = wfrag(w.W1, kKin, mt * 16 + r, 4 * q);
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) R.w2[nt][kt] = wfrag(w.W2, kNh, nt * 16 + r, kt * 16 + 4 * q);
-    R.w3[nt] = wfrag(w.W3, kNh, r, nt * 16 + 4 * q);
+    for (int j = 0; j < 4; ++j) {
+      R.b1[mt][j] = w.p[oB1 + mt * 16 + 4 * q + j];
+      R.b2[mt][j] = w.p[oB2 + mt * 16 + 4 * q + j];
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) R.w2[mt][kt] = wfrag(w.W2, kNh, mt * 16 + r, kt * 16 + 4 * q);
+    R.w3[mt] = wfrag(w.W3, kNh, r, mt * 16 + 4 * q);
   }
-  R.b3 = w.p[oB3 + r];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) R.b3[j] = w.p[oB3 + 4 * q + j];
 }
 
 struct InferTiles {
-  short X[64][kKin];  // bf16 features of the pass's 64 samples
-  short H1[16][kNh];  // bf16 activations of the current 16-sample tile
-  short H2[16][kNh];
-  float Y[64][kKout];
+  alignas(16) short X[64][kKin];  // bf16 features of the pass's 64 samples
+  alignas(16) float Y[64][kKout];
 };
-
-__device__ __forceinline__ bf16x4 lds4(const short* p) {
-  bf16x4 v;
-  v[0] = p[0];
-  v[1] = p[1];
-  v[2] = p[2];
-  v[3] = p[3];
-  return v;
-}
 
 // X (LDS, filled by the caller) -> Y (LDS) for the 64 samples of a pass
 __device__ __forceinline__ void infer64(const WRegs& R, InferTiles& T, int r, int q) {
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   for (int tile = 0; tile < 4; ++tile) {
-    const bf16x4 ax = lds4(&T.X[tile * 16 + r][4 * q]);
+    // B operand of layer 1: features 4q..4q+3 of sample r (one 8-byte read)
+    const bf16x4 xb = *reinterpret_cast<const bf16x4*>(&T.X[tile * 16 + r][4 * q]);
+    bf16x4 h1[4], h2[4];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const f32x4 acc = mfma(ax, R.w1[nt], zero);
+    for (int mt = 0; mt < 4; ++mt) {
+      const f32x4 acc = mfma(R.w1[mt], xb, zero);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) T.H1[4 * q + j][nt * 16 + r] = f2bf(tanh_fast(acc[j] + R.b1[nt]));
+      for (int j = 0; j < 4; ++j) h1[mt][j] = f2bf(tanh_fast(acc[j] + R.b1[mt][j]));
     }
-    wave_sync();
-    bf16x4 a[4];
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) a[kt] = lds4(&T.H1[r][kt * 16 + 4 * q]);
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
+    for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc = zero;
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt) acc = mfma(a[kt], R.w2[nt][kt], acc);
+      for (int kt = 0; kt < 4; ++kt) acc = mfma(R.w2[mt][kt], h1[kt], acc);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) T.H2[4 * q + j][nt * 16 + r] = f2bf(tanh_fast(acc[j] + R.b2[nt]));
+      for (int j = 0; j < 4; ++j) h2[mt][j] = f2bf(tanh_fast(acc[j] + R.b2[mt][j]));
     }
-    wave_sync();
     f32x4 y = zero;
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt) y = mfma(lds4(&T.H2[r][kt * 16 + 4 * q]), R.w3[kt], y);
+    for (int kt = 0; kt < 4; ++kt) y = mfma(R.w3[kt], h2[kt], y);
 #pragma unroll
-    for (int j = 0; j < 4; ++j) T.Y[tile * 16 + 4 * q + j][r] = y[j] + R.b3;
-    wave_sync();
+    for (int j = 0; j < 4; ++j) T.Y[tile * 16 + r][4 * q + j] = y[j] + R.b3[j];
   }
+  wave_sync();
 }
 
 __global__ void __launch_bounds__(kMlpBlock) k_mlp_forward(MlpWeights w, int n, const float* __restrict__ x,
@@ -386,18 +378,40 @@ __global__ void __launch_bounds__(kMlpBlock) k_mlp_grad(MlpWeights w, int n, con
   if (lane == 0) lossp[blockIdx.x * kWaves + wv] = loss;
 }
 
-__global__ void __launch_bounds__(kMlpBlock) k_mlp_reduce(int nparts, const float* __restrict__ partial,
-                                                         const float* __restrict__ lossp, float* __restrict__ grad,
-                                                         float* __restrict__ loss) {
+// Sum of the per-wavefront partials in a fixed order, in two stages so the
+// work spreads over the chip: stage 1 sums groups of kRedGroup consecutive
+// partials (one block row per group), stage 2 sums the group totals.
+constexpr int kRedGroup = 32;
+
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_reduce1(int nparts, const float* __restrict__ partial,
+                                                          const float* __restrict__ lossp, float* __restrict__ part2,
+                                                          float* __restrict__ loss2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p0 = blockIdx.y * kRedGroup, p1 = min(nparts, p0 + kRedGroup);
+  if (i < kNparam) {
+    float s = 0.f;
+    for (int p = p0; p < p1; ++p) s += partial[(long long)p * kNparam + i];
+    part2[(long long)blockIdx.y * kNparam + i] = s;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    float s = 0.f;
+    for (int p = p0; p < p1; ++p) s += lossp[p];
+    loss2[blockIdx.y] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kMlpBlock) k_mlp_reduce2(int ngroup, const float* __restrict__ part2,
+                                                          const float* __restrict__ loss2, float* __restrict__ grad,
+                                                          float* __restrict__ loss) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < kNparam) {
     float s = 0.f;
-    for (int p = 0; p < nparts; ++p) s += partial[(long long)p * kNparam + i];
+    for (int b = 0; b < ngroup; ++b) s += part2[(long long)b * kNparam + i];
     grad[i] = s;
   }
   if (i == 0) {
     float s = 0.f;
-    for (int p = 0; p < nparts; ++p) s += lossp[p];
+    for (int b = 0; b < ngroup; ++b) s += loss2[b];
     *loss = s;
   }
 }
@@ -458,6 +472,8 @@ struct pcfd_mlp {
   short *W1 = nullptr, *W2 = nullptr, *W3 = nullptr;
   float* partial = nullptr;
   float* lossp = nullptr;
+  float* part2 = nullptr;  // group totals of the partials
+  float* loss2 = nullptr;
   int grid = 0;  // blocks of the gradient kernel
   long long adam_t = 0;
   float last_ms = 0.f;
@@ -496,9 +512,13 @@ void grad_device(pcfd_mlp& c, int n, const float* x, const float* t, const int* 
   hipLaunchKernelGGL(k_mlp_grad, dim3(grid), dim3(kMlpBlock), 0, c.stream, weights(c), n, x, t, idx,
                      1.0f / (float)((long long)n * kNout), c.partial, c.lossp);
   mcheck(hipGetLastError(), "k_mlp_grad");
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((kNparam + kMlpBlock - 1) / kMlpBlock), dim3(kMlpBlock), 0, c.stream,
-                     grid * kWaves, c.partial, c.lossp, c.g, c.loss);
-  mcheck(hipGetLastError(), "k_mlp_reduce");
+  const int nparts = grid * kWaves, ngroup = (nparts + kRedGroup - 1) / kRedGroup;
+  hipLaunchKernelGGL(k_mlp_reduce1, dim3((kNparam + kMlpBlock - 1) / kMlpBlock, ngroup), dim3(kMlpBlock), 0,
+                     c.stream, nparts, c.partial, c.lossp, c.part2, c.loss2);
+  mcheck(hipGetLastError(), "k_mlp_reduce1");
+  hipLaunchKernelGGL(k_mlp_reduce2, dim3((kNparam + kMlpBlock - 1) / kMlpBlock), dim3(kMlpBlock), 0, c.stream,
+                     ngroup, c.part2, c.loss2, c.g, c.loss);
+  mcheck(hipGetLastError(), "k_mlp_reduce2");
 }
 
 struct DBuf {
@@ -549,6 +569,9 @@ int pcfd_mlp_create(unsigned long long seed, pcfd_mlp** out) {
     mcheck(hipMalloc((void**)&c->W3, sizeof(short) * kKout * kNh), "hipMalloc");
     mcheck(hipMalloc((void**)&c->partial, sizeof(float) * (size_t)c->grid * kWaves * kNparam), "hipMalloc");
     mcheck(hipMalloc((void**)&c->lossp, sizeof(float) * (size_t)c->grid * kWaves), "hipMalloc");
+    const size_t ngroup = ((size_t)c->grid * kWaves + kRedGroup - 1) / kRedGroup;
+    mcheck(hipMalloc((void**)&c->part2, sizeof(float) * ngroup * kNparam), "hipMalloc");
+    mcheck(hipMalloc((void**)&c->loss2, sizeof(float) * ngroup), "hipMalloc");
     // Glorot-uniform weights, zero biases (host-side, reproducible)
     std::vector<float> h(kNparam, 0.f);
     uint64_t s = seed;
@@ -579,7 +602,7 @@ int pcfd_mlp_create(unsigned long long seed, pcfd_mlp** out) {
 
 int pcfd_mlp_destroy(pcfd_mlp* c) {
   if (!c) return PCFD_ERR_ARG;
-  float* fs[] = {c->p, c->m, c->v, c->g, c->loss, c->partial, c->lossp};
+  float* fs[] = {c->p, c->m, c->v, c->g, c->loss, c->partial, c->lossp, c->part2, c->loss2};
   for (float* f : fs)
     if (f) (void)hipFree(f);
   short* ss[] = {c->W1, c->W2, c->W3};
@@ -631,8 +654,10 @@ int pcfd_mlp_forward_device(pcfd_mlp* c, int n, const float* x, float* y) {
       return PCFD_ERR_ARG;
     }
     if (n == 0) return PCFD_OK;
-    const int ntile = (n + 15) / 16;
-    const int grid = std::max(1, std::min(4096, (ntile + kWaves - 1) / kWaves));
+    // each wavefront loads its weight fragments once and loops over 64-sample
+    // passes: about four blocks per CU
+    const int npass = (n + 63) / 64;
+    const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
     mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
     hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), n, x, y);
     mcheck(hipGetLastError(), "k_mlp_forward");
@@ -743,11 +768,11 @@ int pcfd_mlp_train(pcfd_mlp* c, int n, const float* x, const float* t, int steps
 namespace pcfd {
 
 struct SurrogateDev {
-  float lnT_mid, lnT_half, lnrho_mid, lnrho_half, y_mid, y_scale, y_eps;
+  float lnT_mid, ilnT_half, lnrho_mid, ilnrho_half, y_mid, iy_scale, y_eps;  // i*: reciprocals
   double s0;
   double sigma[5];
   double T_min, T_max, rho_min, rho_max, tol, floor;
-  double mw[5];
+  double imw[5];  // 1 / molecular weight
 };
 
 __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, SurrogateDev sd, int n,
@@ -779,12 +804,12 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
     }
     // features (FP32): lane builds its own sample
     const float irho = (float)(1.0 / rho);
-    Tl.X[lane][0] = f2bf((logf((float)Tc) - sd.lnT_mid) / sd.lnT_half);
-    Tl.X[lane][1] = f2bf((logf((float)rho) - sd.lnrho_mid) / sd.lnrho_half);
+    Tl.X[lane][0] = f2bf((logf((float)Tc) - sd.lnT_mid) * sd.ilnT_half);
+    Tl.X[lane][1] = f2bf((logf((float)rho) - sd.lnrho_mid) * sd.ilnrho_half);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const float Y = (float)ri[k] * irho;
-      Tl.X[lane][2 + k] = f2bf(((sd.y_eps > 0.f ? logf(Y + sd.y_eps) : Y) - sd.y_mid) / sd.y_scale);
+      Tl.X[lane][2 + k] = f2bf(((sd.y_eps > 0.f ? logf(Y + sd.y_eps) : Y) - sd.y_mid) * sd.iy_scale);
     }
 #pragma unroll
     for (int f = kNin; f < kKin; ++f) Tl.X[lane][f] = 0;
@@ -797,7 +822,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
         const float z = (float)sd.sigma[k] * Tl.Y[lane][k];
         const float e = expf(z);
         wv5[k] = rho * sd.s0 * (double)(0.5f * (e - 1.f / e));
-        m[k] = wv5[k] / sd.mw[k];
+        m[k] = wv5[k] * sd.imw[k];
         mag += fabs(m[k]);
       }
       // element balance, species order N, N2, O2, NO, O
@@ -849,16 +874,16 @@ int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_s
     if (n == 0) return PCFD_OK;
     SurrogateDev sd{};
     sd.lnT_mid = (float)d->lnT_mid;
-    sd.lnT_half = (float)d->lnT_half;
+    sd.ilnT_half = (float)(1.0 / d->lnT_half);
     sd.lnrho_mid = (float)d->lnrho_mid;
-    sd.lnrho_half = (float)d->lnrho_half;
+    sd.ilnrho_half = (float)(1.0 / d->lnrho_half);
     sd.y_mid = (float)d->y_mid;
-    sd.y_scale = (float)d->y_scale;
+    sd.iy_scale = (float)(1.0 / d->y_scale);
     sd.y_eps = (float)d->y_eps;
     sd.s0 = d->rate_scale;
     for (int k = 0; k < 5; ++k) {
       sd.sigma[k] = d->sigma[k];
-      sd.mw[k] = chem_mw(chem)[k];
+      sd.imw[k] = 1.0 / chem_mw(chem)[k];
     }
     sd.T_min = d->T_min;
     sd.T_max = d->T_max;
@@ -869,8 +894,10 @@ int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_s
     DBuf dlist(sizeof(int) * (size_t)n + sizeof(int));
     int* nlist = dlist.as<int>() + n;
     mcheck(hipMemsetAsync(nlist, 0, sizeof(int), m.stream), "hipMemset");
-    const int ntile = (n + 15) / 16;
-    const int grid = std::max(1, std::min(8192, (ntile + kWaves - 1) / kWaves));
+    // each wavefront loads its weight fragments once and loops over 64-sample
+    // passes: about four blocks per CU
+    const int npass = (n + 63) / 64;
+    const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
     mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
     hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
                        dlist.as<int>(), nlist);

@@ -94,9 +94,13 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
     const int w = swidth[chunk];
     const long long cb = (long long)sbase[chunk] * kW + lane;
     const T* mb = Moff + (long long)sbase[chunk] * 25 * kW + lane;
+    // the next slot's column index is loaded one iteration ahead, so this
+    // slot's x and block loads issue without waiting for it
+    int jn = (sub < w) ? scol[cb + (long long)sub * kW] : -1;
     for (int s = sub; s < w; s += kTPR) {
-      const int j = scol[cb + (long long)s * kW];
+      const int j = jn;
       if (j < 0) break;
+      jn = (s + kTPR < w) ? scol[cb + (long long)(s + kTPR) * kW] : -1;
       const T* xj = x8 + (long long)j * 8;
       const T x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3], x4 = xj[4];
       const T* blk = mb + (long long)s * 25 * kW;

@@ -386,8 +386,9 @@ PCFD_DEV void aux_tuvw(const double* Q, double gamma, double* t) {
   t[3] = ww;
 }
 
-// Sutherland's law, non-dimensional (ComputeViscosity, ucs/eqnset.h:251-268)
-PCFD_DEV double sutherland(double T, double S) { return (1.0 + S) * pow(T, 1.5) / (T + S); }
+// Sutherland's law, non-dimensional (ComputeViscosity, ucs/eqnset.h:251-268);
+// T^1.5 is evaluated as T*sqrt(T) (two roundings instead of pow's one)
+PCFD_DEV double sutherland(double T, double S) { return (1.0 + S) * (T * sqrt(T)) / (T + S); }
 
 struct ViscParams {
   double gamma, re_tilde, pr, prt, suth;
@@ -438,79 +439,71 @@ PCFD_DEV void viscous_flux(const double* Q, double T, const double* g, const dou
 // Column c of one side of ViscousJacobian (ucs/compressible.tcc:1632-1893)
 // before the left side's sign flip.  Dx: that side's directional weights
 // (+-dx/(rho s2)); u, v, w: edge-averaged velocity; Tn: that side's
-// temperature-gradient weight; rho, uS, vS, wS, P: that side's state.
+// temperature-gradient weight; uS, vS, wS, PS, irS: that side's velocity,
+// pressure and 1/rho.  The columns of rho u, rho v, rho w (c = 1..3, m =
+// c - 1) share one form: row k is -2/3 Dx[m] a[k] + Dx[k] a[m], plus Dx.a on
+// row m (which gives the reference's 4/3 Dx[m] a[m] + the other two terms).
 PCFD_DEV void visc_jac_col(int c, const double* Dx, const double* a, double u, double v, double w, double RK,
-                           double RKT, double Tn, double gm1, double rhoS, double uS, double vS, double wS,
-                           double PS, double* col) {
-  const double c43 = 4.0 / 3.0, mc23 = -2.0 / 3.0;
-  col[0] = 0.0;
+                           double RKT, double Tn, double gm1, double uS, double vS, double wS, double PS, double irS,
+                           double* col) {
+  const double mc23 = -2.0 / 3.0;
+  double d0, d1, d2, e4;
   if (c == 0) {
     const double dux = -u * Dx[0], duy = -u * Dx[1], duz = -u * Dx[2];
     const double dvx = -v * Dx[0], dvy = -v * Dx[1], dvz = -v * Dx[2];
     const double dwx = -w * Dx[0], dwy = -w * Dx[1], dwz = -w * Dx[2];
-    const double dfact = -2.0 / 3.0 * (dux + dvy + dwz);
+    const double dfact = mc23 * (dux + dvy + dwz);
     const double dtauxx = (2.0 * dux + dfact);
     const double dtauyy = (2.0 * dvy + dfact);
     const double dtauzz = (2.0 * dwz + dfact);
     const double dtauxy = duy + dvx;
     const double dtauxz = duz + dwx;
     const double dtauyz = dvz + dwy;
-    const double dtauxn = dtauxx * a[0] + dtauxy * a[1] + dtauxz * a[2];
-    const double dtauyn = dtauxy * a[0] + dtauyy * a[1] + dtauyz * a[2];
-    const double dtauzn = dtauxz * a[0] + dtauyz * a[1] + dtauzz * a[2];
+    d0 = dtauxx * a[0] + dtauxy * a[1] + dtauxz * a[2];
+    d1 = dtauxy * a[0] + dtauyy * a[1] + dtauyz * a[2];
+    d2 = dtauxz * a[0] + dtauyz * a[1] + dtauzz * a[2];
     const double v2 = (uS * uS + vS * vS + wS * wS);
-    const double dP_dr = +gm1 * 0.5 * v2;
-    col[1] = -RKT * (dtauxn);
-    col[2] = -RKT * (dtauyn);
-    col[3] = -RKT * (dtauzn);
-    col[4] = -RKT * (dtauxn * u + dtauyn * v + dtauzn * w) + RK * Tn * (dP_dr - PS / rhoS);
+    e4 = RK * Tn * (gm1 * 0.5 * v2 - PS * irS);
   } else if (c == 4) {
-    col[1] = 0.0;
-    col[2] = 0.0;
-    col[3] = 0.0;
-    col[4] = RK * Tn * (+gm1);
+    d0 = d1 = d2 = 0.0;
+    e4 = RK * Tn * gm1;
   } else {
-    double d2, d3, d4, dP;
-    if (c == 1) {
-      d2 = (c43 * Dx[0] * a[0] + Dx[1] * a[1] + Dx[2] * a[2]);
-      d3 = (mc23 * Dx[0] * a[1] + Dx[1] * a[0]);
-      d4 = (mc23 * Dx[0] * a[2] + Dx[2] * a[0]);
-      dP = -gm1 * uS;
-    } else if (c == 2) {
-      d2 = (mc23 * Dx[1] * a[0] + Dx[0] * a[1]);
-      d3 = (Dx[0] * a[0] + c43 * Dx[1] * a[1] + Dx[2] * a[2]);
-      d4 = (mc23 * Dx[1] * a[2] + Dx[2] * a[1]);
-      dP = -gm1 * vS;
-    } else {
-      d2 = (mc23 * Dx[2] * a[0] + Dx[0] * a[2]);
-      d3 = (mc23 * Dx[2] * a[1] + Dx[1] * a[2]);
-      d4 = (Dx[0] * a[0] + Dx[1] * a[1] + c43 * Dx[2] * a[2]);
-      dP = -gm1 * wS;
-    }
-    col[1] = -RKT * d2;
-    col[2] = -RKT * d3;
-    col[3] = -RKT * d4;
-    col[4] = -RKT * (d2 * u + d3 * v + d4 * w) + RK * Tn * dP;
+    const int m = c - 1;
+    const double Dm = m == 0 ? Dx[0] : (m == 1 ? Dx[1] : Dx[2]);
+    const double am = m == 0 ? a[0] : (m == 1 ? a[1] : a[2]);
+    const double sm = m == 0 ? uS : (m == 1 ? vS : wS);
+    const double DA = Dx[0] * a[0] + Dx[1] * a[1] + Dx[2] * a[2];
+    d0 = mc23 * Dm * a[0] + Dx[0] * am + (m == 0 ? DA : 0.0);
+    d1 = mc23 * Dm * a[1] + Dx[1] * am + (m == 1 ? DA : 0.0);
+    d2 = mc23 * Dm * a[2] + Dx[2] * am + (m == 2 ? DA : 0.0);
+    e4 = RK * Tn * (-gm1 * sm);
   }
+  col[0] = 0.0;
+  col[1] = -RKT * d0;
+  col[2] = -RKT * d1;
+  col[3] = -RKT * d2;
+  col[4] = -RKT * (d0 * u + d1 * v + d2 * w) + e4;
 }
 
 // Columns c of both sides of ViscousJacobian for the edge (QL, QR), left
-// side already sign-flipped; symmetry2D rows zeroed.
+// side already sign-flipped; symmetry2D rows zeroed.  The quantities of
+// ucs/compressible.tcc:1632-1893 with each division by rho or s2 taken as a
+// product with one reciprocal and T^1.5 as T sqrt(T).
 PCFD_DEV void visc_jac_cols(int c, const double* QL, const double* QR, const double* xL, const double* xR,
                             const double* a, const ViscParams& vp, double* colL, double* colR) {
   const double gamma = vp.gamma, gm1 = gamma - 1.0;
-  double Qavg[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k) Qavg[k] = 0.5 * (QL[k] + QR[k]);
-  double ta[4];
-  aux_tuvw(Qavg, gamma, ta);
-  const double mu = sutherland(ta[0], vp.suth);
-  const double tmut = (mu + 0.0);
+  // edge-averaged state -> temperature -> Sutherland viscosity
+  const double ra = 0.5 * (QL[0] + QR[0]), ira = 1.0 / ra;
+  const double ua = 0.5 * (QL[1] + QR[1]) * ira, va = 0.5 * (QL[2] + QR[2]) * ira, wa = 0.5 * (QL[3] + QR[3]) * ira;
+  const double pa = gm1 * (0.5 * (QL[4] + QR[4]) - ra * (0.5 * (ua * ua + va * va + wa * wa)));
+  const double mu = sutherland(gamma * pa * ira, vp.suth);
   const double RK = a[3] / vp.re_tilde;
-  const double RKT = RK * tmut;
-  const double rhoL = QL[0], uL = QL[1] / rhoL, vL = QL[2] / rhoL, wL = QL[3] / rhoL;
-  const double rhoR = QR[0], uR = QR[1] / rhoR, vR = QR[2] / rhoR, wR = QR[3] / rhoR;
-  const double PL = aux_pressure(QL, gm1), PR = aux_pressure(QR, gm1);
+  const double RKT = RK * mu;
+  const double irL = 1.0 / QL[0], irR = 1.0 / QR[0];
+  const double uL = QL[1] * irL, vL = QL[2] * irL, wL = QL[3] * irL;
+  const double uR = QR[1] * irR, vR = QR[2] * irR, wR = QR[3] * irR;
+  const double PL = gm1 * (QL[4] - 0.5 * QL[0] * (uL * uL + vL * vL + wL * wL));
+  const double PR = gm1 * (QR[4] - 0.5 * QR[0] * (uR * uR + vR * vR + wR * wR));
   const double u = 0.5 * (uL + uR), v = 0.5 * (vL + vR), w = 0.5 * (wL + wR);
   double dx[3], s2 = 0.0;
 #pragma unroll
@@ -518,20 +511,16 @@ PCFD_DEV void visc_jac_cols(int c, const double* QL, const double* QR, const dou
     dx[k] = (xR[k] - xL[k]);
     s2 += dx[k] * dx[k];
   }
-  double DxL[3], DxR[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    DxL[k] = -dx[k] / (rhoL * s2);
-    DxR[k] = dx[k] / (rhoR * s2);
-  }
-  const double cp = 1.0 / (vp.gamma - 1.0);
-  const double kk = mu / vp.pr * cp;
-  const double kT = 0.0 / vp.prt * cp;
-  const double c1 = -(kk + kT);
-  const double TnL = (DxL[0] * a[0] + DxL[1] * a[1] + DxL[2] * a[2]) * (gamma / rhoL) * c1;
-  const double TnR = (DxR[0] * a[0] + DxR[1] * a[1] + DxR[2] * a[2]) * (gamma / rhoR) * c1;
-  visc_jac_col(c, DxR, a, u, v, w, RK, RKT, TnR, gm1, rhoR, uR, vR, wR, PR, colR);
-  visc_jac_col(c, DxL, a, u, v, w, RK, RKT, TnL, gm1, rhoL, uL, vL, wL, PL, colL);
+  const double is2 = 1.0 / s2;
+  const double fL = -irL * is2, fR = irR * is2;
+  const double DxL[3] = {dx[0] * fL, dx[1] * fL, dx[2] * fL};
+  const double DxR[3] = {dx[0] * fR, dx[1] * fR, dx[2] * fR};
+  // conductivity k = mu Cp / Pr with Cp = 1/(gamma-1); turbulent part 0
+  const double c1 = -(mu / (vp.pr * gm1));
+  const double TnL = (DxL[0] * a[0] + DxL[1] * a[1] + DxL[2] * a[2]) * (gamma * irL) * c1;
+  const double TnR = (DxR[0] * a[0] + DxR[1] * a[1] + DxR[2] * a[2]) * (gamma * irR) * c1;
+  visc_jac_col(c, DxR, a, u, v, w, RK, RKT, TnR, gm1, uR, vR, wR, PR, irR, colR);
+  visc_jac_col(c, DxL, a, u, v, w, RK, RKT, TnL, gm1, uL, vL, wL, PL, irL, colL);
 #pragma unroll
   for (int k = 0; k < 5; ++k) colL[k] = -colL[k];
   if (vp.sym2d >= 1 && vp.sym2d <= 3) {
