@@ -72,8 +72,8 @@ struct Dev {
   int* piv = nullptr;       // nrow_pad, 3-bit packed row permutation
   double* bsd = nullptr;    // 5 x nrow_pad right-hand side in SELL row order
   float* bsf = nullptr;
-  double* x8d = nullptr;    // 8 x nnode AoS correction (5 used), FP64 mode
-  float* x8f = nullptr;     // 8 x nnode AoS correction, FP32 mode
+  double* xd = nullptr;     // 5 x nnode AoS correction, FP64 mode
+  float* xf = nullptr;      // 5 x nnode AoS correction, FP32 mode
   // reductions
   double* red = nullptr;    // partial sums
   double* red_out = nullptr;

@@ -44,7 +44,7 @@ void free_all(Ctx& c) {
   dfree(d.jdb); dfree(d.jqb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
   dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
-  dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.x8d); dfree(d.x8f); dfree(d.red);
+  dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.xd); dfree(d.xf); dfree(d.red);
   dfree(d.red_out); dfree(d.icount);
   if (c.ev0) (void)hipEventDestroy(c.ev0);
   if (c.ev1) (void)hipEventDestroy(c.ev1);
@@ -94,12 +94,12 @@ void ensure_matrix(Ctx& c) {
     if (!c.d.Mf) dalloc(c, &c.d.Mf, off);
     if (!c.d.Df) dalloc(c, &c.d.Df, dg);
     if (!c.d.bsf) dalloc(c, &c.d.bsf, 5 * (size_t)c.nrow_pad);
-    if (!c.d.x8f) dalloc(c, &c.d.x8f, 8 * (size_t)c.nnode);
+    if (!c.d.xf) dalloc(c, &c.d.xf, 5 * (size_t)c.nnode);
   } else {
     if (!c.d.Md) dalloc(c, &c.d.Md, off);
     if (!c.d.Dd) dalloc(c, &c.d.Dd, dg);
     if (!c.d.bsd) dalloc(c, &c.d.bsd, 5 * (size_t)c.nrow_pad);
-    if (!c.d.x8d) dalloc(c, &c.d.x8d, 8 * (size_t)c.nnode);
+    if (!c.d.xd) dalloc(c, &c.d.xd, 5 * (size_t)c.nnode);
   }
 }
 
@@ -659,10 +659,9 @@ int pcfd_sgs(pcfd_ctx* ctx, int nsgs, double* x_out, double* delta) {
     launch_sgs(c, nsgs, delta);
     if (x_out) {
       std::vector<double> h;
-      if (c.prm.precision == PCFD_CHOSEN_FP32) fetch(h, c.d.x8f, 8 * (size_t)c.nnode);
-      else fetch(h, c.d.x8d, 8 * (size_t)c.nnode);
-      for (int i = 0; i < c.nnode; ++i)
-        for (int k = 0; k < 5; ++k) x_out[(size_t)i * 5 + k] = h[(size_t)i * 8 + k];
+      if (c.prm.precision == PCFD_CHOSEN_FP32) fetch(h, c.d.xf, 5 * (size_t)c.nnode);
+      else fetch(h, c.d.xd, 5 * (size_t)c.nnode);
+      for (size_t k = 0; k < 5 * (size_t)c.nnode; ++k) x_out[k] = h[k];
       return PCFD_OK;
     }
     PCFD_HIP(hipStreamSynchronize(c.stream));

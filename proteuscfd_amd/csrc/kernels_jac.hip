@@ -40,6 +40,11 @@ struct JacArgs {
   ViscParams vp;
 };
 
+// One-sided differences with h = 1e-8 (ucs/jacobian.tcc:268): the quotient
+// is taken as a product with 1/h = 1e8, which is exact in binary64, so it
+// differs from the division by h at most in the last bit.
+constexpr double kInvH = 1.0e8;
+
 // Five lanes per block row: lane c of a row's group owns perturbation
 // column c -- it writes column c of every block of the row and accumulates
 // column c of the diagonal.  Each block entry is produced by exactly the
@@ -106,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
         noslip_state(QPL, QPR, tw, nQ, gamma);
         roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
-        for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) / h;
+        for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
         continue;
       }
       if (is_dirichlet(type)) {
@@ -132,7 +137,7 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
       double fL[5];
       roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
-      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) / h;
+      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
     }
     // the node state after the re-evaluation: the viscous Jacobian, which
     // the reference runs after Bkernel_NumJac, sees it
@@ -207,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     roe_flux_s(QP, SP, QR, SR, a, gamma, A.sym2d, fP);
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) {
-      const double aRL = (fS[rr] - fP[rr]) / h;  // A[R][L] = -dF/dQL
+      const double aRL = (fS[rr] - fP[rr]) * kInvH;  // A[R][L] = -dF/dQL
       if (left) colc[rr] = aRL;
       else offc[rr] = aRL;
     }
@@ -217,7 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     roe_flux_s(QL, SL, QP, SP, a, gamma, A.sym2d, fP);
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) {
-      const double aLR = (fP[rr] - fS[rr]) / h;  // A[L][R] = +dF/dQR
+      const double aLR = (fP[rr] - fS[rr]) * kInvH;  // A[L][R] = +dF/dQR
       if (left) offc[rr] = aLR;
       else colc[rr] = aLR;
     }

@@ -9,13 +9,13 @@
 namespace pcfd {
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_apply_dq(int nnode, int ntot, double gamma, T* __restrict__ x8,
+__global__ void __launch_bounds__(kBlock) k_apply_dq(int nnode, int ntot, double gamma, T* __restrict__ x,
                                                     double* __restrict__ q, int* __restrict__ count) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   double dq[5], Q[5];
   bool zero = false;
-  T* xi = x8 + (long long)i * 8;
+  T* xi = x + (long long)i * 5;
 #pragma unroll
   for (int k = 0; k < 5; ++k) {
     dq[k] = (double)xi[k];
@@ -73,10 +73,10 @@ void launch_apply_dq(Ctx& c, int explicit_mode, int* nzeroed) {
                        c.d.b, c.d.dt, c.d.vol, c.d.q, c.d.icount);
   } else if (c.prm.precision == PCFD_CHOSEN_FP32) {
     hipLaunchKernelGGL(k_apply_dq<float>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.prm.gamma, c.d.x8f, c.d.q, c.d.icount);
+                       c.prm.gamma, c.d.xf, c.d.q, c.d.icount);
   } else {
     hipLaunchKernelGGL(k_apply_dq<double>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot,
-                       c.prm.gamma, c.d.x8d, c.d.q, c.d.icount);
+                       c.prm.gamma, c.d.xd, c.d.q, c.d.icount);
   }
   PCFD_HIP(hipGetLastError());
   if (nzeroed) {

@@ -6,7 +6,8 @@
 // way back); within a colour all rows update in parallel.  Blocks are read
 // from the SELL layout (one lane per row, the 64 rows of a chunk read one
 // 512-byte (FP64) / 256-byte (FP32) line per block entry), x is kept as
-// 8-wide AoS so a neighbour's 5 values come from one line.
+// 5-wide AoS (x[i*5+k]): a neighbour's values are 40 (FP64) / 20 (FP32)
+// consecutive bytes, and the lines the gathers pull in carry no padding.
 #include <stdexcept>
 
 #include "common.h"
@@ -81,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
                                                      const int* __restrict__ swidth, const int* __restrict__ sbase,
                                                      const int* __restrict__ scol, const T* __restrict__ Moff,
                                                      const T* __restrict__ Dg, const int* __restrict__ piv,
-                                                     const T* __restrict__ bs, T* __restrict__ x8) {
+                                                     const T* __restrict__ bs, T* __restrict__ x) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int sub = (threadIdx.x & 63) >> 5;
   const int local = (t >> 6) * 32 + (threadIdx.x & 31);
@@ -101,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
       const int j = jn;
       if (j < 0) break;
       jn = (s + kTPR < w) ? scol[cb + (long long)(s + kTPR) * kW] : -1;
-      const T* xj = x8 + (long long)j * 8;
+      const T* xj = x + (long long)j * 5;
       const T x0 = xj[0], x1 = xj[1], x2 = xj[2], x3 = xj[3], x4 = xj[4];
       const T* blk = mb + (long long)s * 25 * kW;
 #pragma unroll
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_sgs_color(int row0, int nrows, const
 #pragma unroll
   for (int k = 0; k < 25; ++k) A[k] = mload(dg + k * kW);
   lu_solve5(A, piv[r], rhs);
-  T* xo = x8 + (long long)i * 8;
+  T* xo = x + (long long)i * 5;
 #pragma unroll
   for (int k = 0; k < 5; ++k) xo[k] = rhs[k];
 }
@@ -141,13 +142,13 @@ __global__ void __launch_bounds__(kBlock) k_zero(long long n, T* __restrict__ x)
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_sumsq_x8(int nnode, const T* __restrict__ x8, double* __restrict__ red) {
+__global__ void __launch_bounds__(kBlock) k_sumsq_x(int nnode, const T* __restrict__ x, double* __restrict__ red) {
   __shared__ double sh[kBlock];
   double s = 0.0;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nnode; i += gridDim.x * blockDim.x) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-      double v = (double)x8[(long long)i * 8 + k];
+      double v = (double)x[(long long)i * 5 + k];
       s += v * v;
     }
   }
@@ -161,9 +162,9 @@ __global__ void __launch_bounds__(kBlock) k_sumsq_x8(int nnode, const T* __restr
 }
 
 template <typename T>
-static double xnorm(Ctx& c, const T* x8) {
+static double xnorm(Ctx& c, const T* x) {
   const int nb = 512;
-  hipLaunchKernelGGL(k_sumsq_x8<T>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, x8, c.d.red);
+  hipLaunchKernelGGL(k_sumsq_x<T>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, x, c.d.red);
   PCFD_HIP(hipGetLastError());
   double h[nb];
   PCFD_HIP(hipMemcpyAsync(h, c.d.red, sizeof(h), hipMemcpyDeviceToHost, c.stream));
@@ -174,9 +175,9 @@ static double xnorm(Ctx& c, const T* x8) {
 }
 
 template <typename T>
-static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8, double* delta) {
-  long long n8 = 8LL * c.nnode;
-  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n8)), dim3(kBlock), 0, c.stream, n8, x8);
+static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x, double* delta) {
+  long long n5 = 5LL * c.nnode;
+  hipLaunchKernelGGL(k_zero<T>, dim3(nblk(n5)), dim3(kBlock), 0, c.stream, n5, x);
   PCFD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pack_b<T>, dim3(nblk(c.nrow_pad)), dim3(kBlock), 0, c.stream, c.nrow_pad, c.nnode, c.d.srow,
                      c.d.b, bs);
@@ -197,14 +198,14 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
         if (nrows == 0 || col == last) continue;
         last = col;
         hipLaunchKernelGGL(k_sgs_color<T>, dim3(nblk((long long)nrows * kTPR)), dim3(kBlock), 0, c.stream, row0,
-                           nrows, c.d.srow, c.d.swidth, c.d.sbase, c.d.scol, Moff, Dg, c.d.piv, bs, x8);
+                           nrows, c.d.srow, c.d.swidth, c.d.sbase, c.d.scol, Moff, Dg, c.d.piv, bs, x);
         PCFD_HIP(hipGetLastError());
       }
     }
     sub_end(c, 9);
     if (delta && it >= nsgs - 2) {
       xold = xn;
-      xn = xnorm(c, x8);
+      xn = xnorm(c, x);
     }
   }
   if (delta) *delta = std::fabs(xold - xn);
@@ -212,9 +213,9 @@ static void sgs_impl(Ctx& c, int nsgs, const T* Moff, const T* Dg, T* bs, T* x8,
 
 void launch_sgs(Ctx& c, int nsgs, double* delta) {
   if (c.prm.precision == PCFD_CHOSEN_FP32)
-    sgs_impl<float>(c, nsgs, c.d.Mf, c.d.Df, c.d.bsf, c.d.x8f, delta);
+    sgs_impl<float>(c, nsgs, c.d.Mf, c.d.Df, c.d.bsf, c.d.xf, delta);
   else
-    sgs_impl<double>(c, nsgs, c.d.Md, c.d.Dd, c.d.bsd, c.d.x8d, delta);
+    sgs_impl<double>(c, nsgs, c.d.Md, c.d.Dd, c.d.bsd, c.d.xd, delta);
 }
 
 }  // namespace pcfd
