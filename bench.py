@@ -1,7 +1,9 @@
 """Benchmark: time to an 8-order residual drop, chosen form (FP32 Jacobian +
 SGS, FP64 residual/gradients/norms) vs the plain FP64 form, on the 1.03M-node
-second-order Euler bump case (default) or the laminar flat plate (--case
-plate, 8M cells by default), on one MI355X.
+second-order Euler bump case (the headline line), followed by the same two
+solves of the 8M-cell laminar flat plate as a "laminar_plate" record
+(--no-plate skips it; --case plate makes the plate the headline), on one
+MI355X.
 
 A "step" here is one complete steady solve from freestream until
 residGlobal <= 1e-8 * residGlobal(iteration 1); --steps K times K solves of
@@ -138,6 +140,80 @@ def cpu_baseline(case, budget_s):
     return it_s, tm, init
 
 
+def make_case(kind, args):
+    """The workload's mesh, parameters and description."""
+    from proteuscfd_amd import cases
+    from proteuscfd_amd.cases import Params
+    if kind == "bump":
+        base = dict(sorder=2, limiter=3, venkat_K=args.venkat_k, cfl=args.cfl or 50.0, cfl_start=1.0, cfl_ramp=20,
+                    nsgs=8)
+        case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
+        m = case.mesh
+        workload = (f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
+                    f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
+                    f"M=0.5 farfield, weighted LSQ (9 terms), modified Venkat K={args.venkat_k:g}, "
+                    f"first-order FD Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 20 iterations")
+    else:
+        nx, ny, nz = (int(v) for v in args.plate.split(","))
+        cfl = (args.cfl if kind == args.case else 0.0) or 100.0
+        base = dict(sorder=2, limiter=0, cfl=cfl, cfl_start=1.0, cfl_ramp=100, nsgs=8, viscous=1,
+                    reynolds=1.0e4, mach_ref=0.3)
+        case = cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=20, first_h=1e-4, growth=1.15, twall=1.0,
+                                top_bc=cases.BC_SYMMETRY, params=Params(**base, precision=0))
+        m = case.mesh
+        ncell = nx * ny * (20 * 2 + (nz - 20) * 6)
+        workload = (f"second-order laminar Navier-Stokes flat-plate channel, Re=1e4 per unit length, Pr=0.72, "
+                    f"Sutherland, M=0.3, isothermal noSlip floor T=1; {nx}x{ny}x{nz} hexes: 20 prism "
+                    f"layers (first 1e-4, growth 1.15) under a Kuhn-tet core = {ncell} cells, {m.nnode} nodes, "
+                    f"{m.nedge} edges; farfield inflow/outflow, symmetry top and sides; unlimited MUSCL, "
+                    f"first-order FD Jacobian + analytic viscous Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 100 "
+                    f"iterations")
+    return case, base, workload
+
+
+def run_forms(case, base, warmup, profile_iters, iters_only, steps, max_iter, drop_goal):
+    """Warm up, profile and solve the case in the plain and the chosen form."""
+    from proteuscfd_amd import gpu
+    from proteuscfd_amd.cases import Params
+    results = {}
+    for prec, name in [(0, "plain_fp64"), (1, "chosen_fp32")]:
+        p = Params(**base, precision=prec)
+        ctx = gpu.Context(case, p, gradient_terms=9)
+        info = ctx.info()
+        ctx.upload_q(case.initial_state())
+        ctx.reset_iteration()
+        for _ in range(warmup):
+            ctx.iterate()
+        ctx.synchronize()
+        prof = phase_profile(ctx, case, profile_iters)
+        log(f"[bench] {name} phase ms/iter: " + ", ".join(f"{k}={v:.3f}" for k, v in prof.items()))
+        if iters_only:
+            ctx.upload_q(case.initial_state())
+            ctx.reset_iteration()
+            ctx.synchronize()
+            t = time.time()
+            for _ in range(iters_only):
+                r = ctx.iterate()
+            ctx.synchronize()
+            times, iters, hists = [time.time() - t], [iters_only], [np.array([r[0]])]
+        else:
+            times, iters, hists = timed_solves(ctx, case, steps, max_iter, drop_goal)
+        h = hists[-1]
+        drop = float(h[-1] / h[0]) if len(h) > 1 else 1.0
+        log(f"[bench] {name}: {iters} iterations, {['%.3f' % x for x in times]} s, drop {drop:.2e}")
+        qfinal = ctx.download_q()
+        results[name] = dict(times=times, iters=iters, prof=prof, info=info, drop=drop, q=qfinal)
+        ctx.close()
+    return results
+
+
+def field_error(results, nnode):
+    """Per-variable max error of the chosen form's converged field against the plain FP64 path."""
+    qp, qc = results["plain_fp64"]["q"][:nnode, :5], results["chosen_fp32"]["q"][:nnode, :5]
+    return {v: float(np.abs(qc[:, k] - qp[:, k]).max() / np.abs(qp[:, k]).max())
+            for k, v in enumerate(["rho", "rhou", "rhov", "rhow", "E"])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +234,8 @@ def main():
                     help="bump: second-order Euler bump (the headline workload); plate: laminar flat plate")
     ap.add_argument("--plate", default="200,40,180", help="plate hexes nx,ny,nz (the lowest 20 become prism layers)")
     ap.add_argument("--cfl", type=float, default=0.0, help="final CFL (default 50 bump, 100 plate)")
+    ap.add_argument("--no-plate", action="store_true",
+                    help="skip the laminar-plate record that follows the bump line")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -168,66 +246,15 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
-    from proteuscfd_amd import cases, gpu
-    from proteuscfd_amd.cases import Params
+    from proteuscfd_amd import gpu
 
     t0 = time.time()
-    if args.case == "bump":
-        base = dict(sorder=2, limiter=3, venkat_K=args.venkat_k, cfl=args.cfl or 50.0, cfl_start=1.0, cfl_ramp=20,
-                    nsgs=8)
-        case = cases.bump_case(n=args.n, params=Params(**base, precision=0))
-        m = case.mesh
-        workload = (f"second-order Euler, sin^2 bump in a unit cube, {args.n}^3 hexes -> 6 tets each "
-                    f"({m.nnode} nodes, {m.nedge} edges, {m.nbedge} half-edges), RCM, jitter 3%, "
-                    f"M=0.5 farfield, weighted LSQ (9 terms), modified Venkat K={args.venkat_k:g}, "
-                    f"first-order FD Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 20 iterations")
-    else:
-        nx, ny, nz = (int(v) for v in args.plate.split(","))
-        base = dict(sorder=2, limiter=0, cfl=args.cfl or 100.0, cfl_start=1.0, cfl_ramp=100, nsgs=8, viscous=1,
-                    reynolds=1.0e4, mach_ref=0.3)
-        case = cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=20, first_h=1e-4, growth=1.15, twall=1.0,
-                                top_bc=cases.BC_SYMMETRY, params=Params(**base, precision=0))
-        m = case.mesh
-        ncell = nx * ny * (20 * 2 + (nz - 20) * 6)
-        workload = (f"second-order laminar Navier-Stokes flat-plate channel, Re=1e4 per unit length, Pr=0.72, "
-                    f"Sutherland, M=0.3, isothermal noSlip floor T=1; {nx}x{ny}x{nz} hexes: 20 prism "
-                    f"layers (first 1e-4, growth 1.15) under a Kuhn-tet core = {ncell} cells, {m.nnode} nodes, "
-                    f"{m.nedge} edges; farfield inflow/outflow, symmetry top and sides; unlimited MUSCL, "
-                    f"first-order FD Jacobian + analytic viscous Jacobian, 8 SGS, CFL 1->{base['cfl']:g} over 100 "
-                    f"iterations")
+    case, base, workload = make_case(args.case, args)
+    m = case.mesh
     log(f"[bench] mesh {args.case}: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
         f"({time.time() - t0:.1f}s)")
-
-    results = {}
-    for prec, name in [(0, "plain_fp64"), (1, "chosen_fp32")]:
-        p = Params(**base, precision=prec)
-        ctx = gpu.Context(case, p, gradient_terms=9)
-        info = ctx.info()
-        # warmup
-        ctx.upload_q(case.initial_state())
-        ctx.reset_iteration()
-        for _ in range(args.warmup):
-            ctx.iterate()
-        ctx.synchronize()
-        prof = phase_profile(ctx, case, args.profile_iters)
-        log(f"[bench] {name} phase ms/iter: " + ", ".join(f"{k}={v:.3f}" for k, v in prof.items()))
-        if args.iters_only:
-            ctx.upload_q(case.initial_state())
-            ctx.reset_iteration()
-            ctx.synchronize()
-            t = time.time()
-            for _ in range(args.iters_only):
-                r = ctx.iterate()
-            ctx.synchronize()
-            times, iters, hists = [time.time() - t], [args.iters_only], [np.array([r[0]])]
-        else:
-            times, iters, hists = timed_solves(ctx, case, args.steps, args.max_iter, args.drop)
-        h = hists[-1]
-        drop = float(h[-1] / h[0]) if len(h) > 1 else 1.0
-        log(f"[bench] {name}: {iters} iterations, {['%.3f' % x for x in times]} s, drop {drop:.2e}")
-        qfinal = ctx.download_q()
-        results[name] = dict(times=times, iters=iters, prof=prof, info=info, drop=drop, q=qfinal)
-        ctx.close()
+    results = run_forms(case, base, args.warmup, args.profile_iters, args.iters_only, args.steps, args.max_iter,
+                        args.drop)
 
     plain, chosen = results["plain_fp64"], results["chosen_fp32"]
     t_plain = float(np.mean(plain["times"]))
@@ -268,9 +295,7 @@ def main():
     traffic = measured_traffic("k_sgs_color<float>") if (args.case == "bump" and args.n == 100) else None
     edge_flux = (m.nedge + m.nbedge) / (pp["k_residual"] * 1e-3)
     # converged-field error of the chosen form against the plain FP64 path
-    qp, qc = plain["q"][: m.nnode, :5], chosen["q"][: m.nnode, :5]
-    field_err = {v: float(np.abs(qc[:, k] - qp[:, k]).max() / np.abs(qp[:, k]).max())
-                 for k, v in enumerate(["rho", "rhou", "rhov", "rhow", "E"])}
+    field_err = field_error(results, m.nnode)
 
     cpu = None
     if not args.no_cpu:
@@ -326,7 +351,35 @@ def main():
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern["residual_2nd_fp64"]["frac"]},
         "cpu_baseline": cpu,
     }
+    if args.case == "bump" and not args.iters_only and not args.no_plate:
+        out["laminar_plate"] = plate_record(args)
     print(json.dumps(out))
+
+
+def plate_record(args):
+    """The laminar flat-plate channel (8M cells) solved in both forms: the
+    case the chosen form's 2x target is quoted on, reported beside the
+    headline bump line."""
+    t0 = time.time()
+    case, base, workload = make_case("plate", args)
+    m = case.mesh
+    log(f"[bench] mesh plate: {m.nnode} nodes {m.nedge} edges ({time.time() - t0:.1f}s)")
+    res = run_forms(case, base, 2, 5, 0, 1, 4000, args.drop)
+    tp = float(np.mean(res["plain_fp64"]["times"]))
+    tc = float(np.mean(res["chosen_fp32"]["times"]))
+    ip, ic = res["plain_fp64"]["iters"][0], res["chosen_fp32"]["iters"][0]
+    return {
+        "workload": workload,
+        "plain_fp64_s": tp,
+        "chosen_fp32_s": tc,
+        "speedup_chosen_vs_plain": tp / tc if tc > 0 else None,
+        "iterations": {"plain": ip, "chosen": ic},
+        "residual_drop": {"plain": res["plain_fp64"]["drop"], "chosen": res["chosen_fp32"]["drop"]},
+        "wall_ms_per_iteration": {"plain": 1e3 * tp / max(ip, 1), "chosen": 1e3 * tc / max(ic, 1)},
+        "chosen_vs_plain_field_max_rel_err": field_error(res, m.nnode),
+        "phase_ms": {"plain": res["plain_fp64"]["prof"], "chosen": res["chosen_fp32"]["prof"]},
+        "cell_iterations_per_s": m.nnode * ic / tc if tc > 0 else None,
+    }
 
 
 if __name__ == "__main__":

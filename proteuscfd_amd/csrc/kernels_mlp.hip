@@ -777,8 +777,8 @@ struct SurrogateDev {
 
 __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, SurrogateDev sd, int n,
                                                         const double* __restrict__ rhoi, const double* __restrict__ T,
-                                                        double* __restrict__ wdot, int* __restrict__ list,
-                                                        int* __restrict__ nlist) {
+                                                        double* __restrict__ wdot, int* __restrict__ lst,
+                                                        int* __restrict__ cnt) {
   __shared__ InferTiles tiles[kWaves];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
@@ -815,6 +815,7 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
     for (int f = kNin; f < kKin; ++f) Tl.X[lane][f] = 0;
     wave_sync();
     infer64(R, Tl, r, q);
+    bool fb = false;
     if (valid) {
       double wv5[5], m[5], mag = 0.0;
 #pragma unroll
@@ -831,18 +832,59 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
       const bool bad = (eN + eO) > sd.tol * (mag + rho * sd.floor);
 #pragma unroll
       for (int k = 0; k < 5; ++k) wdot[(long long)c * 5 + k] = wv5[k];
-      if (out || bad) list[atomicAdd(nlist, 1)] = c;
+      fb = out || bad;
     }
+    // the pass's fallback cells, compacted in lane order into its own 64
+    // slots (no atomics): cnt[pass] of them at lst[pass*64 ...]
+    const unsigned long long mask = __ballot(fb);
+    if (fb) lst[(long long)pass * 64 + __popcll(mask & ((1ull << lane) - 1ull))] = c;
+    if (lane == 0) cnt[pass] = __popcll(mask);
     wave_sync();
   }
 }
 
-__global__ void __launch_bounds__(kChemBlock) k_wdot_list(const ChemDev* __restrict__ md, const int* __restrict__ list,
-                                                         const int* __restrict__ nlist, const double* __restrict__ rhoi,
+// Exclusive prefix sum of the per-pass fallback counts (one workgroup, fixed
+// order): off[p] = sum of cnt[0..p-1], off[npass] = total.
+__global__ void __launch_bounds__(kMlpBlock) k_fallback_offsets(int npass, const int* __restrict__ cnt,
+                                                               int* __restrict__ off) {
+  __shared__ int sh[kMlpBlock];
+  const int t = threadIdx.x;
+  const int per = (npass + kMlpBlock - 1) / kMlpBlock;
+  const int p0 = min(npass, t * per), p1 = min(npass, p0 + per);
+  int local = 0;
+  for (int p = p0; p < p1; ++p) local += cnt[p];
+  sh[t] = local;
+  __syncthreads();
+  // inclusive scan of the thread totals (Hillis-Steele)
+  for (int d = 1; d < kMlpBlock; d <<= 1) {
+    const int v = t >= d ? sh[t - d] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = sh[t] - local;
+  for (int p = p0; p < p1; ++p) {
+    off[p] = run;
+    run += cnt[p];
+  }
+  if (t == kMlpBlock - 1) off[npass] = sh[t];
+}
+
+// Exact rates of the fallback cells: entry t of the compacted list lives in
+// the pass p with off[p] <= t < off[p+1] (binary search).
+__global__ void __launch_bounds__(kChemBlock) k_wdot_list(const ChemDev* __restrict__ md, int npass,
+                                                         const int* __restrict__ lst, const int* __restrict__ off,
+                                                         const double* __restrict__ rhoi,
                                                          const double* __restrict__ T, double* __restrict__ wdot) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= *nlist) return;
-  const int c = list[t];
+  if (t >= off[npass]) return;
+  int lo = 0, hi = npass;  // off[lo] <= t < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  const int c = lst[(long long)lo * 64 + (t - off[lo])];
   const ChemDev& m = *md;
   double r[PCFD_CHEM_MAXS], w[PCFD_CHEM_MAXS];
   for (int k = 0; k < m.ns; ++k) r[k] = rhoi[(long long)c * m.ns + k];
@@ -891,23 +933,26 @@ int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_s
     sd.rho_max = d->rho_max;
     sd.tol = d->tol;
     sd.floor = d->floor;
-    DBuf dlist(sizeof(int) * (size_t)n + sizeof(int));
-    int* nlist = dlist.as<int>() + n;
-    mcheck(hipMemsetAsync(nlist, 0, sizeof(int), m.stream), "hipMemset");
     // each wavefront loads its weight fragments once and loops over 64-sample
     // passes: about four blocks per CU
     const int npass = (n + 63) / 64;
     const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
+    // fallback list: 64 slots per pass, per-pass counts, their offsets
+    DBuf dlst(sizeof(int) * (size_t)npass * 64), dcnt(sizeof(int) * (size_t)npass),
+        doff(sizeof(int) * ((size_t)npass + 1));
     mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
     hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
-                       dlist.as<int>(), nlist);
+                       dlst.as<int>(), dcnt.as<int>());
     mcheck(hipGetLastError(), "k_surrogate");
+    hipLaunchKernelGGL(k_fallback_offsets, dim3(1), dim3(kMlpBlock), 0, m.stream, npass, dcnt.as<int>(),
+                       doff.as<int>());
+    mcheck(hipGetLastError(), "k_fallback_offsets");
     hipLaunchKernelGGL(k_wdot_list, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, m.stream,
-                       chem_device_model(chem), dlist.as<int>(), nlist, rhoi, T, wdot);
+                       chem_device_model(chem), npass, dlst.as<int>(), doff.as<int>(), rhoi, T, wdot);
     mcheck(hipGetLastError(), "k_wdot_list");
     mcheck(hipEventRecord(m.e1, m.stream), "hipEventRecord");
     int nf = 0;
-    mcheck(hipMemcpyAsync(&nf, nlist, sizeof(int), hipMemcpyDeviceToHost, m.stream), "copy");
+    mcheck(hipMemcpyAsync(&nf, doff.as<int>() + npass, sizeof(int), hipMemcpyDeviceToHost, m.stream), "copy");
     mcheck(hipStreamSynchronize(m.stream), "hipStreamSynchronize");
     mcheck(hipEventElapsedTime(&m.last_ms, m.e0, m.e1), "hipEventElapsedTime");
     if (nfallback) *nfallback = nf;
