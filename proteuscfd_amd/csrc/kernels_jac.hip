@@ -180,6 +180,21 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     xi[2] = A.xyz[2 * A.nnode + i];
   }
   // --- interior edges (Kernel_NumJac): row blocks + diagonal column sums ---
+  // Each edge flux is evaluated from this node's side: G(x, y) = F(x, y, n_i)
+  // with n_i the face normal pointing out of node i.  The Roe flux with the
+  // Harten-Hyman fix is antisymmetric, F(L, R, a) = -F(R, L, -a), so the
+  // reference's F(L, R, a) is +G when i is the left node and -G when it is
+  // the right node, and the two signs cancel in what the row needs:
+  //   diagonal column c:   -A[j][i] column c = (G(q_i + h e_c, q_j) - G(q_i, q_j)) / h
+  //   off-diagonal column c:  A[i][j] column c = (G(q_i, q_j + h e_c) - G(q_i, q_j)) / h
+  // (the same one-sided differences as the reference's (L,R) evaluation up
+  // to the rounding of the flux).  The own node's states, unperturbed and
+  // perturbed in component c, are formed once per row.
+  double qiP[5];
+  perturb(qi, c, h, qiP);
+  RoeSide S0, SP;
+  roe_side_state(qi, gamma, S0);
+  roe_side_state(qiP, gamma, SP);
   const long long gb = (long long)A.gbase[i >> 6] * kW + (i & 63);
   const int w = A.gwidth[i >> 6];
   T* ob = Moff + (long long)A.sbase[chunk] * 25 * kW + lane + (long long)c * kW;
@@ -189,43 +204,31 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
     if (j < 0) break;
     const bool left = i < j;
     const double a[4] = {A.gnrm[gi], A.gnrm[A.gstride + gi], A.gnrm[2 * A.gstride + gi], A.gnrm[3 * A.gstride + gi]};
-    double qj[5];
+    const double sg = left ? 1.0 : -1.0;
+    const double ai[4] = {sg * a[0], sg * a[1], sg * a[2], a[3]};  // out of node i
+    double qj[5], qjP[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) qj[k] = A.q[k * A.ntot + j];
-    const double* QL = left ? qi : qj;
-    const double* QR = left ? qj : qi;
-    RoeSide SL, SR, SP;
-    roe_side(QL, a, gamma, SL);
-    roe_side(QR, a, gamma, SR);
-    double fS[5], fP[5], QP[5];
-    roe_flux_s(QL, SL, QR, SR, a, gamma, A.sym2d, fS);
+    perturb(qj, c, h, qjP);
+    roe_side_th(S0, ai);
+    roe_side_th(SP, ai);
+    RoeSide Sj, SjP;
+    roe_side(qj, ai, gamma, Sj);
+    double fS[5], fP[5];
+    roe_flux_s(qi, S0, qj, Sj, ai, gamma, A.sym2d, fS);
     T* blk = ob + (long long)s * 25 * kW;
-    // the three evaluations are retired one after another (scheduler
-    // fences keep them from being interleaved, which would triple the
-    // live registers)
-    // offc: column c of this row's block; colc: column c of the other
-    // row's block (this row's diagonal gets its negative)
-    double offc[5], colc[5];
+    // offc: column c of this row's block A[i][j]; dcol: this edge's part of
+    // column c of the diagonal block
+    double offc[5], dcol[5];
     __builtin_amdgcn_sched_barrier(0);
-    perturb(QL, c, h, QP);
-    roe_side(QP, a, gamma, SP);
-    roe_flux_s(QP, SP, QR, SR, a, gamma, A.sym2d, fP);
+    roe_flux_s(qiP, SP, qj, Sj, ai, gamma, A.sym2d, fP);
 #pragma unroll
-    for (int rr = 0; rr < 5; ++rr) {
-      const double aRL = (fS[rr] - fP[rr]) * kInvH;  // A[R][L] = -dF/dQL
-      if (left) colc[rr] = aRL;
-      else offc[rr] = aRL;
-    }
+    for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fP[rr] - fS[rr]) * kInvH;
     __builtin_amdgcn_sched_barrier(0);
-    perturb(QR, c, h, QP);
-    roe_side(QP, a, gamma, SP);
-    roe_flux_s(QL, SL, QP, SP, a, gamma, A.sym2d, fP);
+    roe_side(qjP, ai, gamma, SjP);
+    roe_flux_s(qi, S0, qjP, SjP, ai, gamma, A.sym2d, fP);
 #pragma unroll
-    for (int rr = 0; rr < 5; ++rr) {
-      const double aLR = (fP[rr] - fS[rr]) * kInvH;  // A[L][R] = +dF/dQR
-      if (left) offc[rr] = aLR;
-      else colc[rr] = aLR;
-    }
+    for (int rr = 0; rr < 5; ++rr) offc[rr] = (fP[rr] - fS[rr]) * kInvH;
     __builtin_amdgcn_sched_barrier(0);
     if (VISC) {
       // Kernel_Viscous_Jac (ucs/jacobian.tcc:734-767): A[R][L] += aL,
@@ -245,14 +248,14 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) {
         offc[rr] += left ? vR[rr] : vL[rr];
-        colc[rr] += left ? vL[rr] : vR[rr];
+        dcol[rr] -= left ? vL[rr] : vR[rr];
       }
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int rr = 0; rr < 5; ++rr) {
       blk[rr * 5 * kW] = (T)offc[rr];
-      Dc[rr] += -colc[rr];
+      Dc[rr] += dcol[rr];
     }
   }
   // --- temporal term, steady pseudo-time: A_ii += 1*vol/dtau ---

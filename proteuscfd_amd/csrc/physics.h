@@ -190,6 +190,23 @@ PCFD_DEV void roe_side(const double* Q, const double* n, double gamma, RoeSide& 
   s.c = sqrt(gamma * s.p * s.ir);
 }
 
+// The normal-independent part of roe_side, and its normal velocity: for a
+// state that meets many faces (a node's own state in an edge loop).
+PCFD_DEV void roe_side_state(const double* Q, double gamma, RoeSide& s) {
+  const double gm1 = gamma - 1.0;
+  s.ir = 1.0 / Q[0];
+  s.u = Q[1] * s.ir;
+  s.v = Q[2] * s.ir;
+  s.w = Q[3] * s.ir;
+  const double v2 = s.u * s.u + s.v * s.v + s.w * s.w;
+  s.p = gm1 * (Q[4] - 0.5 * Q[0] * v2);
+  s.h = (Q[4] + s.p) * s.ir;
+  s.th = 0.0;
+  s.c = sqrt(gamma * s.p * s.ir);
+}
+
+PCFD_DEV void roe_side_th(RoeSide& s, const double* n) { s.th = s.u * n[0] + s.v * n[1] + s.w * n[2]; }
+
 PCFD_DEV double hh_fix(double e, double eL, double eR) {
   const double eps = dmax(0.0, dmax(e - eL, eR - e));
   return (fabs(e) < eps) ? 0.5 * (e * e / eps + eps) : fabs(e);
