@@ -843,31 +843,49 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
   }
 }
 
-// Exclusive prefix sum of the per-pass fallback counts (one workgroup, fixed
-// order): off[p] = sum of cnt[0..p-1], off[npass] = total.
-__global__ void __launch_bounds__(kMlpBlock) k_fallback_offsets(int npass, const int* __restrict__ cnt,
-                                                               int* __restrict__ off) {
-  __shared__ int sh[kMlpBlock];
-  const int t = threadIdx.x;
-  const int per = (npass + kMlpBlock - 1) / kMlpBlock;
-  const int p0 = min(npass, t * per), p1 = min(npass, p0 + per);
-  int local = 0;
-  for (int p = p0; p < p1; ++p) local += cnt[p];
-  sh[t] = local;
+// Exclusive prefix sum of the per-pass fallback counts (one workgroup of
+// 1024, fixed order): off[p] = sum of cnt[0..p-1], off[npass] = total.
+// Counts are read in coalesced tiles of 1024, up to kScanTiles tiles at a
+// time so their loads are in flight together; each tile is scanned with
+// wavefront shuffles plus one pass over the 16 wavefront totals.
+constexpr int kScanBlock = 1024;
+constexpr int kScanTiles = 16;
+
+__global__ void __launch_bounds__(kScanBlock) k_fallback_offsets(int npass, const int* __restrict__ cnt,
+                                                                int* __restrict__ off) {
+  __shared__ int wsum[kScanBlock / 64];
+  __shared__ int carry_sh;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry_sh = 0;
   __syncthreads();
-  // inclusive scan of the thread totals (Hillis-Steele)
-  for (int d = 1; d < kMlpBlock; d <<= 1) {
-    const int v = t >= d ? sh[t - d] : 0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
+  for (int base = 0; base < npass; base += kScanBlock * kScanTiles) {
+    int v[kScanTiles];
+#pragma unroll
+    for (int k = 0; k < kScanTiles; ++k) {
+      const int p = base + k * kScanBlock + t;
+      v[k] = p < npass ? cnt[p] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanTiles; ++k) {
+      // inclusive scan within the wavefront
+      int x = v[k];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63) wsum[wv] = x;
+      __syncthreads();
+      int before = carry_sh;
+      for (int u = 0; u < wv; ++u) before += wsum[u];
+      const int p = base + k * kScanBlock + t;
+      if (p < npass) off[p] = before + x - v[k];
+      __syncthreads();
+      if (t == kScanBlock - 1) carry_sh = before + x;
+      __syncthreads();
+    }
   }
-  int run = sh[t] - local;
-  for (int p = p0; p < p1; ++p) {
-    off[p] = run;
-    run += cnt[p];
-  }
-  if (t == kMlpBlock - 1) off[npass] = sh[t];
+  if (t == 0) off[npass] = carry_sh;
 }
 
 // Exact rates of the fallback cells: entry t of the compacted list lives in
@@ -944,7 +962,7 @@ int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_s
     hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
                        dlst.as<int>(), dcnt.as<int>());
     mcheck(hipGetLastError(), "k_surrogate");
-    hipLaunchKernelGGL(k_fallback_offsets, dim3(1), dim3(kMlpBlock), 0, m.stream, npass, dcnt.as<int>(),
+    hipLaunchKernelGGL(k_fallback_offsets, dim3(1), dim3(kScanBlock), 0, m.stream, npass, dcnt.as<int>(),
                        doff.as<int>());
     mcheck(hipGetLastError(), "k_fallback_offsets");
     hipLaunchKernelGGL(k_wdot_list, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, m.stream,
