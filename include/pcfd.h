@@ -85,7 +85,22 @@ typedef struct {
   double prandtl;        /* prandtlNumber (0.72) */
   double prandtl_t;      /* PrT (0.85, ucs/param.tcc:89) */
   double sutherland_s;   /* 110.4 K / ref_temperature (ucs/eqnset.h:251-268) */
+  /* Options that are off in the reference's defaults; 0 selects the default. */
+  int gradient_type;     /* gradientType: 0 weighted LSQ, 1 Green-Gauss (ucs/gradient.tcc:56-112, 172-250) */
+  int global_dt;         /* 1: localTimeStepping off -- one dt everywhere: timeStep if fixed_dt > 0, else the
+                            minimum over nodes (ucs/timestep.tcc:48-74) */
+  double fixed_dt;       /* timeStep (non-dimensional); used with global_dt when > 0 */
+  int enable_vnn;        /* enableVNN: dt = min(dt, VNN vol^(2/3)) (ucs/timestep.tcc:37-45) */
+  double vnn;            /* VNN (reference default 20) */
+  int dynamic_cfl;       /* dynamicCFL: CFL changes by +-10% per iteration with the residual trend
+                            (ucs/param.tcc:697-713, ucs/solutionSpace.tcc:629-633) */
+  int jac_field_type;    /* jacobianFieldType: 0 one-sided FD (h = 1e-8), 1 centred FD (ucs/jacobian.tcc:307-367) */
+  int flux_type;         /* PCFD_FLUX_ROE (roeFlux, the default) or PCFD_FLUX_ALGEBRAIC (algebraicFlux,
+                            ucs/eqnset.tcc:56-77) */
 } pcfd_params;
+
+#define PCFD_FLUX_ROE 0
+#define PCFD_FLUX_ALGEBRAIC 1
 
 const char* pcfd_last_error(const pcfd_ctx* ctx);
 const char* pcfd_version(void);

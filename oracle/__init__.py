@@ -31,7 +31,9 @@ class OParams(C.Structure):
                 ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double),
                 ("cfl_ramp", C.c_int), ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int),
                 ("clip_jacobi", C.c_int), ("viscous", C.c_int), ("re_tilde", C.c_double), ("pr", C.c_double),
-                ("prt", C.c_double), ("sutherland_s", C.c_double)]
+                ("prt", C.c_double), ("sutherland_s", C.c_double), ("gradient_type", C.c_int),
+                ("global_dt", C.c_int), ("fixed_dt", C.c_double), ("enable_vnn", C.c_int), ("vnn", C.c_double),
+                ("dynamic_cfl", C.c_int), ("jac_field_type", C.c_int), ("flux_type", C.c_int)]
 
 
 class OChem(C.Structure):
@@ -52,7 +54,7 @@ class OSolver(C.Structure):
     _fields_ = [("m", C.POINTER(OMesh)), ("p", OParams), ("q", dp), ("grad", dp), ("l", dp), ("sw", dp),
                 ("b", dp), ("x", dp), ("dt", dp), ("A", OCrs), ("iter", C.c_int), ("cfl", C.c_double),
                 ("order", ip), ("t_jac", C.c_double), ("t_grad", C.c_double), ("t_res", C.c_double),
-                ("t_lin", C.c_double), ("t_upd", C.c_double)]
+                ("t_lin", C.c_double), ("t_upd", C.c_double), ("res", C.c_double), ("res_nm1", C.c_double)]
 
 
 def lib():
@@ -74,12 +76,15 @@ def lib():
         L.o_update_bcs.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp]
         L.o_lsq_moments.argtypes = [C.POINTER(OMesh), dp]
         L.o_gradient.argtypes = [C.POINTER(OMesh), dp, dp, dp]
+        L.o_gradient_gg.argtypes = [C.POINTER(OMesh), dp, dp]
         L.o_limiter.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, dp, dp]
         L.o_residual.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), C.c_int, dp, dp, dp, dp, dp]
         L.o_timestep.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, C.c_double, dp]
         L.o_timestep.restype = C.c_double
         L.o_cfl.argtypes = [C.POINTER(OParams), C.c_int]
         L.o_cfl.restype = C.c_double
+        L.o_cfl_update.argtypes = [C.POINTER(OParams), C.c_int, C.c_double, C.c_double, C.c_double]
+        L.o_cfl_update.restype = C.c_double
         L.o_crs_init.argtypes = [C.POINTER(OMesh), C.POINTER(OCrs)]
         L.o_crs_free.argtypes = [C.POINTER(OCrs)]
         L.o_jacobian.argtypes = [C.POINTER(OMesh), C.POINTER(OParams), dp, dp, C.POINTER(OCrs)]
@@ -144,7 +149,9 @@ class Problem:
                          params.limiter_refresh, params.limiter_freeze, params.cfl, params.cfl_start,
                          params.cfl_ramp, params.nsgs, params.symmetry2d, params.jac_freq, int(clip_jacobi),
                          int(params.viscous), params.re_tilde, params.prandtl, params.prandtl_t,
-                         params.sutherland_s)
+                         params.sutherland_s, params.gradient_type, params.global_dt, params.fixed_dt,
+                         params.enable_vnn, params.vnn, params.dynamic_cfl, params.jac_field_type,
+                         params.flux_type)
 
     @property
     def bnormal(self):
@@ -177,9 +184,12 @@ class Problem:
         return sw
 
     def gradient(self, q, sw=None):
+        g = np.zeros((self.nnode, 27))
+        if self.p.gradient_type == 1:
+            lib().o_gradient_gg(C.byref(self.m), _d(_f64(q)), _d(g))
+            return g
         if sw is None:
             sw = self.lsq_moments()
-        g = np.zeros((self.nnode, 27))
         lib().o_gradient(C.byref(self.m), _d(_f64(q)), _d(_f64(sw)), _d(g))
         return g
 
@@ -203,6 +213,9 @@ class Problem:
         dt = np.zeros(self.nnode)
         dtmin = lib().o_timestep(C.byref(self.m), C.byref(self.p), _d(_f64(q)), cfl, _d(dt))
         return dt, dtmin
+
+    def cfl_update(self, it, res, res_nm1, cfl_prev):
+        return lib().o_cfl_update(C.byref(self.p), it, res, res_nm1, cfl_prev)
 
     def jacobian(self, q, dt):
         """Returns (ia, ja, iau, M[nblocks,5,5], q_after) -- q_after holds the

@@ -507,6 +507,68 @@ void o_gradient(const o_mesh* m, const double* q, const double* sw, double* grad
   }
 }
 
+/* EqnSet::NumericalFlux (ucs/eqnset.tcc:56-88): fluxType 1 (algebraicFlux)
+ * is the average of the two physical fluxes, 0.5*|a|*(F(QL)+F(QR)), with the
+ * NaN components zeroed; otherwise the Roe flux. */
+static void numflux(const o_params* p, const double* QL, const double* QR, const double* a, double* F) {
+  if (p->flux_type == 1) {
+    double f2[5];
+    phys_flux(QL, a, 0.0, p->gamma, F);
+    phys_flux(QR, a, 0.0, p->gamma, f2);
+    for (int i = 0; i < NE; i++) {
+      F[i] = 0.5 * a[3] * (F[i] + f2[i]);
+      if (isnan(F[i])) F[i] = 0.0;
+    }
+    return;
+  }
+  o_roe_flux(QL, QR, a, 0.0, p->gamma, p->symmetry2d, F);
+}
+
+/* Green-Gauss gradient (gradientType 1): Kernel_Green_Gauss_Gradient and
+ * Bkernel_Green_Gauss_Gradient (ucs/gradient.tcc:172-250) add the face
+ * average times a*|a| to the left node and subtract it from the right one
+ * (boundary half-edges: to the node, with the phantom as the other state),
+ * then Gradient::Compute divides by the volume and applies the symmetry
+ * fix (:68-95, 544-565). */
+void o_gradient_gg(const o_mesh* m, const double* q, double* grad) {
+  memset(grad, 0, sizeof(double) * NT * 3 * (size_t)m->nnode);
+  for (int e = 0; e < m->nedge; e++) {
+    int L = m->edges[2 * e], R = m->edges[2 * e + 1];
+    const double* a = &m->edge_a[4 * e];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < NT; j++) {
+        int k = kGradList[j];
+        double f = 0.5 * (q[NV * L + k] + q[NV * R + k]);
+        double t = f * a[i] * a[3];
+        grad[NT * 3 * L + 3 * j + i] += t;
+        grad[NT * 3 * R + 3 * j + i] += -t;
+      }
+  }
+  for (int e = 0; e < m->nbnode; e++) {
+    int L = m->bedges[2 * e], R = m->bedges[2 * e + 1];
+    const double* a = &m->bedge_a[4 * e];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < NT; j++) {
+        int k = kGradList[j];
+        double f = 0.5 * (q[NV * L + k] + q[NV * R + k]);
+        grad[NT * 3 * L + 3 * j + i] += f * a[i] * a[3];
+      }
+  }
+  for (int n = 0; n < m->nnode; n++)
+    for (int j = 0; j < NT * 3; j++) grad[NT * 3 * n + j] /= m->vol[n];
+  for (int e = 0; e < m->nbnode; e++) {
+    int tag = m->bfactag[e];
+    if (m->bc_type[tag] != O_BC_SYMMETRY) continue;
+    int L = m->bedges[2 * e];
+    const double* a = &m->bedge_a[4 * e];
+    double* g = &grad[NT * 3 * L];
+    for (int i = 0; i < NT; i++) {
+      double d = g[3 * i] * a[0] + g[3 * i + 1] * a[1] + g[3 * i + 2] * a[2];
+      for (int j = 0; j < 3; j++) g[3 * i + j] -= d * a[j];
+    }
+  }
+}
+
 /* ExtrapolateVariables + ExtrapolateCorrection: ucs/compressible.tcc:1038-1052,
  * ucs/eqnset.h:231-238 */
 static void extrapolate(double* Qho, const double* q, const double* dQ, const double* g, const double* dx,
@@ -886,7 +948,7 @@ void o_residual(const o_mesh* m, const o_params* p, int iter, const double* q, c
       o_aux(QL, p->gamma);
       o_aux(QR, p->gamma);
     }
-    o_roe_flux(QL, QR, &m->edge_a[4 * e], 0.0, p->gamma, p->symmetry2d, F);
+    numflux(p, QL, QR, &m->edge_a[4 * e], F);
     for (int i = 0; i < NE; i++) {
       b[NE * L + i] += -F[i];
       b[NE * R + i] += F[i];
@@ -895,7 +957,7 @@ void o_residual(const o_mesh* m, const o_params* p, int iter, const double* q, c
   for (int e = 0; e < m->nbnode; e++) {
     int L = m->bedges[2 * e], R = m->bedges[2 * e + 1];
     double F[5];
-    o_roe_flux(&q[NV * L], &q[NV * R], &m->bedge_a[4 * e], 0.0, p->gamma, p->symmetry2d, F);
+    numflux(p, &q[NV * L], &q[NV * R], &m->bedge_a[4 * e], F);
     for (int i = 0; i < NE; i++) b[NE * L + i] += -F[i];
   }
   if (p->viscous) viscous_residual(m, p, iter, q, grad, b);
@@ -925,6 +987,10 @@ static double max_eig(const double* Q, const double* a, double gamma) {
 double o_timestep(const o_mesh* m, const o_params* p, const double* q, double cfl, double* dt) {
   int nn = m->nnode;
   for (int i = 0; i < nn; i++) dt[i] = 0.0;
+  if (p->global_dt && p->fixed_dt > 0.0) { /* ucs/timestep.tcc:48-55 */
+    for (int i = 0; i < nn; i++) dt[i] = p->fixed_dt;
+    return p->fixed_dt;
+  }
   for (int e = 0; e < m->nedge; e++) {
     int L = m->edges[2 * e], R = m->edges[2 * e + 1];
     double t[NV];
@@ -942,10 +1008,24 @@ double o_timestep(const o_mesh* m, const o_params* p, const double* q, double cf
     o_aux(t, p->gamma);
     dt[L] += max_eig(t, &m->bedge_a[4 * e], p->gamma) * m->bedge_a[4 * e + 3];
   }
+  if (p->global_dt) {
+    /* one dt, the minimum over nodes (ucs/timestep.tcc:56-74); as in the
+     * reference, the VNN limit is applied from node 1 on */
+    double dtmin = cfl * (m->vol[0] / dt[0]);
+    for (int i = 1; i < nn; i++) {
+      dtmin = omin(dtmin, cfl * (m->vol[i] / dt[i]));
+      if (p->enable_vnn) dtmin = omin(dtmin, p->vnn * pow(m->vol[i], 2.0 / 3.0));
+    }
+    for (int i = 0; i < nn; i++) dt[i] = dtmin;
+    return dtmin;
+  }
+  /* local time stepping (ucs/timestep.tcc:30-47); the VNN limit is applied
+   * from node 1 on, as in the reference */
   dt[0] = cfl * (m->vol[0] / dt[0]);
   double dtmin = dt[0];
   for (int i = 1; i < nn; i++) {
     dt[i] = cfl * (m->vol[i] / dt[i]);
+    if (p->enable_vnn) dt[i] = omin(dt[i], p->vnn * pow(m->vol[i], 2.0 / 3.0));
     dtmin = omin(dtmin, dt[i]);
   }
   return dtmin;
@@ -956,6 +1036,21 @@ double o_cfl(const o_params* p, int iter) {
   if (p->cfl_ramp > 0 && iter < p->cfl_ramp)
     return (p->cfl - p->cfl_start) / (double)p->cfl_ramp * (double)iter + p->cfl_start;
   return p->cfl;
+}
+
+/* Param::UpdateCFL (ucs/param.tcc:695-726) with the residual trend of
+ * PreTimeAdvance (ucs/solutionSpace.tcc:629-633): res = residGlobal of the
+ * previous iteration, res_nm1 of the one before (0 until they exist). */
+double o_cfl_update(const o_params* p, int iter, double res, double res_nm1, double cfl_prev) {
+  if (!p->dynamic_cfl) return o_cfl(p, iter);
+  double delta = 9999.0;
+  if (res_nm1 > 1.0e-21) delta = (res_nm1 - res) / res_nm1;
+  if (iter > 1) {
+    double sgn = delta > 0.0 ? 1.0 : -1.0;
+    double A = omax(0.0, cfl_prev + 0.1 * sgn * cfl_prev);
+    return omin(A, p->cfl);
+  }
+  return p->cfl_start;
 }
 
 static int cmp_int(const void* a, const void* b) {
@@ -1041,9 +1136,34 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
     const double* QR = &q[NV * R];
     const double* a = &m->edge_a[4 * e];
     double fS[5], fL[5], fR[5], QPL[NV], QPR[NV];
-    o_roe_flux(QL, QR, a, 0.0, g, p->symmetry2d, fS);
     double* bRL = &A->M[25 * (size_t)A->e_rl[e]];
     double* bLR = &A->M[25 * (size_t)A->e_lr[e]];
+    if (p->jac_field_type == 1) {
+      /* Kernel_NumJac_Centered (ucs/jacobian.tcc:307-367) */
+      double fLd[5], fRd[5];
+      for (int i = 0; i < NE; i++) {
+        memcpy(QPL, QL, sizeof(QPL));
+        memcpy(QPR, QR, sizeof(QPR));
+        QPL[i] += h;
+        QPR[i] += h;
+        o_aux(QPL, g);
+        o_aux(QPR, g);
+        numflux(p, QPL, QR, a, fL);
+        numflux(p, QL, QPR, a, fR);
+        memcpy(QPL, QL, sizeof(QPL));
+        memcpy(QPR, QR, sizeof(QPR));
+        QPL[i] -= h;
+        QPR[i] -= h;
+        o_aux(QPL, g);
+        o_aux(QPR, g);
+        numflux(p, QPL, QR, a, fLd);
+        numflux(p, QL, QPR, a, fRd);
+        for (int j = 0; j < NE; j++) bRL[j * NE + i] += (fLd[j] - fL[j]) / (2.0 * h);
+        for (int j = 0; j < NE; j++) bLR[j * NE + i] += (fR[j] - fRd[j]) / (2.0 * h);
+      }
+      continue;
+    }
+    numflux(p, QL, QR, a, fS);
     for (int i = 0; i < NE; i++) {
       memcpy(QPL, QL, sizeof(QPL));
       memcpy(QPR, QR, sizeof(QPR));
@@ -1051,8 +1171,8 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
       QPR[i] += h;
       o_aux(QPL, g);
       o_aux(QPR, g);
-      o_roe_flux(QPL, QR, a, 0.0, g, p->symmetry2d, fL);
-      o_roe_flux(QL, QPR, a, 0.0, g, p->symmetry2d, fR);
+      numflux(p, QPL, QR, a, fL);
+      numflux(p, QL, QPR, a, fR);
       for (int j = 0; j < NE; j++) bRL[j * NE + i] += (fS[j] - fL[j]) / h;
       for (int j = 0; j < NE; j++) bLR[j * NE + i] += (fR[j] - fS[j]) / h;
     }
@@ -1069,7 +1189,7 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
     double tw = tag_twall(m, tag);
     const double* ns = normal_state(m, q, e);
     o_bc_state_ex(bct, QL, QR, Qref, a, g, tw, ns);
-    o_roe_flux(QL, QR, a, 0.0, g, p->symmetry2d, fS);
+    numflux(p, QL, QR, a, fS);
     double* D = &A->M[25 * (size_t)A->iau[L]];
     for (int i = 0; i < NE; i++) {
       memcpy(QPL, QL, sizeof(QPL));
@@ -1078,7 +1198,7 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
       memcpy(QPR, QR, sizeof(QPR));
       o_aux(QPR, g);
       o_bc_state_ex(bct, QPL, QPR, Qref, a, g, tw, ns);
-      o_roe_flux(QPL, QPR, a, 0.0, g, p->symmetry2d, fL);
+      numflux(p, QPL, QPR, a, fL);
       for (int j = 0; j < NE; j++) D[j * NE + i] += (fL[j] - fS[j]) / h;
     }
   }
@@ -1260,7 +1380,8 @@ int o_solver_init(o_solver* s, const o_mesh* m, const o_params* p, const double*
   if (p->nsgs > 0) o_crs_init(m, &s->A);
   s->iter = 1;
   /* PreIterate: UpdateCFL + ComputeTimesteps, ucs/solutionSpace.tcc:509-569 */
-  s->cfl = o_cfl(p, s->iter);
+  s->cfl = o_cfl_update(p, s->iter, 0.0, 0.0, p->cfl_start); /* PreIterate (ucs/solutionSpace.tcc:546-551) */
+  s->res = s->res_nm1 = 0.0;
   o_timestep(m, &s->p, s->q, s->cfl, s->dt);
   return 0;
 }
@@ -1276,11 +1397,14 @@ double o_solver_step(o_solver* s, double* norms, int* nzeroed) {
     o_jacobian(m, p, s->q, s->dt, &s->A);
   double t1 = now_s();
   s->t_jac += t1 - t0;
-  s->cfl = o_cfl(p, s->iter);
+  s->cfl = o_cfl_update(p, s->iter, s->res, s->res_nm1, s->cfl);
   o_timestep(m, p, s->q, s->cfl, s->dt);
   o_update_bcs(m, p, s->q);
   double t2 = now_s();
-  if ((p->sorder > 1 && s->iter > p->nfirst) || p->viscous) o_gradient(m, s->q, s->sw, s->grad);
+  if ((p->sorder > 1 && s->iter > p->nfirst) || p->viscous) {
+    if (p->gradient_type == 1) o_gradient_gg(m, s->q, s->grad);
+    else o_gradient(m, s->q, s->sw, s->grad);
+  }
   int lr = p->limiter_refresh > 0 ? p->limiter_refresh : 1;
   if ((s->iter % lr == 0) && p->sorder > 1 && p->limiter && s->iter > p->nfirst && s->iter < p->limiter_freeze)
     o_limiter(m, p, s->q, s->grad, s->l);
@@ -1288,6 +1412,8 @@ double o_solver_step(o_solver* s, double* norms, int* nzeroed) {
   s->t_grad += t3 - t2;
   double nrm[6];
   o_residual(m, p, s->iter, s->q, s->grad, s->l, s->b, nrm);
+  s->res_nm1 = s->res; /* ucs/solutionSpace.tcc:705-706 */
+  s->res = nrm[0];
   double t4 = now_s();
   s->t_res += t4 - t3;
   if (p->nsgs > 0) {

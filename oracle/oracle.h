@@ -61,6 +61,14 @@ typedef struct {
   double re_tilde;              /* Re / Mach (ReTilde, ucs/compressible.tcc:753-755) */
   double pr, prt;               /* Prandtl and turbulent Prandtl numbers */
   double sutherland_s;          /* 110.4 K / T_ref (ucs/eqnset.h:251-268) */
+  int gradient_type;            /* gradientType: 0 weighted LSQ, 1 Green-Gauss */
+  int global_dt;                /* 1: localTimeStepping off */
+  double fixed_dt;              /* timeStep (> 0: fixed dt with global_dt) */
+  int enable_vnn;               /* enableVNN */
+  double vnn;                   /* VNN */
+  int dynamic_cfl;              /* dynamicCFL */
+  int jac_field_type;           /* jacobianFieldType: 0 one-sided FD, 1 centred FD */
+  int flux_type;                /* 0 Roe, 1 algebraic */
 } o_params;
 
 typedef struct {
@@ -94,11 +102,13 @@ void o_viscous_jacobian(const o_params* p, const double* QL, const double* QR, c
 void o_update_bcs(const o_mesh* m, const o_params* p, double* q);
 void o_lsq_moments(const o_mesh* m, double* sw);
 void o_gradient(const o_mesh* m, const double* q, const double* sw, double* grad);
+void o_gradient_gg(const o_mesh* m, const double* q, double* grad);
 void o_limiter(const o_mesh* m, const o_params* p, const double* q, const double* grad, double* l);
 void o_residual(const o_mesh* m, const o_params* p, int iter, const double* q, const double* grad,
                 const double* l, double* b, double* norms /* [6] */);
 double o_timestep(const o_mesh* m, const o_params* p, const double* q, double cfl, double* dt);
 double o_cfl(const o_params* p, int iter);
+double o_cfl_update(const o_params* p, int iter, double res, double res_nm1, double cfl_prev);
 
 int o_crs_init(const o_mesh* m, o_crs* A);
 void o_crs_free(o_crs* A);
@@ -154,6 +164,7 @@ typedef struct {
   double cfl;
   const int* order; /* SGS row order (NULL = natural) */
   double t_jac, t_grad, t_res, t_lin, t_upd; /* phase timers (s) */
+  double res, res_nm1; /* residGlobal of the last two iterations (dynamic CFL) */
 } o_solver;
 
 int o_solver_init(o_solver* s, const o_mesh* m, const o_params* p, const double* q0);

@@ -79,6 +79,15 @@ class Params:
     prandtl: float = 0.72         # prandtlNumber
     prandtl_t: float = 0.85       # PrT (ucs/param.tcc:89)
     sutherland_s: float = 110.4 / 300.0   # 110.4 K / T_ref (ucs/eqnset.h:251-268)
+    # options off by default in the reference
+    gradient_type: int = 0        # gradientType: 0 weighted LSQ, 1 Green-Gauss
+    global_dt: int = 0            # 1: localTimeStepping off (one dt everywhere)
+    fixed_dt: float = -1.0        # timeStep (used with global_dt when > 0)
+    enable_vnn: int = 0           # enableVNN
+    vnn: float = 20.0             # VNN
+    dynamic_cfl: int = 0          # dynamicCFL
+    jac_field_type: int = 0       # jacobianFieldType: 0 one-sided, 1 centred
+    flux_type: int = 0            # 0 Roe (roeFlux), 1 algebraic (algebraicFlux)
 
     @property
     def re_tilde(self) -> float:  # Re / Mach (ucs/compressible.tcc:753-755)

@@ -32,6 +32,7 @@ struct Dev {
   // geometry
   double* xyz = nullptr;    // 3 x nnode
   double* vol = nullptr;    // nnode
+  double* vol23 = nullptr;  // nnode, vol^(2/3) (the von Neumann time-step limit's volume factor)
   double* sw = nullptr;     // 6 x nnode, weighted LSQ moments
   // gELL gather lists
   int* gwidth = nullptr;    // nchunk_g
@@ -98,6 +99,7 @@ struct Ctx {
   bool have_params = false;
   int iter = 1;
   double cfl = 0.0;
+  double res = 0.0, res_nm1 = 0.0;  // residGlobal of the last two iterations (dynamic CFL)
   bool lu_done = false;
   bool jac_valid = false;
   int nred = 0;  // number of partial-sum blocks

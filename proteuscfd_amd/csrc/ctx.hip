@@ -39,7 +39,7 @@ void dfree(T*& p) {
 
 void free_all(Ctx& c) {
   Dev& d = c.d;
-  dfree(d.xyz); dfree(d.vol); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
+  dfree(d.xyz); dfree(d.vol); dfree(d.vol23); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
   dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
   dfree(d.jdb); dfree(d.jqb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
@@ -343,6 +343,12 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     Dev& d = c.d;
     dput(c, &d.xyz, xyz);
     dput(c, &d.vol, vol);
+    {
+      // host pow, so the limit has the same bits as a host evaluation
+      std::vector<double> v23(vol.size());
+      for (size_t i = 0; i < vol.size(); ++i) v23[i] = std::pow(vol[i], 2.0 / 3.0);
+      dput(c, &d.vol23, v23);
+    }
     dalloc(c, &d.sw, 6 * (size_t)nn);
     dput(c, &d.gwidth, gwidth);
     dput(c, &d.gbase, gbase);
@@ -408,6 +414,14 @@ int pcfd_set_params(pcfd_ctx* ctx, const pcfd_params* p) {
         (p->viscous && (p->gradient_terms != 9 || !(p->re_tilde > 0.0) || !(p->prandtl > 0.0) ||
                         !(p->prandtl_t > 0.0) || !(p->sutherland_s >= 0.0)))) {
       c.err = "pcfd_set_params: parameter out of range (viscous runs need gradient_terms 9, Re/M, Pr, PrT > 0)";
+      return PCFD_ERR_ARG;
+    }
+    if ((p->gradient_type != 0 && p->gradient_type != 1) || (p->global_dt != 0 && p->global_dt != 1) ||
+        (p->enable_vnn != 0 && p->enable_vnn != 1) || (p->enable_vnn && !(p->vnn > 0.0)) ||
+        (p->dynamic_cfl != 0 && p->dynamic_cfl != 1) || (p->jac_field_type != 0 && p->jac_field_type != 1) ||
+        (p->flux_type != PCFD_FLUX_ROE && p->flux_type != PCFD_FLUX_ALGEBRAIC)) {
+      c.err = "pcfd_set_params: option out of range (gradient_type, jac_field_type, global_dt, enable_vnn, "
+              "dynamic_cfl 0/1; flux_type PCFD_FLUX_ROE/PCFD_FLUX_ALGEBRAIC; vnn > 0 when enabled)";
       return PCFD_ERR_ARG;
     }
     c.prm = *p;

@@ -229,7 +229,102 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g
   for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = gr[k];
 }
 
+// Green-Gauss gradient, gradientType 1 (Kernel_/Bkernel_Green_Gauss_Gradient,
+// ucs/gradient.tcc:172-250): each face adds its averaged terms times a*|a|
+// to the left node and subtracts them from the right one, every boundary
+// half-edge adds the average with its phantom state; the sum is divided by
+// the volume and the symmetry projection applied (:68-95).  Gathered per
+// node: the contributions arrive in ascending neighbour order (the order of
+// the reference's edge loop), then in half-edge order.
+template <int NT>
+__global__ void __launch_bounds__(kBlock) k_gradient_gg(int nnode, int ntot, Gell g, const double* __restrict__ vol,
+                                                       const double* __restrict__ q,
+                                                       const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                       const int* __restrict__ he_tag,
+                                                       const double* __restrict__ he_a, int nbnode,
+                                                       const int* __restrict__ bc_type, double gamma,
+                                                       double* __restrict__ grad) {
+  int i = xcd_block() * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const double gm1 = gamma - 1.0;
+  double gr[NT * 3];
+#pragma unroll
+  for (int k = 0; k < NT * 3; ++k) gr[k] = 0.0;
+  double Qi[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) Qi[k] = q[k * ntot + i];
+  double ti[NT];
+  grad_terms<NT>(Qi, gm1, gamma, ti);
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  for (int s = 0; s < w; ++s) {
+    const long long idx = b + (long long)s * kW;
+    const int j = g.col[idx];
+    if (j < 0) break;
+    double Qj[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Qj[k] = q[k * ntot + j];
+    double tj[NT];
+    grad_terms<NT>(Qj, gm1, gamma, tj);
+    const double a[4] = {g.nrm[idx], g.nrm[g.stride + idx], g.nrm[2 * g.stride + idx], g.nrm[3 * g.stride + idx]};
+    const bool left = i < j;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double f = 0.5 * (ti[t] + tj[t]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double v = f * a[k] * a[3];
+        gr[3 * t + k] += left ? v : -v;
+      }
+    }
+  }
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    const int h = he_id[p];
+    double Qp[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Qp[k] = q[k * ntot + nnode + h];
+    double tp[NT];
+    grad_terms<NT>(Qp, gm1, gamma, tp);
+    const double a[4] = {he_a[h], he_a[nbnode + h], he_a[2 * nbnode + h], he_a[3 * nbnode + h]};
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const double f = 0.5 * (ti[t] + tp[t]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gr[3 * t + k] += f * a[k] * a[3];
+    }
+  }
+  const double vi = vol[i];
+#pragma unroll
+  for (int k = 0; k < NT * 3; ++k) gr[k] /= vi;
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    int h = he_id[p];
+    if (bc_type[he_tag[h]] != BC_SYMMETRY) continue;
+    double a0 = he_a[h], a1 = he_a[nbnode + h], a2 = he_a[2 * nbnode + h];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      double dt = gr[3 * t] * a0 + gr[3 * t + 1] * a1 + gr[3 * t + 2] * a2;
+      gr[3 * t] -= dt * a0;
+      gr[3 * t + 1] -= dt * a1;
+      gr[3 * t + 2] -= dt * a2;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = gr[k];
+}
+
 void launch_gradient(Ctx& c, int nterms) {
+  if (c.prm.gradient_type == 1) {
+    if (nterms == 9)
+      hipLaunchKernelGGL(k_gradient_gg<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
+                         c.d.vol, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
+                         c.prm.gamma, c.d.grad);
+    else
+      hipLaunchKernelGGL(k_gradient_gg<5>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
+                         c.d.vol, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
+                         c.prm.gamma, c.d.grad);
+    PCFD_HIP(hipGetLastError());
+    return;
+  }
   if (nterms == 9) {
     hipLaunchKernelGGL(k_gradient<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
                        c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
@@ -395,7 +490,7 @@ void launch_limiter(Ctx& c) {
 // b[L] -= F, b[R] += F) gathered per node, then Bkernel_Inviscid_Flux
 // (:298-387, b[L] -= F on the unextrapolated state vs the phantom).
 // Also writes per-block partial sums of b^2 (total + per equation).
-template <bool HO>
+template <bool HO, bool ALG>
 __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double gamma, double chi, int sym2d, Gell g,
                                                     const double* __restrict__ xyz, const double* __restrict__ q,
                                                     const double* __restrict__ grad, const double* __restrict__ lim,
@@ -453,7 +548,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
         }
       }
       double F[5];
-      roe_flux_fast(QL, QR, a, gamma, sym2d, F);
+      num_flux<ALG>(QL, QR, a, gamma, sym2d, F);
 #pragma unroll
       for (int k = 0; k < 5; ++k) bi[k] += left ? -F[k] : F[k];
     }
@@ -464,7 +559,7 @@ __global__ void __launch_bounds__(kBlock) k_residual(int nnode, int ntot, double
 #pragma unroll
       for (int k = 0; k < 5; ++k) QR[k] = q[k * ntot + nnode + h];
       double F[5];
-      roe_flux_fast(qi, QR, a, gamma, sym2d, F);
+      num_flux<ALG>(qi, QR, a, gamma, sym2d, F);
 #pragma unroll
       for (int k = 0; k < 5; ++k) bi[k] += -F[k];
     }
@@ -496,15 +591,12 @@ void launch_residual(Ctx& c, int ho) {
   const pcfd_params& p = c.prm;
   int nb = nblk(c.nnode);
   sub_begin(c);
-  if (ho) {
-    hipLaunchKernelGGL(k_residual<true>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
-                       p.symmetry2d, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a,
-                       c.nbnode, c.d.b, c.d.red);
-  } else {
-    hipLaunchKernelGGL(k_residual<false>, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi,
-                       p.symmetry2d, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a,
-                       c.nbnode, c.d.b, c.d.red);
-  }
+  const bool alg = p.flux_type == PCFD_FLUX_ALGEBRAIC;
+  auto kern = ho ? (alg ? k_residual<true, true> : k_residual<true, false>)
+                 : (alg ? k_residual<false, true> : k_residual<false, false>);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, p.chi, p.symmetry2d,
+                     gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim, c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.b,
+                     c.d.red);
   PCFD_HIP(hipGetLastError());
   c.nred = nb;
   launch_viscous(c, ho);
@@ -536,11 +628,14 @@ __device__ __forceinline__ double max_eig_r(const double* Q, const double* a, do
   return dmax(fabs(th + c), fabs(th - c));
 }
 
-// ComputeTimesteps, local time stepping (ucs/timestep.tcc:6-143)
+// ComputeTimesteps, local time stepping (ucs/timestep.tcc:6-47).  With
+// vol23 set, the von Neumann limit dt = min(dt, VNN vol^(2/3)) is applied
+// from node 1 on, as the reference's loop does (:37-45).
 __global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double gamma, double cfl, Gell g,
                                                     const double* __restrict__ q, const double* __restrict__ vol,
                                                     const int* __restrict__ he_ptr, const int* __restrict__ he_id,
                                                     const double* __restrict__ he_a, int nbnode,
+                                                    const double* __restrict__ vol23, double vnn,
                                                     double* __restrict__ dt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
@@ -568,7 +663,33 @@ __global__ void __launch_bounds__(kBlock) k_timestep(int nnode, int ntot, double
     for (int k = 0; k < 5; ++k) t[k] = 0.5 * (qi[k] + q[k * ntot + nnode + h]);
     s += max_eig_r(t, a, gamma) * a[3];
   }
-  dt[i] = cfl * (vol[i] / s);
+  double d = cfl * (vol[i] / s);
+  if (vol23 && i > 0) d = dmin(d, vnn * vol23[i]);
+  dt[i] = d;
+}
+
+// global time step: every node gets the same value, timeStep when given
+// (ucs/timestep.tcc:48-55), else the minimum of the local values
+// (:56-74; with the VNN limit that minimum is the same, node 0 excepted
+// from the limit in both).  The minimum is reduced in two fixed stages.
+__global__ void __launch_bounds__(kBlock) k_fill(int n, const double* __restrict__ src, double value,
+                                                double* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = src ? src[0] : value;
+}
+
+__global__ void __launch_bounds__(kBlock) k_min_final(int nparts, const double* __restrict__ red,
+                                                     double* __restrict__ out) {
+  __shared__ double sh[kBlock];
+  double m = red[0];
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) m = dmin(m, red[p]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = dmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 __global__ void __launch_bounds__(kBlock) k_min_partial(int n, const double* __restrict__ v, double* __restrict__ red) {
@@ -585,9 +706,26 @@ __global__ void __launch_bounds__(kBlock) k_min_partial(int n, const double* __r
 }
 
 void launch_timestep(Ctx& c, double cfl) {
-  hipLaunchKernelGGL(k_timestep, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma, cfl,
-                     gell(c), c.d.q, c.d.vol, c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode, c.d.dt);
+  const pcfd_params& p = c.prm;
+  if (p.global_dt && p.fixed_dt > 0.0) {
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, nullptr, p.fixed_dt, c.d.dt);
+    PCFD_HIP(hipGetLastError());
+    return;
+  }
+  hipLaunchKernelGGL(k_timestep, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.gamma, cfl,
+                     gell(c), c.d.q, c.d.vol, c.d.he_ptr, c.d.he_id, c.d.he_a, c.nbnode,
+                     p.enable_vnn ? c.d.vol23 : nullptr, p.vnn, c.d.dt);
   PCFD_HIP(hipGetLastError());
+  if (p.global_dt) {
+    const int nb = std::min(nblk(c.nnode), 1024);
+    hipLaunchKernelGGL(k_min_partial, dim3(nb), dim3(kBlock), 0, c.stream, c.nnode, c.d.dt, c.d.red);
+    PCFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kBlock), 0, c.stream, nb, c.d.red, c.d.red_out + 7);
+    PCFD_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fill, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.d.red_out + 7, 0.0,
+                       c.d.dt);
+    PCFD_HIP(hipGetLastError());
+  }
 }
 
 double launch_dtmin(Ctx& c) {

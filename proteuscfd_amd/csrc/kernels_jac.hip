@@ -38,12 +38,17 @@ struct JacArgs {
   const int* he_nrm;       // per half-edge, wall-normal neighbour (noSlip)
   int viscous;
   ViscParams vp;
+  int flux_alg;            // 1: algebraic flux (generic form only)
+  int centred;             // 1: centred differences, jacobianFieldType 1 (generic form only)
 };
 
 // One-sided differences with h = 1e-8 (ucs/jacobian.tcc:268): the quotient
 // is taken as a product with 1/h = 1e8, which is exact in binary64, so it
 // differs from the division by h at most in the last bit.
 constexpr double kInvH = 1.0e8;
+// centred differences (Kernel_NumJac_Centered, ucs/jacobian.tcc:307-367):
+// the quotient by 2h as a product with 1/(2h) = 5e7, exact in binary64
+constexpr double kInv2H = 5.0e7;
 
 // Five lanes per block row: lane c of a row's group owns perturbation
 // column c -- it writes column c of every block of the row and accumulates
@@ -59,6 +64,7 @@ __device__ __forceinline__ void perturb(const double* Q, int c, double h, double
 // lanes per node as in k_jacobian.  Writes the node's boundary part of the
 // diagonal block to Db[t*25 + r*5 + c] (t = boundary-node index) and the
 // re-evaluated phantom states (lane 0).
+template <bool ALG>
 __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd,
                                                         double* __restrict__ Db, double* __restrict__ Qb) {
   const int lane64 = threadIdx.x & 63;
@@ -105,11 +111,11 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
           for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
         }
         double fS[5], fL[5], QPL[5];
-        roe_flux_fast(QL, QR, a, gamma, A.sym2d, fS);
+        num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
         perturb(QL, c, h, QPL);
         double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
         noslip_state(QPL, QPR, tw, nQ, gamma);
-        roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
+        num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
         continue;
@@ -129,13 +135,13 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
         for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
       }
       double fS[5];
-      roe_flux_fast(QL, QR, a, gamma, A.sym2d, fS);
+      num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
       double QPL[5];
       perturb(QL, c, h, QPL);
       double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
       bc_state(type, QPL, QPR, Qr, pref, a, gamma);
       double fL[5];
-      roe_flux_fast(QPL, QPR, a, gamma, A.sym2d, fL);
+      num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
     }
@@ -150,7 +156,18 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
   for (int rr = 0; rr < 5; ++rr) Db[(long long)t * 25 + rr * 5 + c] = Dc[rr];
 }
 
-template <typename T, bool VISC>
+// the edge flux of the generic form, evaluated from node i's side
+__device__ __forceinline__ void jac_flux(const JacArgs& A, const double* x, const double* y, const double* ai,
+                                         double* F) {
+  if (A.flux_alg) alg_flux(x, y, ai, A.gamma, F);
+  else roe_flux_fast(x, y, ai, A.gamma, A.sym2d, F);
+}
+
+// GENERIC = false: Roe flux, one-sided differences (the reference defaults),
+// with the own node's Roe-side quantities formed once per row.
+// GENERIC = true: the flux (Roe or algebraic) and the difference form
+// (one-sided or centred) chosen at run time, each evaluation from scratch.
+template <typename T, bool VISC, bool GENERIC>
 __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __restrict__ bidx,
                                                     const double* __restrict__ Db, const double* __restrict__ Qb,
                                                     T* __restrict__ Moff, T* __restrict__ Dg) {
@@ -190,11 +207,16 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
   // (the same one-sided differences as the reference's (L,R) evaluation up
   // to the rounding of the flux).  The own node's states, unperturbed and
   // perturbed in component c, are formed once per row.
-  double qiP[5];
+  // (with centred differences: dcol = (G(q_i + h e_c, q_j) - G(q_i - h e_c, q_j)) / 2h,
+  //  offc = (G(q_i, q_j + h e_c) - G(q_i, q_j - h e_c)) / 2h)
+  double qiP[5], qiM[5];
   perturb(qi, c, h, qiP);
+  perturb(qi, c, -h, qiM);
   RoeSide S0, SP;
-  roe_side_state(qi, gamma, S0);
-  roe_side_state(qiP, gamma, SP);
+  if constexpr (!GENERIC) {
+    roe_side_state(qi, gamma, S0);
+    roe_side_state(qiP, gamma, SP);
+  }
   const long long gb = (long long)A.gbase[i >> 6] * kW + (i & 63);
   const int w = A.gwidth[i >> 6];
   T* ob = Moff + (long long)A.sbase[chunk] * 25 * kW + lane + (long long)c * kW;
@@ -210,26 +232,50 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
 #pragma unroll
     for (int k = 0; k < 5; ++k) qj[k] = A.q[k * A.ntot + j];
     perturb(qj, c, h, qjP);
-    roe_side_th(S0, ai);
-    roe_side_th(SP, ai);
-    RoeSide Sj, SjP;
-    roe_side(qj, ai, gamma, Sj);
-    double fS[5], fP[5];
-    roe_flux_s(qi, S0, qj, Sj, ai, gamma, A.sym2d, fS);
     T* blk = ob + (long long)s * 25 * kW;
     // offc: column c of this row's block A[i][j]; dcol: this edge's part of
     // column c of the diagonal block
     double offc[5], dcol[5];
-    __builtin_amdgcn_sched_barrier(0);
-    roe_flux_s(qiP, SP, qj, Sj, ai, gamma, A.sym2d, fP);
+    if constexpr (!GENERIC) {
+      roe_side_th(S0, ai);
+      roe_side_th(SP, ai);
+      RoeSide Sj, SjP;
+      roe_side(qj, ai, gamma, Sj);
+      double fS[5], fP[5];
+      roe_flux_s(qi, S0, qj, Sj, ai, gamma, A.sym2d, fS);
+      __builtin_amdgcn_sched_barrier(0);
+      roe_flux_s(qiP, SP, qj, Sj, ai, gamma, A.sym2d, fP);
 #pragma unroll
-    for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fP[rr] - fS[rr]) * kInvH;
-    __builtin_amdgcn_sched_barrier(0);
-    roe_side(qjP, ai, gamma, SjP);
-    roe_flux_s(qi, S0, qjP, SjP, ai, gamma, A.sym2d, fP);
+      for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fP[rr] - fS[rr]) * kInvH;
+      __builtin_amdgcn_sched_barrier(0);
+      roe_side(qjP, ai, gamma, SjP);
+      roe_flux_s(qi, S0, qjP, SjP, ai, gamma, A.sym2d, fP);
 #pragma unroll
-    for (int rr = 0; rr < 5; ++rr) offc[rr] = (fP[rr] - fS[rr]) * kInvH;
-    __builtin_amdgcn_sched_barrier(0);
+      for (int rr = 0; rr < 5; ++rr) offc[rr] = (fP[rr] - fS[rr]) * kInvH;
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+      double fA[5], fB[5];
+      if (A.centred) {
+        double qjM[5];
+        perturb(qj, c, -h, qjM);
+        jac_flux(A, qiP, qj, ai, fA);
+        jac_flux(A, qiM, qj, ai, fB);
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fA[rr] - fB[rr]) * kInv2H;
+        jac_flux(A, qi, qjP, ai, fA);
+        jac_flux(A, qi, qjM, ai, fB);
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) offc[rr] = (fA[rr] - fB[rr]) * kInv2H;
+      } else {
+        jac_flux(A, qi, qj, ai, fB);
+        jac_flux(A, qiP, qj, ai, fA);
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fA[rr] - fB[rr]) * kInvH;
+        jac_flux(A, qi, qjP, ai, fA);
+#pragma unroll
+        for (int rr = 0; rr < 5; ++rr) offc[rr] = (fA[rr] - fB[rr]) * kInvH;
+      }
+    }
     if (VISC) {
       // Kernel_Viscous_Jac (ucs/jacobian.tcc:734-767): A[R][L] += aL,
       // A[L][R] += aR, added before the column sums; it runs after the
@@ -348,29 +394,28 @@ void launch_jacobian(Ctx& c) {
   JacArgs A{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase, c.d.gcol,
             c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type, c.d.qref, c.d.vol,
             c.d.dt, c.d.q, c.d.srow, c.d.sbase, c.d.xyz, c.d.twall, c.d.he_nrm, p.viscous,
-            ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d}};
+            ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d},
+            p.flux_type == PCFD_FLUX_ALGEBRAIC ? 1 : 0, p.jac_field_type == 1 ? 1 : 0};
   sub_begin(c);
   if (c.nbnd > 0) {
     const long long bw = (c.nbnd + 11) / 12;
-    hipLaunchKernelGGL(k_jac_boundary, dim3(nblk(bw * kW)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd,
-                       c.d.jdb, c.d.jqb);
+    hipLaunchKernelGGL(A.flux_alg ? k_jac_boundary<true> : k_jac_boundary<false>, dim3(nblk(bw * kW)),
+                       dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd, c.d.jdb, c.d.jqb);
     PCFD_HIP(hipGetLastError());
   }
   const long long waves = (c.nrow_pad + 11) / 12;  // 12 rows x 5 lanes per wavefront
   const int grid = nblk(waves * kW);
   const bool f32 = p.precision == PCFD_CHOSEN_FP32;
-  if (f32 && p.viscous)
-    hipLaunchKernelGGL((k_jacobian<float, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.jqb, c.d.Mf, c.d.Df);
-  else if (f32)
-    hipLaunchKernelGGL((k_jacobian<float, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.jqb, c.d.Mf, c.d.Df);
-  else if (p.viscous)
-    hipLaunchKernelGGL((k_jacobian<double, true>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.jqb, c.d.Md, c.d.Dd);
-  else
-    hipLaunchKernelGGL((k_jacobian<double, false>), dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb,
-                       c.d.jqb, c.d.Md, c.d.Dd);
+  const bool gen = A.flux_alg || A.centred;
+  if (f32) {
+    auto kern = p.viscous ? (gen ? k_jacobian<float, true, true> : k_jacobian<float, true, false>)
+                          : (gen ? k_jacobian<float, false, true> : k_jacobian<float, false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Mf, c.d.Df);
+  } else {
+    auto kern = p.viscous ? (gen ? k_jacobian<double, true, true> : k_jacobian<double, true, false>)
+                          : (gen ? k_jacobian<double, false, true> : k_jacobian<double, false, false>);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Md, c.d.Dd);
+  }
   PCFD_HIP(hipGetLastError());
   if (c.nbnd > 0) {
     hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.ntot,

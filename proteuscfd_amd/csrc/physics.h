@@ -268,6 +268,28 @@ PCFD_DEV void roe_flux_fast(const double* QL, const double* QR, const double* a,
   roe_flux_s(QL, L, QR, R, a, gamma, sym2d, F);
 }
 
+// EqnSet::NumericalFlux with fluxType algebraicFlux (ucs/eqnset.tcc:56-63,
+// 74-88): the average of the two physical fluxes times |a|, NaN components
+// zeroed (no symmetry2d zeroing on this branch).
+PCFD_DEV void alg_flux(const double* QL, const double* QR, const double* a, double gamma, double* F) {
+  double f1[5], f2[5];
+  phys_flux(QL, a, gamma - 1.0, f1);
+  phys_flux(QR, a, gamma - 1.0, f2);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    F[i] = 0.5 * a[3] * (f1[i] + f2[i]);
+    if (__builtin_isnan(F[i])) F[i] = 0.0;
+  }
+}
+
+// EqnSet::NumericalFlux: the Roe flux (roeFlux, the reference default) or the
+// algebraic flux
+template <bool ALG>
+PCFD_DEV void num_flux(const double* QL, const double* QR, const double* a, double gamma, int sym2d, double* F) {
+  if constexpr (ALG) alg_flux(QL, QR, a, gamma, F);
+  else roe_flux_fast(QL, QR, a, gamma, sym2d, F);
+}
+
 // MaxEigenvalue on the edge-averaged state (ucs/compressible.tcc:798-821)
 PCFD_DEV double max_eig(const double* Q, const double* a, double gamma) {
   const double gm1 = gamma - 1.0;

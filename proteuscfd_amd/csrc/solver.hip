@@ -30,6 +30,22 @@ double cfl_at(const pcfd_params& p, int iter) {
   return p.cfl;
 }
 
+// Param::UpdateCFL (ucs/param.tcc:695-726): with dynamicCFL the CFL moves by
+// 10% per iteration in the direction of the residual trend that
+// PreTimeAdvance computes from the last two residuals
+// (ucs/solutionSpace.tcc:629-633), capped at cfl; cflStart on iteration 1.
+double cfl_update(const pcfd_params& p, int iter, double res, double res_nm1, double cfl_prev) {
+  if (!p.dynamic_cfl) return cfl_at(p, iter);
+  double delta = 9999.0;
+  if (res_nm1 > 1.0e-21) delta = (res_nm1 - res) / res_nm1;
+  if (iter > 1) {
+    const double sgn = delta > 0.0 ? 1.0 : -1.0;
+    const double A = 0.0 > cfl_prev + 0.1 * sgn * cfl_prev ? 0.0 : cfl_prev + 0.1 * sgn * cfl_prev;
+    return A < p.cfl ? A : p.cfl;
+  }
+  return p.cfl_start;
+}
+
 void iterate(Ctx& c, double* resid) {
   const pcfd_params& p = c.prm;
   // PreTimeAdvance
@@ -41,7 +57,7 @@ void iterate(Ctx& c, double* resid) {
     c.jac_valid = true;
     c.lu_done = false;
   }
-  c.cfl = cfl_at(p, c.iter);
+  c.cfl = cfl_update(p, c.iter, c.res, c.res_nm1, c.cfl);
   phase_begin(c);
   launch_timestep(c, c.cfl);
   phase_end(c, 4);
@@ -65,6 +81,8 @@ void iterate(Ctx& c, double* resid) {
   launch_residual(c, ho ? 1 : 0);
   phase_end(c, 3);
   launch_norms(c, resid);
+  c.res_nm1 = c.res;  // ucs/solutionSpace.tcc:705-706
+  c.res = resid[0];
   if (p.nsgs > 0) {
     if (!c.lu_done) {
       phase_begin(c);
@@ -104,7 +122,8 @@ int pcfd_reset_iteration(pcfd_ctx* ctx) {
     // Limiter ctor fills 1.0 (ucs/limiters.tcc:39-43); PreIterate computes dt
     std::vector<double> ones(5 * (size_t)c.nnode, 1.0);
     PCFD_HIP(hipMemcpy(c.d.lim, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice));
-    c.cfl = cfl_at(c.prm, c.iter);
+    c.res = c.res_nm1 = 0.0;
+    c.cfl = cfl_update(c.prm, c.iter, 0.0, 0.0, c.prm.cfl_start);
     launch_timestep(c, c.cfl);
     PCFD_HIP(hipStreamSynchronize(c.stream));
   } catch (const std::exception& e) {

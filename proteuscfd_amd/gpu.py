@@ -36,7 +36,10 @@ class CParams(C.Structure):
                 ("limiter_freeze", C.c_int), ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_ramp", C.c_int),
                 ("nsgs", C.c_int), ("symmetry2d", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int),
                 ("gradient_terms", C.c_int), ("viscous", C.c_int), ("re_tilde", C.c_double),
-                ("prandtl", C.c_double), ("prandtl_t", C.c_double), ("sutherland_s", C.c_double)]
+                ("prandtl", C.c_double), ("prandtl_t", C.c_double), ("sutherland_s", C.c_double),
+                ("gradient_type", C.c_int), ("global_dt", C.c_int), ("fixed_dt", C.c_double),
+                ("enable_vnn", C.c_int), ("vnn", C.c_double), ("dynamic_cfl", C.c_int),
+                ("jac_field_type", C.c_int), ("flux_type", C.c_int)]
 
 
 SYMBOLS = [
@@ -117,7 +120,8 @@ def to_cparams(p: Params, gradient_terms: int = 9) -> CParams:
     return CParams(p.gamma, p.sorder, p.nfirst, p.chi, p.limiter, p.venkat_K, p.limiter_refresh, p.limiter_freeze,
                    p.cfl, p.cfl_start, p.cfl_ramp, p.nsgs, p.symmetry2d, p.jac_freq, p.precision,
                    9 if p.viscous else gradient_terms, int(p.viscous), p.re_tilde, p.prandtl, p.prandtl_t,
-                   p.sutherland_s)
+                   p.sutherland_s, p.gradient_type, p.global_dt, p.fixed_dt, p.enable_vnn, p.vnn, p.dynamic_cfl,
+                   p.jac_field_type, p.flux_type)
 
 
 class Context:

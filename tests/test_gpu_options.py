@@ -1,0 +1,152 @@
+"""Parity of the HIP path against the oracle for the reference's non-default
+options: Green-Gauss gradient, VNN / global / fixed time steps, dynamic
+CFL, the centred finite-difference Jacobian and the algebraic flux.
+
+Bars as for the default path: FP64 phases 1e-12 relative (1e-13 for the
+time step), the finite-difference Jacobian 1e-6 (flux rounding is
+amplified by 1/h), a few iterations 1e-8 on the residual history.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import cases
+from proteuscfd_amd.cases import Params
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+CASES = {
+    "wedge": cases.wedge_case(params=Params(sorder=2, limiter=2, cfl=0.9, cfl_start=0.9, nsgs=0)),
+    "bump": cases.bump_case(n=12, params=Params(sorder=2, limiter=3, cfl=20.0, cfl_start=1.0, cfl_ramp=10,
+                                                nsgs=4)),
+}
+
+
+def with_(p, **kw):
+    return Params(**{**p.__dict__, **kw})
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def setup(request):
+    from proteuscfd_amd import gpu
+    case = CASES[request.param]
+    ctx = gpu.Context(case)
+    q0 = cases.perturbed_state(case)
+    print(f"[options] case={request.param} nnode={case.mesh.nnode}", flush=True)
+    yield request.param, case, ctx, q0
+    ctx.close()
+
+
+def problem(case, p):
+    return O.Problem(case.mesh, case.bc_type, case.qref, p, clip_jacobi=True, twall=case.twall)
+
+
+def test_green_gauss_gradient(setup):
+    name, case, ctx, q0 = setup
+    p = with_(case.params, gradient_type=1)
+    pb = problem(case, p)
+    qo = pb.update_bcs(q0)
+    go = pb.gradient(qo)
+    ctx.set_params(p)
+    ctx.upload_q(qo)
+    gg = ctx.gradient()
+    ctx.set_params(case.params)
+    for t in range(9):
+        assert rel(gg[:, 3 * t:3 * t + 3], go[:, 3 * t:3 * t + 3]) < 1e-12, f"term {t}"
+
+
+def test_timestep_modes(setup):
+    name, case, ctx, q0 = setup
+    pb = problem(case, case.params)
+    qo = pb.update_bcs(q0)
+    dloc, _ = pb.timestep(qo, 3.0)
+    # a VNN number that binds on about half of the nodes
+    vnn = float(np.median(dloc / case.mesh.vol ** (2.0 / 3.0)))
+    modes = [dict(enable_vnn=1, vnn=vnn), dict(global_dt=1), dict(global_dt=1, enable_vnn=1, vnn=vnn),
+             dict(global_dt=1, fixed_dt=0.003)]
+    for kw in modes:
+        p = with_(case.params, **kw)
+        dto, mino = problem(case, p).timestep(qo, 3.0)
+        ctx.set_params(p)
+        ctx.upload_q(qo)
+        dtg, ming = ctx.timestep(3.0)
+        assert rel(dtg, dto) < 1e-13, kw
+        assert abs(ming - mino) <= 1e-13 * mino, kw
+    ctx.set_params(case.params)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_algebraic_residual(setup, order):
+    name, case, ctx, q0 = setup
+    p = with_(case.params, sorder=order, flux_type=1)
+    pb = problem(case, p)
+    qo = pb.update_bcs(q0)
+    go = pb.gradient(qo)
+    lo = pb.limiter(qo, go) if order == 2 else np.ones((case.mesh.nnode, 5))
+    bo, no = pb.residual(qo, go, lo, it=1)
+    ctx.set_params(p)
+    ctx.upload_q(qo)
+    ctx.gradient()
+    ctx.set_limiter(lo)
+    bg, ng = ctx.residual(it=1)
+    ctx.set_params(case.params)
+    assert rel(bg, bo) < 1e-12
+    np.testing.assert_allclose(ng, no, rtol=1e-12)
+
+
+@pytest.mark.parametrize("flux,field", [(0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_option_jacobian(setup, flux, field, precision):
+    name, case, ctx, q0 = setup
+    p = with_(case.params, flux_type=flux, jac_field_type=field, nsgs=4, precision=precision)
+    pb = problem(case, p)
+    qo = pb.update_bcs(q0)
+    dto, _ = pb.timestep(qo, 5.0)
+    ia, ja, iau, Mo, qafter = pb.jacobian(qo, dto)
+    ctx.set_params(p)
+    ctx.upload_q(qo)
+    ctx.set_timestep(dto)
+    ctx.jacobian()
+    Mg = ctx.download_jacobian(ia, ja)
+    ctx.set_params(case.params)
+    tol = 1e-6 if precision == 0 else 1e-6 + 2.0 ** -23
+    assert rel(Mg, Mo) < tol
+    assert rel(Mg[iau], Mo[iau]) < tol
+    qg = ctx.download_q()
+    assert rel(qg[:, :5], qafter[:, :5]) < 1e-13
+
+
+RUNS = {
+    # implicit: Green-Gauss, VNN, dynamic CFL, centred Jacobian
+    "bump": dict(gradient_type=1, enable_vnn=1, vnn=4.0, dynamic_cfl=1, jac_field_type=1),
+    # explicit: algebraic flux, one global dt, dynamic CFL
+    "wedge": dict(flux_type=1, global_dt=1, dynamic_cfl=1, sorder=1),
+}
+
+
+def test_option_iterations(setup):
+    name, case, ctx, q0 = setup
+    p = with_(case.params, **RUNS[name])
+    pb = problem(case, p)
+    ctx.set_params(p)
+    q = case.initial_state()
+    ctx.upload_q(q)
+    ctx.reset_iteration()
+    order = ctx.sgs_order() if p.nsgs > 0 else None
+    so = O.Solver(pb, q, order=order)
+    for _ in range(8):
+        rg = ctx.iterate()
+        ro, _, _ = so.step()
+        assert abs(rg[0] - ro) <= 1e-8 * ro
+    qg = ctx.download_q()
+    assert rel(qg[:, :5], so.q[:, :5]) < 1e-9
+    so.close()
+    ctx.set_params(case.params)

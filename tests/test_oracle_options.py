@@ -1,0 +1,290 @@
+"""Pins for the oracle's non-default options: Green-Gauss gradient
+(gradientType 1), the time-step modes (VNN limit, one global dt, a fixed
+dt), dynamic CFL, the centred finite-difference Jacobian (jacobianFieldType
+1) and the algebraic flux (fluxType algebraicFlux).
+
+The checks do not restate the oracle's formulas: Green-Gauss exactness on
+linear fields and closure on constant ones, the symmetry projection, closed
+forms of the time-step limits, complex-step derivatives of a textbook Euler
+flux for the Jacobian blocks, freestream preservation and discrete
+conservation with a textbook flux, and a matrix-free derivative of the
+residual.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import cases
+from proteuscfd_amd import mesh as M
+from proteuscfd_amd.cases import Params
+
+G = 1.4
+
+
+def euler_flux(Q, n, g=G):
+    """Textbook inviscid flux F(Q).n (works on complex states)."""
+    rho, u = Q[0], Q[1:4] / Q[0]
+    p = (g - 1) * (Q[4] - 0.5 * rho * (u @ u))
+    un = u @ n
+    return np.array([rho * un, *(rho * un * u + p * n), (Q[4] + p) * un])
+
+
+def euler_jacobian_cs(Q, n, g=G):
+    """dF.n/dQ by complex-step differentiation (exact to rounding)."""
+    J = np.zeros((5, 5))
+    hc = 1e-30
+    for k in range(5):
+        Qc = Q[:5].astype(complex)
+        Qc[k] += 1j * hc
+        J[:, k] = euler_flux(Qc, n.astype(complex), g).imag / hc
+    return J
+
+
+def box_case(bc, mach=0.5, n=4, direction=(1, 0.3, -0.2), params=None):
+    m = M.Mesh.build(M.BOX, nx=n, ny=n + 1, nz=n, jitter=0.1, seed=9)
+    q0 = cases.qinf(G, mach, direction=direction)
+    bct = np.array([0] + [bc] * 6, dtype=np.int32)
+    return cases.Case("box", m, bct, np.tile(q0, (7, 1)), q0, params or Params(sorder=1, nsgs=2))
+
+
+def interior_nodes(m):
+    onb = np.zeros(m.nnode, dtype=bool)
+    onb[m.bedges[:, 0]] = True
+    return np.flatnonzero(~onb)
+
+
+def _block(ia, ja, M_, r, c):
+    k = ia[r] + int(np.flatnonzero(ja[ia[r]:ia[r + 1]] == c)[0])
+    return M_[k]
+
+
+# --- Green-Gauss gradient -------------------------------------------------
+
+def test_green_gauss_exact_for_linear_fields_at_interior_nodes():
+    """The median-dual Green-Gauss sum with edge-midpoint averages is exact
+    for linear fields wherever the dual cell is closed by interior faces."""
+    case = cases.bump_case(n=6, params=Params(sorder=2, gradient_type=1))
+    pb = O.Problem.from_case(case)
+    m = case.mesh
+    rng = np.random.default_rng(4)
+    A = rng.normal(size=(5, 3)) * 0.1
+    c0 = np.array([1.0, 0.5, 0.2, -0.1, 3.0])
+    q = case.initial_state()
+    q[: m.nnode, :5] = c0 + m.xyz @ A.T
+    q = cases.aux(q, G)
+    g = pb.gradient(q)
+    inner = interior_nodes(m)
+    assert len(inner) > 50
+    for t in range(5):
+        np.testing.assert_allclose(g[inner, 3 * t:3 * t + 3], np.tile(A[t], (len(inner), 1)), atol=1e-12)
+    # the weighted LSQ gradient is exact there too
+    pbl = O.Problem(m, case.bc_type, case.qref, Params(sorder=2, gradient_type=0))
+    gl = pbl.gradient(q)
+    np.testing.assert_allclose(g[inner, :15], gl[inner, :15], atol=1e-12)
+
+
+def test_green_gauss_zero_for_constant_fields_everywhere():
+    """A constant field (phantoms equal to it) has zero gradient at every
+    node, boundary nodes included: each dual cell's face vectors sum to 0."""
+    case = box_case(cases.BC_FARFIELD, params=Params(sorder=2, gradient_type=1))
+    pb = O.Problem.from_case(case)
+    q = pb.update_bcs(case.initial_state())
+    g = pb.gradient(q)
+    scale = np.abs(case.qinf[:5]).max()
+    assert np.abs(g).max() < 1e-12 * scale
+
+
+def test_green_gauss_symmetry_projection():
+    """Nodes on the z-symmetry planes of the wedge get no z derivative."""
+    case = cases.wedge_case(nx=14, ny=6, nz=3, params=Params(sorder=2, gradient_type=1))
+    pb = O.Problem.from_case(case)
+    m = case.mesh
+    q = pb.update_bcs(cases.perturbed_state(case, amp=0.1))
+    g = pb.gradient(q)
+    sym = np.unique(m.bedges[case.bc_type[m.bfactag] == cases.BC_SYMMETRY, 0])
+    assert len(sym) > 20
+    assert np.abs(g[sym][:, 2::3]).max() < 1e-14 * np.abs(g).max()
+    assert np.abs(g[sym][:, 0::3]).max() > 1e-3 * np.abs(g).max()
+
+
+# --- time-step modes --------------------------------------------------------
+
+def test_timestep_modes_closed_forms():
+    case = cases.bump_case(n=5)
+    m = case.mesh
+    q0 = cases.perturbed_state(case, amp=0.1)
+    base = dict(case.params.__dict__)
+    pb = O.Problem(m, case.bc_type, case.qref, Params(**base))
+    q = pb.update_bcs(q0)
+    dloc, mloc = pb.timestep(q, 4.0)
+    assert mloc == dloc.min()
+    lim = m.vol ** (2.0 / 3.0)
+    # VNN limit, applied from node 1 on (node 0 keeps its CFL value)
+    for vnn in (1e-6, 1e6):
+        pv = O.Problem(m, case.bc_type, case.qref, Params(**{**base, "enable_vnn": 1, "vnn": vnn}))
+        dv, mv = pv.timestep(q, 4.0)
+        assert dv[0] == dloc[0]
+        np.testing.assert_allclose(dv[1:], np.minimum(dloc[1:], vnn * lim[1:]), rtol=1e-15)
+        assert mv == dv.min()
+    assert np.all(dv == dloc)  # a large VNN never binds
+    # one global dt: the minimum of the local values everywhere
+    pg = O.Problem(m, case.bc_type, case.qref, Params(**{**base, "global_dt": 1}))
+    dg, mg = pg.timestep(q, 4.0)
+    assert np.all(dg == dloc.min()) and mg == dloc.min()
+    # global with VNN: min of node 0's CFL value and every other node's limited value
+    pgv = O.Problem(m, case.bc_type, case.qref, Params(**{**base, "global_dt": 1, "enable_vnn": 1, "vnn": 1e-6}))
+    dgv, _ = pgv.timestep(q, 4.0)
+    expect = min(dloc[0], np.minimum(dloc[1:], 1e-6 * lim[1:]).min())
+    np.testing.assert_allclose(dgv, expect, rtol=1e-15)
+    # a fixed dt
+    pf = O.Problem(m, case.bc_type, case.qref, Params(**{**base, "global_dt": 1, "fixed_dt": 0.0125}))
+    df, mf = pf.timestep(q, 4.0)
+    assert np.all(df == 0.0125) and mf == 0.0125
+
+
+# --- dynamic CFL -----------------------------------------------------------
+
+def test_dynamic_cfl_rule():
+    p = Params(cfl=10.0, cfl_start=1.0, dynamic_cfl=1)
+    pb = O.Problem.from_case(box_case(cases.BC_FARFIELD, params=p))
+    assert pb.cfl_update(1, 0.0, 0.0, 123.0) == 1.0                  # iteration 1: cflStart
+    assert pb.cfl_update(2, 0.5, 0.0, 1.0) == pytest.approx(1.1)      # no history: grow
+    assert pb.cfl_update(5, 0.5, 1.0, 2.0) == pytest.approx(2.2)      # falling residual: +10%
+    assert pb.cfl_update(5, 1.5, 1.0, 2.0) == pytest.approx(1.8)      # rising residual: -10%
+    assert pb.cfl_update(5, 1.0, 1.0, 2.0) == pytest.approx(1.8)      # flat counts as rising
+    assert pb.cfl_update(5, 0.5, 1.0, 9.5) == 10.0                   # capped at cfl
+    # ramping when off
+    pr = O.Problem.from_case(box_case(cases.BC_FARFIELD, params=Params(cfl=10.0, cfl_start=1.0, cfl_ramp=9)))
+    assert pr.cfl_update(3, 0.5, 1.0, 7.0) == pytest.approx(1.0 + 9.0 / 9 * 3)
+
+
+def test_dynamic_cfl_follows_residual_history():
+    """Iteration n uses the residuals of iterations n-1 and n-2.  The
+    explicit wedge run overshoots its stability limit, so the residual
+    rises and falls and the CFL is capped, cut and raised again."""
+    p = Params(sorder=1, cfl=3.0, cfl_start=1.5, dynamic_cfl=1, nsgs=0)
+    case = cases.wedge_case(nx=14, ny=6, nz=2, params=p)
+    pb = O.Problem.from_case(case)
+    so = O.Solver(pb, case.initial_state())
+    hist, cfls = [], []
+    nit = 20
+    for _ in range(nit):
+        _, norms, _ = so.step()
+        hist.append(norms[0])
+        cfls.append(so.s.cfl)
+    so.close()
+    expect = [1.5]
+    for n in range(2, nit + 1):
+        r1 = hist[n - 2]
+        r2 = hist[n - 3] if n >= 3 else 0.0
+        delta = (r2 - r1) / r2 if r2 > 1e-21 else 9999.0
+        prev = expect[-1]
+        expect.append(min(max(0.0, prev * (1.1 if delta > 0 else 0.9)), 3.0))
+    np.testing.assert_allclose(cfls, expect, rtol=1e-14)
+    steps = np.sign(np.diff(cfls))
+    assert (steps > 0).any() and (steps < 0).any() and cfls.count(3.0) >= 1
+
+
+# --- algebraic flux and the centred Jacobian ----------------------------------
+
+def test_algebraic_flux_freestream_and_conservation():
+    p = Params(sorder=1, nsgs=2, flux_type=1)
+    case = box_case(cases.BC_FARFIELD, params=p)
+    pb = O.Problem.from_case(case)
+    m = case.mesh
+    q = pb.update_bcs(case.initial_state())
+    b, _ = pb.residual(q)
+    scale = np.abs(euler_flux(case.qinf, np.array([1.0, 0, 0]))).max() * m.edge_a[:, 3].max()
+    assert np.abs(b).max() < 1e-13 * scale
+    # interior fluxes cancel: sum b = -sum of boundary fluxes 0.5|a|(F(q_i) + F(q_phantom))
+    qp = pb.update_bcs(cases.perturbed_state(case, amp=0.1))
+    b, _ = pb.residual(qp)
+    fb = np.zeros(5)
+    for h in range(m.nbedge):
+        L, R = m.bedges[h]
+        a = m.bedge_a[h]
+        fb += 0.5 * a[3] * (euler_flux(qp[L], a[:3]) + euler_flux(qp[R], a[:3]))
+    np.testing.assert_allclose(b.sum(axis=0), -fb, atol=1e-13 * np.abs(fb).max())
+    # first-order algebraic residual against the textbook edge sum at every node
+    bt = np.zeros((m.nnode, 5))
+    for e in range(m.nedge):
+        L, R = m.edges[e]
+        a = m.edge_a[e]
+        F = 0.5 * a[3] * (euler_flux(qp[L], a[:3]) + euler_flux(qp[R], a[:3]))
+        bt[L] -= F
+        bt[R] += F
+    for h in range(m.nbedge):
+        L, R = m.bedges[h]
+        a = m.bedge_a[h]
+        bt[L] -= 0.5 * a[3] * (euler_flux(qp[L], a[:3]) + euler_flux(qp[R], a[:3]))
+    np.testing.assert_allclose(b, bt, atol=1e-13 * np.abs(bt).max())
+
+
+@pytest.mark.parametrize("field", [0, 1])
+def test_algebraic_jacobian_blocks_match_exact_flux_jacobian(field):
+    """With the algebraic flux, A[L][R] = 0.5|a| dF/dQ(Q_R) and
+    A[R][L] = -0.5|a| dF/dQ(Q_L) exactly; the one-sided and the centred
+    finite differences reproduce them to their truncation/rounding error."""
+    p = Params(sorder=1, nsgs=2, flux_type=1, jac_field_type=field)
+    case = cases.bump_case(n=4, params=p)
+    pb = O.Problem.from_case(case)
+    m = case.mesh
+    q = pb.update_bcs(cases.perturbed_state(case, amp=0.1))
+    dt, _ = pb.timestep(q, 1.0)
+    ia, ja, iau, M_, _ = pb.jacobian(q, dt)
+    rng = np.random.default_rng(3)
+    for e in rng.choice(m.nedge, 40, replace=False):
+        L, R = m.edges[e]
+        a = m.edge_a[e]
+        ALR = 0.5 * a[3] * euler_jacobian_cs(q[R], a[:3])
+        ARL = -0.5 * a[3] * euler_jacobian_cs(q[L], a[:3])
+        tol = 2e-6 * max(np.abs(ALR).max(), np.abs(ARL).max())
+        np.testing.assert_allclose(_block(ia, ja, M_, L, R), ALR, atol=tol)
+        np.testing.assert_allclose(_block(ia, ja, M_, R, L), ARL, atol=tol)
+
+
+def test_centred_roe_jacobian_agrees_with_one_sided():
+    case = cases.bump_case(n=4, params=Params(sorder=1, nsgs=2))
+    m = case.mesh
+    q0 = cases.perturbed_state(case, amp=0.1)
+    out = []
+    for field in (0, 1):
+        pb = O.Problem(m, case.bc_type, case.qref, Params(sorder=1, nsgs=2, jac_field_type=field))
+        q = pb.update_bcs(q0)
+        dt, _ = pb.timestep(q, 1.0)
+        out.append(pb.jacobian(q, dt)[3])
+    d = np.abs(out[0] - out[1]).max()
+    assert 0 < d < 1e-5 * np.abs(out[0]).max()
+
+
+def _dense_from_crs(ia, ja, M_):
+    n = len(ia) - 1
+    A = np.zeros((5 * n, 5 * n))
+    for i in range(n):
+        for k in range(ia[i], ia[i + 1]):
+            A[5 * i:5 * i + 5, 5 * ja[k]:5 * ja[k] + 5] = M_[k]
+    return A
+
+
+@pytest.mark.parametrize("flux,field", [(0, 1), (1, 0), (1, 1)])
+def test_option_jacobian_matches_matrix_free_residual_derivative(flux, field):
+    """(A - V/dt) v == -(b(q + eps v) - b(q))/eps, boundary states re-evaluated."""
+    p = Params(sorder=1, nsgs=2, flux_type=flux, jac_field_type=field)
+    case = cases.bump_case(n=4, params=p)
+    pb = O.Problem.from_case(case)
+    m = case.mesh
+    q = pb.update_bcs(cases.perturbed_state(case, amp=0.1))
+    dt, _ = pb.timestep(q, 1.0)
+    ia, ja, iau, M_, qj = pb.jacobian(q, dt)
+    A = _dense_from_crs(ia, ja, M_)
+    A -= np.kron(np.diag(m.vol / dt), np.eye(5))
+    rng = np.random.default_rng(6)
+    v = rng.normal(size=(m.nnode, 5)) * np.abs(qj[: m.nnode, :5])
+    eps = 1e-6
+    b0, _ = pb.residual(pb.update_bcs(qj))
+    qp = qj.copy()
+    qp[: m.nnode, :5] += eps * v
+    qp = cases.aux(qp, G)
+    bp, _ = pb.residual(pb.update_bcs(qp))
+    jv = -(bp - b0) / eps
+    np.testing.assert_allclose(A @ v.reshape(-1), jv.reshape(-1), rtol=2e-4, atol=2e-4 * np.abs(jv).max())
