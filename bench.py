@@ -347,7 +347,7 @@ def main():
                      "unit": "GB/s", "frac": dom["frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": sgs_launch_bytes, "avg_launch_us": sgs_launch_us,
                      "launches_per_sgs_call": npass},
-        "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true>", "achieved": kern["residual_2nd_fp64"]["gbs"],
+        "roofline_residual": {"bound": "hbm", "kernel": "k_residual<true, false>", "achieved": kern["residual_2nd_fp64"]["gbs"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kern["residual_2nd_fp64"]["frac"]},
         "cpu_baseline": cpu,
     }

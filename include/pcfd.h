@@ -94,9 +94,10 @@ typedef struct {
   double vnn;            /* VNN (reference default 20) */
   int dynamic_cfl;       /* dynamicCFL: CFL changes by +-10% per iteration with the residual trend
                             (ucs/param.tcc:697-713, ucs/solutionSpace.tcc:629-633) */
-  int jac_field_type;    /* jacobianFieldType: 0 one-sided FD (h = 1e-8), 1 centred FD (ucs/jacobian.tcc:307-367) */
+  int jac_field_type;    /* jacobianFieldType: 0 one-sided FD (h = 1e-8), 1 centred FD (ucs/jacobian.tcc:307-367);
+                            2 (complex step) gives PCFD_ERR_UNSUPPORTED */
   int flux_type;         /* PCFD_FLUX_ROE (roeFlux, the default) or PCFD_FLUX_ALGEBRAIC (algebraicFlux,
-                            ucs/eqnset.tcc:56-77) */
+                            ucs/eqnset.tcc:56-77); 2 (HLLC) gives PCFD_ERR_UNSUPPORTED */
 } pcfd_params;
 
 #define PCFD_FLUX_ROE 0

@@ -150,3 +150,13 @@ def test_option_iterations(setup):
     assert rel(qg[:, :5], so.q[:, :5]) < 1e-9
     so.close()
     ctx.set_params(case.params)
+
+
+def test_unbuilt_and_out_of_range_options_rejected(setup):
+    name, case, ctx, q0 = setup
+    for kw, code in [(dict(flux_type=2), "(-4)"), (dict(jac_field_type=2), "(-4)"), (dict(gradient_type=2), "(-1)"),
+                     (dict(enable_vnn=1, vnn=0.0), "(-1)"), (dict(dynamic_cfl=3), "(-1)")]:
+        with pytest.raises(RuntimeError) as e:
+            ctx.set_params(with_(case.params, **kw))
+        assert code in str(e.value), (kw, str(e.value))
+    ctx.set_params(case.params)
