@@ -14,10 +14,10 @@
  *   Mesh<Type>::ReorderMeshCuthillMcKee ucs/mesh.tcc:2411-2535 (RCM numbering)
  *   Edges<Type>/HalfEdges<Type>       ucs/uns_base.h:12-37     (array layouts)
  *
- * The reference reads meshes from disk; this library instead generates the
- * synthetic benchmark meshes (hex grids split into 6 Kuhn tetrahedra, or
- * prism layers under a Kuhn-tet core) from a seed with a counter-based RNG,
- * so the CPU oracle and the GPU path see bit-identical inputs.
+ * It generates the synthetic benchmark meshes (hex grids split into 6 Kuhn
+ * tetrahedra, or prism layers under a Kuhn-tet core) from a seed with a
+ * counter-based RNG, so the CPU oracle and the GPU path see bit-identical
+ * inputs, and reads the reference's ASCII mesh files (pcfd_mesh_read).
  *
  * All arrays are owned by the pcfd_mesh object and stay valid until
  * pcfd_mesh_free().  Return codes: 0 ok, <0 error (pcfd_mesh_error()).
@@ -105,6 +105,20 @@ int  pcfd_mesh_build(const pcfd_mesh_spec* spec, pcfd_mesh** out);
 int  pcfd_mesh_build_from(int nnode, const double* xyz, int nelem, const int* elem_type, const int* elem_ptr,
                           const int* elem_nodes, int nbface, const int* bface_type, const int* bface_ptr,
                           const int* bface_nodes, const int* bface_tag, int rcm, pcfd_mesh** out);
+/* Mesh file readers (Mesh<Type>::Read*, ucs/mesh.tcc): read the file,
+ * drop nodes no volume element uses, and build as pcfd_mesh_build_from.
+ * PCFD_MESH_FMT_AUTO picks the format from the extension: .msh (GMSH 2.2 /
+ * 4.0 / 4.1 ASCII, ReadGMSH_Master ucs/mesh.tcc:3553-3597), .ugrid (UGRID
+ * ASCII, ReadUGRID_Ascii :6740-6932), .crunch/.fv/.uns (FieldView ASCII,
+ * ReadCRUNCH_Ascii :6933-7300), .su2 (SU2 ASCII, ReadSU2_Ascii :3197-3548).
+ * Face tags become bface_tag / bfactag; nfactag = max(PCFD_NFACTAG, largest
+ * tag + 1). */
+#define PCFD_MESH_FMT_AUTO      0
+#define PCFD_MESH_FMT_GMSH      1
+#define PCFD_MESH_FMT_UGRID     2
+#define PCFD_MESH_FMT_FIELDVIEW 3
+#define PCFD_MESH_FMT_SU2       4
+int  pcfd_mesh_read(const char* path, int format, int rcm, pcfd_mesh** out);
 const pcfd_mesh_data* pcfd_mesh_get(const pcfd_mesh* m);
 void pcfd_mesh_free(pcfd_mesh* m);
 const char* pcfd_mesh_error(void);

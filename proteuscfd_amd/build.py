@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 
-MESH_SRC = ["mesh.cpp"]
+MESH_SRC = ["mesh.cpp", "mesh_io.cpp"]
 HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_visc.hip", "kernels_jac.hip", "kernels_sgs.hip",
            "kernels_misc.hip", "solver.hip", "kernels_chem.hip", "kernels_mlp.hip"]
 
@@ -50,7 +50,7 @@ def build_hip() -> str:
     srcs = [os.path.join(CSRC, s) for s in HIP_SRC]
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     _run([hipcc, *HIPCC_FLAGS, "-I", INCLUDE, *srcs,
-          os.path.join(CSRC, "mesh.cpp"), "-o", out + ".tmp"])
+          *[os.path.join(CSRC, s) for s in MESH_SRC], "-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out
 

@@ -173,6 +173,35 @@ def plate_case(nx=100, ny=20, nz=40, nlayers=20, first_h=1e-4, growth=1.15, seed
     return Case("plate", m, bct, qref, q0, params, twall=tw)
 
 
+def file_case(path: str, bc: dict, mach: float = 0.5, alpha_deg: float = 0.0, beta_deg: float = 0.0,
+              gamma: float = 1.4, rcm: bool = True, params: Params | None = None, twall: dict | None = None,
+              name: str | None = None) -> Case:
+    """A case on a mesh file (GMSH / UGRID / FieldView / SU2, read by
+    pcfd_mesh_read).  bc maps every face tag in the file to a BC type id
+    (BCTypes, ucs/bc_defines.h); the freestream has angle of attack alpha
+    (x-z plane) and sideslip beta; twall maps noSlip tags to T_wall/T_ref."""
+    m = M.Mesh.read(path, rcm=rcm)
+    tags = set(int(t) for t in np.unique(m.bface_tag))
+    missing = tags - set(bc)
+    if missing:
+        raise ValueError(f"{path}: no boundary condition for face tags {sorted(missing)}")
+    a, b = np.deg2rad(alpha_deg), np.deg2rad(beta_deg)
+    q0 = qinf(gamma, mach, direction=(np.cos(a) * np.cos(b), np.sin(b), np.sin(a) * np.cos(b)))
+    bct = np.full(m.nfactag, BC_FARFIELD, dtype=np.int32)
+    bct[0] = BC_PARALLEL
+    for t, v in bc.items():
+        if not 0 < t < m.nfactag:
+            raise ValueError(f"face tag {t} is outside 1..{m.nfactag - 1}")
+        bct[t] = v
+    qref = np.tile(q0, (m.nfactag, 1))
+    tw = np.full(m.nfactag, -1.0)
+    for t, v in (twall or {}).items():
+        tw[t] = v
+    if params is None:
+        params = Params(gamma=gamma, sorder=2, limiter=3, cfl=20.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
+    return Case(name or path, m, bct, qref, q0, params, twall=tw)
+
+
 def perturbed_state(case: Case, amp: float = 0.05, seed: int = 7) -> np.ndarray:
     """Freestream plus smooth and random perturbations (for parity tests)."""
     rng = np.random.default_rng(seed)

@@ -397,6 +397,9 @@ struct pcfd_mesh {
   std::vector<int> edges, bedges, bfactag, etype, eptr, enodes, btype, bptr, bnodes, btag, perm;
 };
 
+// used by the file readers (mesh_io.cpp)
+void pcfd_mesh_set_error(const std::string& msg) { g_err = msg; }
+
 extern "C" {
 
 const char* pcfd_mesh_error(void) { return g_err.c_str(); }
@@ -694,7 +697,9 @@ static int finish(Work& w, int rcm, pcfd_mesh** out) {
   d.nedge = (int)ned;
   d.nbface = (int)nbf;
   d.nbedge = (int)nhe;
-  d.nfactag = PCFD_NFACTAG;
+  int maxtag = 0;
+  for (int t : m->btag) maxtag = std::max(maxtag, t);
+  d.nfactag = std::max(PCFD_NFACTAG, maxtag + 1);
   d.xyz = m->xyz.data();
   d.vol = m->vol.data();
   d.edges = m->edges.data();

@@ -16,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 WEDGE, BUMP, BOX, PLATE = 1, 2, 3, 4
 TET, PRISM = 2, 4
 NFACTAG = 7
+# pcfd_mesh_read formats
+FMT_AUTO, FMT_GMSH, FMT_UGRID, FMT_FIELDVIEW, FMT_SU2 = 0, 1, 2, 3, 4
 
 
 class MeshSpec(C.Structure):
@@ -68,6 +70,8 @@ def lib():
         L.pcfd_mesh_build_from.argtypes = [C.c_int, dpt, C.c_int, ipt, ipt, ipt, C.c_int, ipt, ipt, ipt, ipt,
                                            C.c_int, C.POINTER(C.c_void_p)]
         L.pcfd_mesh_build_from.restype = C.c_int
+        L.pcfd_mesh_read.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+        L.pcfd_mesh_read.restype = C.c_int
         _lib = L
     return _lib
 
@@ -142,6 +146,16 @@ class Mesh:
         rc = lib().pcfd_mesh_build_from(len(xyz), xyz.ctypes.data_as(C.POINTER(C.c_double)), len(et), ip(et),
                                         ip(ep), ip(en), len(bt), ip(bt), ip(bp), ip(bn), ip(bg), int(rcm),
                                         C.byref(h))
+        if rc != 0:
+            raise RuntimeError(lib().pcfd_mesh_error().decode())
+        return cls(None, _handle=h)
+
+    @classmethod
+    def read(cls, path: str, fmt: int = 0, rcm=False) -> "Mesh":
+        """Read a GMSH 2.2/4.0/4.1, UGRID, FieldView or SU2 ASCII mesh file
+        (format from the extension when fmt is FMT_AUTO)."""
+        h = C.c_void_p()
+        rc = lib().pcfd_mesh_read(os.fsencode(path), int(fmt), int(rcm), C.byref(h))
         if rc != 0:
             raise RuntimeError(lib().pcfd_mesh_error().decode())
         return cls(None, _handle=h)

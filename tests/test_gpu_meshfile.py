@@ -1,0 +1,99 @@
+"""The HIP path on the reference's own mesh files (read by pcfd_mesh_read):
+the structured prism cube (GMSH 4.0) with farfield boundaries and the
+two-plane NACA 0012 prism mesh (FieldView) with symmetry sidewalls, a
+farfield and a slip wall.  Parity with the oracle per phase (bars as in
+test_gpu_parity: 1e-12 / 1e-6 Jacobian) and over implicit iterations.
+"""
+import gzip
+import shutil
+
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import cases
+from proteuscfd_amd.cases import Params
+
+pytestmark = pytest.mark.gpu
+
+GOLD = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def unpack(tmp_path_factory, name):
+    d = tmp_path_factory.mktemp("meshes")
+    out = d / name[:-3]
+    with gzip.open(f"{GOLD}/{name}") as a, open(out, "wb") as b:
+        shutil.copyfileobj(a, b)
+    return str(out)
+
+
+@pytest.fixture(scope="module", params=["cube40", "naca"])
+def setup(request, tmp_path_factory):
+    from proteuscfd_amd import gpu
+    if request.param == "cube40":
+        path = unpack(tmp_path_factory, "cube_structured_v40.msh.gz")
+        tags = [1, 13, 17, 21, 25, 26]
+        case = cases.file_case(path, {t: cases.BC_FARFIELD for t in tags}, mach=0.5, alpha_deg=5.0, beta_deg=3.0,
+                               params=Params(sorder=2, limiter=3, cfl=10.0, cfl_start=1.0, cfl_ramp=10, nsgs=4))
+        q0 = cases.perturbed_state(case, amp=0.05)
+    else:
+        path = unpack(tmp_path_factory, "naca0012_two_plane.crunch.gz")
+        bc = {1: cases.BC_SYMMETRY, 2: cases.BC_SYMMETRY, 3: cases.BC_FARFIELD, 4: cases.BC_IMPERMEABLE}
+        case = cases.file_case(path, bc, mach=0.3, params=Params(sorder=1, cfl=1.0, cfl_start=1.0, nsgs=8))
+        q0 = case.initial_state()
+    pb = O.Problem.from_case(case, clip_jacobi=True)
+    ctx = gpu.Context(case)
+    print(f"[meshfile] {request.param} nnode={case.mesh.nnode} nedge={case.mesh.nedge} nbedge={case.mesh.nbedge}",
+          flush=True)
+    yield request.param, case, pb, ctx, q0
+    ctx.close()
+
+
+def test_phases(setup):
+    name, case, pb, ctx, q0 = setup
+    qo = pb.update_bcs(q0)
+    ctx.upload_q(q0)
+    ctx.update_bcs()
+    assert rel(ctx.download_q()[:, :5], qo[:, :5]) < 1e-13
+    go = pb.gradient(qo)
+    gg = ctx.gradient()
+    for t in range(9):
+        assert rel(gg[:, 3 * t:3 * t + 3], go[:, 3 * t:3 * t + 3]) < 1e-12, f"term {t}"
+    lo = pb.limiter(qo, go) if case.params.limiter else np.ones((case.mesh.nnode, 5))
+    if case.params.limiter:
+        assert rel(ctx.limiter(), lo) < 1e-12
+    ctx.set_limiter(lo)
+    bo, no = pb.residual(qo, go, lo, it=case.params.nfirst + 1)
+    bg, ng = ctx.residual(it=case.params.nfirst + 1)
+    assert rel(bg, bo) < 1e-12
+    np.testing.assert_allclose(ng, no, rtol=1e-12)
+    dto, _ = pb.timestep(qo, 3.0)
+    dtg, _ = ctx.timestep(3.0)
+    assert rel(dtg, dto) < 1e-13
+    ia, ja, iau, Mo, qafter = pb.jacobian(qo, dto)
+    ctx.set_timestep(dto)
+    ctx.jacobian()
+    Mg = ctx.download_jacobian(ia, ja)
+    assert rel(Mg, Mo) < 1e-6
+    assert rel(Mg[iau], Mo[iau]) < 1e-6
+
+
+def test_iterations(setup):
+    name, case, pb, ctx, q0 = setup
+    ctx.set_params(case.params)
+    ctx.upload_q(q0)
+    ctx.reset_iteration()
+    so = O.Solver(pb, q0, order=ctx.sgs_order())
+    for _ in range(6):
+        rg = ctx.iterate()
+        ro, _, _ = so.step()
+        assert abs(rg[0] - ro) <= 1e-8 * ro
+    assert rel(ctx.download_q()[:, :5], so.q[:, :5]) < 1e-9
+    so.close()

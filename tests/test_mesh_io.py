@@ -1,0 +1,278 @@
+"""Mesh file readers (pcfd_mesh_read) on the CPU.
+
+Pinned by: the reference's own known answers on its test meshes (element
+and node counts and total dual volume, ucs unitTest/meshTest.h:97-181; the
+structured cube series volume 1, unitTest/gradientTest.h:44-46), the
+identity sum(dual volumes) == sum(element volumes), and round trips: a mixed
+hex / pyramid / prism / tet mesh written in every format (in each format's
+own node order) reads back to the same edges, areas and volumes as the same
+elements given directly.
+"""
+import gzip
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from proteuscfd_amd import mesh as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TRI, QUAD, TET, PYR, PRI, HEX = 0, 1, 2, 3, 4, 5
+
+
+def gunzip(name, tmp_path):
+    out = tmp_path / name[:-3]
+    with gzip.open(os.path.join(GOLD, name)) as a, open(out, "wb") as b:
+        shutil.copyfileobj(a, b)
+    return str(out)
+
+
+def closure(m):
+    s = np.zeros((m.nnode, 3))
+    A = m.edge_a[:, :3] * m.edge_a[:, 3:4]
+    np.add.at(s, m.edges[:, 0], A)
+    np.add.at(s, m.edges[:, 1], -A)
+    np.add.at(s, m.bedges[:, 0], m.bedge_a[:, :3] * m.bedge_a[:, 3:4])
+    return np.abs(s).max()
+
+
+# --- the reference's test meshes -------------------------------------------
+
+def test_gmsh2_cube_known_answer():
+    m = M.Mesh.read(os.path.join(GOLD, "cube.msh"))
+    assert (m.nnode, m.nelem + m.nbface) == (14, 48)          # meshTest.h:97-109
+    assert abs(m.total_volume - 1.0) < 1e-7
+    assert closure(m) < 1e-12
+
+
+def test_ugrid_nhex_known_answer():
+    m = M.Mesh.read(os.path.join(GOLD, "nhex.ugrid"))
+    assert (m.nnode, m.nelem + m.nbface) == (4276, 10244)     # meshTest.h:140-152
+    assert abs(m.total_volume - 0.00990099) < 1e-7
+    assert closure(m) < 1e-12 and (m.vol > 0).all()
+
+
+def test_fieldview_naca_known_answer(tmp_path):
+    m = M.Mesh.read(gunzip("naca0012_two_plane.crunch.gz", tmp_path))
+    assert (m.nnode, m.nelem + m.nbface) == (5188, 14724)     # meshTest.h:154-166
+    assert abs(m.total_volume - 14.92148669) < 1e-7
+    assert abs(m.total_volume - m.elem_volume) < 1e-12 * m.total_volume
+    assert closure(m) < 1e-12 and (m.vol > 0).all()
+    assert sorted(set(m.bface_tag.tolist())) == [1, 2, 3, 4]   # sidewall_2, sidewall_1, outer, airfoil
+
+
+def test_gmsh40_structured_cube(tmp_path):
+    m = M.Mesh.read(gunzip("cube_structured_v40.msh.gz", tmp_path))
+    assert m.nnode == 21 ** 3
+    assert abs(m.total_volume - 1.0) < 1e-13                  # gradientTest.h:44-46
+    assert abs(m.elem_volume - 1.0) < 1e-13
+    assert closure(m) < 1e-14 and (m.vol > 0).all()
+    # six surfaces without physical groups: faces carry their entity tags
+    assert len(set(m.bface_tag.tolist())) == 6
+
+
+def test_ugrid_fluent_prism():
+    m = M.Mesh.read(os.path.join(GOLD, "prism.ugrid"))
+    assert (m.nnode, m.nelem, m.nbface) == (10, 4, 12)
+    assert abs(m.total_volume - m.elem_volume) < 1e-12 * m.elem_volume
+    assert closure(m) < 1e-9
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(RuntimeError, match="cannot open"):
+        M.Mesh.read(str(tmp_path / "missing.msh"))
+    with pytest.raises(RuntimeError, match="extension"):
+        M.Mesh.read(str(tmp_path / "mesh.xyz"))
+    bad = tmp_path / "second_order.msh"
+    bad.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n1\n1 0 0 0\n$EndNodes\n"
+                   "$Elements\n1\n1 11 2 1 1 1 1 1 1 1 1 1 1 1 1\n$EndElements\n")
+    with pytest.raises(RuntimeError, match="not supported"):
+        M.Mesh.read(str(bad))
+
+
+# --- round trips over a mixed mesh -------------------------------------------
+
+def mixed_mesh():
+    """A hex [0,1]^3, a pyramid on its top, a prism on its x=1 face, a tet on
+    a pyramid face, plus one node that no element uses (index 12)."""
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1],
+                    [0.5, 0.45, 1.6], [1.55, 0, 0.5], [1.5, 1, 0.52], [0.5, -0.7, 1.5], [9, 9, 9]], float)
+    elems = [(HEX, [0, 1, 2, 3, 4, 5, 6, 7]), (PYR, [4, 5, 6, 7, 8]), (PRI, [1, 9, 5, 2, 10, 6]),
+             (TET, [4, 5, 8, 11])]
+    faces = {}
+    local = {TET: [[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]],
+             PYR: [[0, 3, 2, 1], [0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]],
+             PRI: [[0, 2, 1], [3, 4, 5], [0, 1, 4, 3], [1, 2, 5, 4], [2, 0, 3, 5]],
+             HEX: [[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4], [1, 2, 6, 5], [2, 3, 7, 6], [3, 0, 4, 7]]}
+    for t, n in elems:
+        for f in local[t]:
+            key = tuple(sorted(n[k] for k in f))
+            faces.setdefault(key, []).append([n[k] for k in f])
+    bfaces = []
+    for i, (key, fl) in enumerate(sorted(faces.items())):
+        if len(fl) == 1:
+            f = fl[0]
+            bfaces.append((TRI if len(f) == 3 else QUAD, f, 1 + i % 3))
+    return xyz, elems, bfaces
+
+
+def to_file_order(nodes, perm):
+    """local = file[perm]  =>  file[perm[k]] = local[k]"""
+    out = [0] * len(nodes)
+    for k, p in enumerate(perm):
+        out[p] = nodes[k]
+    return out
+
+
+def write_gmsh2(path, xyz, elems, bfaces):
+    code = {TRI: 2, QUAD: 3, TET: 4, HEX: 5, PRI: 6, PYR: 7}
+    rows = [(code[t], [3, 0], n) for t, n, g in [(t, n, 0) for t, n in elems]]
+    rows += [(code[t], [g, g], n) for t, n, g in bfaces]
+    rows.append((15, [1, 1], [0]))                            # a point element, ignored
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(xyz))
+        for i, p in enumerate(xyz):
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, *p))
+        f.write("$EndNodes\n$Elements\n%d\n" % len(rows))
+        for k, (c, tags, n) in enumerate(rows):
+            f.write("%d %d %d %s %s\n" % (k + 1, c, len(tags), " ".join(map(str, tags)),
+                                          " ".join(str(v + 1) for v in n)))
+        f.write("$EndElements\n")
+
+
+def write_gmsh41(path, xyz, elems, bfaces):
+    code = {TRI: 2, QUAD: 3, TET: 4, HEX: 5, PRI: 6, PYR: 7}
+    # surface entity s+10 per tag s, with physical group 100+s; one volume entity
+    tags = sorted({g for _, _, g in bfaces})
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Entities\n0 0 %d 1\n" % len(tags))
+        for s in tags:
+            f.write("%d 0 0 0 1 1 1 1 %d 0\n" % (s + 10, 100 + s))
+        f.write("1 0 0 0 1 1 1 0 %d %s\n$EndEntities\n" % (len(tags), " ".join(str(s + 10) for s in tags)))
+        n = len(xyz)
+        # node tags offset by 1000 and given in two blocks
+        f.write("$Nodes\n2 %d 1001 %d\n" % (n, 1000 + n))
+        h = n // 2
+        for lo, hi in [(0, h), (h, n)]:
+            f.write("3 1 0 %d\n" % (hi - lo))
+            for i in range(lo, hi):
+                f.write("%d\n" % (1001 + i))
+            for i in range(lo, hi):
+                f.write("%.17g %.17g %.17g\n" % tuple(xyz[i]))
+        f.write("$EndNodes\n")
+        blocks = []
+        for s in tags:
+            for t in (TRI, QUAD):
+                fl = [nn for tt, nn, g in bfaces if g == s and tt == t]
+                if fl:
+                    blocks.append((2, s + 10, code[t], fl))
+        for t in (TET, PYR, PRI, HEX):
+            el = [nn for tt, nn in elems if tt == t]
+            if el:
+                blocks.append((3, 1, code[t], el))
+        tot = sum(len(b[3]) for b in blocks)
+        f.write("$Elements\n%d %d 1 %d\n" % (len(blocks), tot, tot))
+        k = 1
+        for d, e, c, el in blocks:
+            f.write("%d %d %d %d\n" % (d, e, c, len(el)))
+            for nn in el:
+                f.write("%d %s\n" % (k, " ".join(str(1001 + v) for v in nn)))
+                k += 1
+        f.write("$EndElements\n")
+
+
+def write_ugrid(path, xyz, elems, bfaces, zero_tags=False):
+    tris = [(n, g) for t, n, g in bfaces if t == TRI]
+    quads = [(n, g) for t, n, g in bfaces if t == QUAD]
+    by = {t: [n for tt, n in elems if tt == t] for t in (TET, PYR, PRI, HEX)}
+    with open(path, "w") as f:
+        f.write("%d %d %d %d %d %d %d\n" % (len(xyz), len(tris), len(quads), len(by[TET]), len(by[PYR]),
+                                            len(by[PRI]), len(by[HEX])))
+        for p in xyz:
+            f.write("%.17g %.17g %.17g\n" % tuple(p))
+        for n, _ in tris + quads:
+            f.write(" ".join(str(v + 1) for v in n[::-1]) + "\n")
+        for _, g in tris + quads:
+            f.write("%d\n" % (-(g - 1) if zero_tags else g))
+        for t in (TET, PYR, PRI, HEX):
+            for n in by[t]:
+                if t == PYR:
+                    n = to_file_order(n, [0, 3, 4, 1, 2])
+                f.write(" ".join(str(v + 1) for v in n) + "\n")
+
+
+def write_fieldview(path, xyz, elems, bfaces):
+    perm = {TET: [0, 1, 3, 2], PYR: [0, 1, 2, 3, 4], PRI: [0, 3, 5, 1, 2, 4], HEX: [0, 1, 3, 2, 4, 5, 7, 6]}
+    code = {TET: 1, HEX: 2, PRI: 3, PYR: 4}
+    with open(path, "w") as f:
+        f.write("FIELDVIEW 2 4\nCONSTANTS\n0.0\n0.0\n0.0\n0.0\nGRIDS 1\nBoundary Table 3\n0 a\n0 b\n0 c\n"
+                "Variable Names 0\nNodes %d\n" % len(xyz))
+        for p in xyz:
+            f.write("%.17g %.17g %.17g\n" % tuple(p))
+        f.write("Boundary Faces\n%d\n" % len(bfaces))
+        for t, n, g in bfaces:
+            f.write("%d %d %s\n" % (g, len(n), " ".join(str(v + 1) for v in n)))
+        f.write("Elements\n")
+        for t, n in elems:
+            f.write("%d 1 %s\n" % (code[t], " ".join(str(v + 1) for v in to_file_order(n, perm[t]))))
+        f.write("Variables\n")
+
+
+def write_su2(path, xyz, elems, bfaces):
+    code = {TRI: 5, QUAD: 9, TET: 10, HEX: 12, PRI: 13, PYR: 14}
+    tags = sorted({g for _, _, g in bfaces})
+    with open(path, "w") as f:
+        f.write("%% mixed test mesh\nNDIME= 3\nNELEM= %d\n" % len(elems))
+        for k, (t, n) in enumerate(elems):
+            f.write("%d %s %d\n" % (code[t], " ".join(map(str, n)), k))
+        f.write("NPOIN= %d\n" % len(xyz))
+        for i, p in enumerate(xyz):
+            f.write("%.17g %.17g %.17g %d\n" % (*p, i))
+        f.write("NMARK= %d\n" % len(tags))
+        for s in tags:
+            fl = [(t, n) for t, n, g in bfaces if g == s]
+            f.write("MARKER_TAG= wall%d\nMARKER_ELEMS= %d\n" % (s, len(fl)))
+            for t, n in fl:
+                f.write("%d %s\n" % (code[t], " ".join(map(str, n))))
+
+
+def same_mesh(a, b, tag_map=lambda g: g):
+    assert (a.nnode, a.nelem, a.nedge, a.nbface, a.nbedge) == (b.nnode, b.nelem, b.nedge, b.nbface, b.nbedge)
+    np.testing.assert_array_equal(a.edges, b.edges)
+    np.testing.assert_allclose(a.edge_a, b.edge_a, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(a.vol, b.vol, rtol=1e-14)
+    # half-edges follow the file's face order: compare them as a set
+    def rows(m, tags):
+        r = np.column_stack([tags, m.bedges[:, 0], np.round(m.bedge_a, 12)])
+        return r[np.lexsort(r.T[::-1])]
+    ra, rb = rows(a, a.bfactag), rows(b, np.vectorize(tag_map)(b.bfactag))
+    np.testing.assert_allclose(ra, rb, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("fmt", ["gmsh2", "gmsh41", "ugrid", "ugrid0", "fieldview", "su2"])
+def test_round_trip(fmt, tmp_path):
+    xyz, elems, bfaces = mixed_mesh()
+    # the direct build drops the unused node as the readers do
+    ref = M.Mesh.from_elements(xyz[:12], elems, bfaces)
+    assert abs(ref.total_volume - ref.elem_volume) < 1e-14
+    assert closure(ref) < 1e-14
+    ext = {"gmsh2": "msh", "gmsh41": "msh", "ugrid": "ugrid", "ugrid0": "ugrid", "fieldview": "crunch",
+           "su2": "su2"}[fmt]
+    path = str(tmp_path / ("mesh." + ext))
+    tag_map = lambda g: g
+    if fmt == "gmsh2":
+        write_gmsh2(path, xyz, elems, bfaces)
+    elif fmt == "gmsh41":
+        write_gmsh41(path, xyz, elems, bfaces)
+        tag_map = lambda g: g - 100 if g > 0 else g          # physical groups 100+s
+    elif fmt == "ugrid":
+        write_ugrid(path, xyz, elems, bfaces)
+    elif fmt == "ugrid0":
+        write_ugrid(path, xyz, elems, bfaces, zero_tags=True)  # tags 0..2 on disk: shifted to 1..3
+    elif fmt == "fieldview":
+        write_fieldview(path, xyz, elems, bfaces)
+    else:
+        write_su2(path, xyz, elems, bfaces)                   # marker index + 1 == tag here
+    got = M.Mesh.read(path)
+    same_mesh(ref, got, tag_map)
