@@ -61,6 +61,8 @@ struct Dev {
   double* lim2 = nullptr;   // 5 x nnode (pre-clip copy)
   double* b = nullptr;      // 5 x nnode
   double* dt = nullptr;     // nnode
+  double* dt_next = nullptr;  // nnode: the time step being computed while the Jacobian rows read dt
+  double* qjac = nullptr;   // 5 x ntot: states the Jacobian rows read when hard-set nodes are committed early
   // SELL block matrix
   int* srow = nullptr;      // nrow_pad: node of each padded row (-1 = padding)
   int* swidth = nullptr;    // nchunk_s
@@ -84,6 +86,11 @@ struct Dev {
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // Jacobian rows, overlapped with the edge passes (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool overlap = true;            // PCFD_JACOBIAN_STREAM=0 turns the overlap off
+  bool jac_pending = false;
+  bool has_hardset = false;       // a Dirichlet / sonic-inflow / noSlip tag exists
   std::string err;
   int nnode = 0, nbnode = 0, nedge = 0, ntot = 0, ntag = 0, nnz = 0, nbnd = 0;
   int nchunk_g = 0, nchunk_s = 0, nrow_pad = 0;
@@ -109,6 +116,7 @@ struct Ctx {
   double phase_ms[PCFD_NPHASE] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // phase timer
   hipEvent_t ev2 = nullptr, ev3 = nullptr;  // kernel sub-timer inside a phase
+  hipEvent_t ev4 = nullptr, ev5 = nullptr;  // Jacobian rows on the second stream
   long long device_bytes = 0;
 };
 
@@ -192,6 +200,8 @@ void launch_norms(Ctx& c, double* host_norms6);
 void launch_timestep(Ctx& c, double cfl);
 double launch_dtmin(Ctx& c);
 void launch_jacobian(Ctx& c);
+void launch_jacobian_overlapped(Ctx& c);  // rows on stream2; jacobian_join() before the matrix is used
+void jacobian_join(Ctx& c);
 void launch_prepare_sgs(Ctx& c);
 void launch_sgs(Ctx& c, int nsgs, double* delta_xnorm);
 void launch_apply_dq(Ctx& c, int explicit_mode, int* nzeroed);

@@ -1,6 +1,7 @@
 // C ABI of libpcfd.so: context lifetime, layouts, transfers and single phases.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -45,14 +46,14 @@ void free_all(Ctx& c) {
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
   dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
   dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.xd); dfree(d.xf); dfree(d.red);
-  dfree(d.red_out); dfree(d.icount);
-  if (c.ev0) (void)hipEventDestroy(c.ev0);
-  if (c.ev1) (void)hipEventDestroy(c.ev1);
-  if (c.ev2) (void)hipEventDestroy(c.ev2);
-  if (c.ev3) (void)hipEventDestroy(c.ev3);
+  dfree(d.red_out); dfree(d.icount); dfree(d.dt_next); dfree(d.qjac);
+  for (hipEvent_t* e : {&c.ev0, &c.ev1, &c.ev2, &c.ev3, &c.ev4, &c.ev5, &c.ev_fork, &c.ev_join}) {
+    if (*e) (void)hipEventDestroy(*e);
+    *e = nullptr;
+  }
+  if (c.stream2) (void)hipStreamDestroy(c.stream2);
   if (c.stream) (void)hipStreamDestroy(c.stream);
-  c.ev0 = c.ev1 = c.ev2 = c.ev3 = nullptr;
-  c.stream = nullptr;
+  c.stream = c.stream2 = nullptr;
 }
 
 double aux_pressure_host(const double* Q, double gm1) {
@@ -130,6 +131,10 @@ void sub_end(Ctx& c, int k) {
   c.phase_ms[k] += ms;
 }
 void ensure_matrix_alloc(Ctx& c) { ensure_matrix(c); }
+void ensure_overlap_buffers(Ctx& c) {
+  if (!c.d.dt_next) dalloc(c, &c.d.dt_next, (size_t)c.nnode);
+  if (c.has_hardset && !c.d.qjac) dalloc(c, &c.d.qjac, 5 * (size_t)c.ntot);
+}
 }  // namespace pcfd
 
 extern "C" {
@@ -179,6 +184,13 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     PCFD_HIP(hipEventCreate(&c.ev1));
     PCFD_HIP(hipEventCreate(&c.ev2));
     PCFD_HIP(hipEventCreate(&c.ev3));
+    PCFD_HIP(hipEventCreate(&c.ev4));
+    PCFD_HIP(hipEventCreate(&c.ev5));
+    // second stream for the Jacobian rows (default priority, like the first)
+    PCFD_HIP(hipStreamCreateWithFlags(&c.stream2, hipStreamNonBlocking));
+    PCFD_HIP(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+    PCFD_HIP(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
+    if (const char* e = std::getenv("PCFD_JACOBIAN_STREAM")) c.overlap = std::strcmp(e, "0") != 0;
     c.nnode = nn;
     c.nbnode = nb;
     c.nedge = ne;
@@ -262,6 +274,11 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
       for (int k = 0; k < 3; ++k) xyz[(size_t)k * nn + i] = m->xyz[3 * (size_t)i + k];
     std::vector<double> vol(m->vol, m->vol + nn);
     c.h_bc_type.assign(m->bc_type, m->bc_type + nt);
+    // hard-set BC types overwrite boundary node states inside the Jacobian
+    for (int h = 0; h < nb; ++h) {
+      const int ty = c.h_bc_type[m->bfactag[h]];
+      if (ty == 1 /* dirichlet */ || ty == 7 /* sonicInflow */ || ty == 4 /* noSlip */) c.has_hardset = true;
+    }
     std::vector<double> qref(6 * (size_t)nt, 0.0);
     for (int t = 0; t < nt; ++t)
       for (int k = 0; k < 5; ++k) qref[(size_t)k * nt + t] = m->qref[5 * (size_t)t + k];
@@ -397,6 +414,7 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
 int pcfd_destroy(pcfd_ctx* ctx) {
   if (!ctx) return PCFD_ERR_ARG;
   (void)hipStreamSynchronize(ctx->c.stream);
+  if (ctx->c.stream2) (void)hipStreamSynchronize(ctx->c.stream2);
   free_all(ctx->c);
   delete ctx;
   return PCFD_OK;

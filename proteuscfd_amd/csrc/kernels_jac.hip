@@ -389,20 +389,32 @@ __global__ void __launch_bounds__(kBlock) k_noslip_rows(JacArgs A, const int* __
   }
 }
 
-void launch_jacobian(Ctx& c) {
+static JacArgs jac_args(Ctx& c) {
   const pcfd_params& p = c.prm;
-  JacArgs A{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase, c.d.gcol,
-            c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type, c.d.qref, c.d.vol,
-            c.d.dt, c.d.q, c.d.srow, c.d.sbase, c.d.xyz, c.d.twall, c.d.he_nrm, p.viscous,
-            ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d},
-            p.flux_type == PCFD_FLUX_ALGEBRAIC ? 1 : 0, p.jac_field_type == 1 ? 1 : 0};
-  sub_begin(c);
-  if (c.nbnd > 0) {
-    const long long bw = (c.nbnd + 11) / 12;
-    hipLaunchKernelGGL(A.flux_alg ? k_jac_boundary<true> : k_jac_boundary<false>, dim3(nblk(bw * kW)),
-                       dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes, c.nbnd, c.d.jdb, c.d.jqb);
-    PCFD_HIP(hipGetLastError());
-  }
+  return JacArgs{c.nnode, c.ntot, c.ntag, c.nbnode, c.nrow_pad, p.symmetry2d, p.gamma, c.d.gwidth, c.d.gbase,
+                 c.d.gcol, c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type,
+                 c.d.qref, c.d.vol, c.d.dt, c.d.q, c.d.srow, c.d.sbase, c.d.xyz, c.d.twall, c.d.he_nrm, p.viscous,
+                 ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d},
+                 p.flux_type == PCFD_FLUX_ALGEBRAIC ? 1 : 0, p.jac_field_type == 1 ? 1 : 0};
+}
+
+static void jac_boundary(Ctx& c, const JacArgs& A, hipStream_t s) {
+  if (c.nbnd == 0) return;
+  const long long bw = (c.nbnd + 11) / 12;
+  hipLaunchKernelGGL(A.flux_alg ? k_jac_boundary<true> : k_jac_boundary<false>, dim3(nblk(bw * kW)), dim3(kBlock),
+                     0, s, A, c.d.bnd_nodes, c.nbnd, c.d.jdb, c.d.jqb);
+  PCFD_HIP(hipGetLastError());
+}
+
+static void jac_commit(Ctx& c, hipStream_t s) {
+  if (c.nbnd == 0) return;
+  hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, s, c.nbnd, c.ntot, c.d.bnd_nodes,
+                     c.d.jqb, c.d.q);
+  PCFD_HIP(hipGetLastError());
+}
+
+static void jac_rows(Ctx& c, const JacArgs& A, hipStream_t s) {
+  const pcfd_params& p = c.prm;
   const long long waves = (c.nrow_pad + 11) / 12;  // 12 rows x 5 lanes per wavefront
   const int grid = nblk(waves * kW);
   const bool f32 = p.precision == PCFD_CHOSEN_FP32;
@@ -410,28 +422,76 @@ void launch_jacobian(Ctx& c) {
   if (f32) {
     auto kern = p.viscous ? (gen ? k_jacobian<float, true, true> : k_jacobian<float, true, false>)
                           : (gen ? k_jacobian<float, false, true> : k_jacobian<float, false, false>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Mf, c.d.Df);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Mf, c.d.Df);
   } else {
     auto kern = p.viscous ? (gen ? k_jacobian<double, true, true> : k_jacobian<double, true, false>)
                           : (gen ? k_jacobian<double, false, true> : k_jacobian<double, false, false>);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, c.stream, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Md, c.d.Dd);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s, A, c.d.bidx, c.d.jdb, c.d.jqb, c.d.Md, c.d.Dd);
   }
   PCFD_HIP(hipGetLastError());
-  if (c.nbnd > 0) {
-    hipLaunchKernelGGL(k_commit_hardset, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, c.nbnd, c.ntot,
-                       c.d.bnd_nodes, c.d.jqb, c.d.q);
-    PCFD_HIP(hipGetLastError());
-  }
-  if (c.has_noslip) {
-    if (p.precision == PCFD_CHOSEN_FP32)
-      hipLaunchKernelGGL(k_noslip_rows<float>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes,
-                         c.nbnd, c.d.spos, c.d.swidth, c.d.scol, c.d.Mf, c.d.Df);
-    else
-      hipLaunchKernelGGL(k_noslip_rows<double>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, c.stream, A, c.d.bnd_nodes,
-                         c.nbnd, c.d.spos, c.d.swidth, c.d.scol, c.d.Md, c.d.Dd);
-    PCFD_HIP(hipGetLastError());
-  }
+}
+
+static void jac_noslip(Ctx& c, const JacArgs& A, hipStream_t s) {
+  if (!c.has_noslip) return;
+  if (c.prm.precision == PCFD_CHOSEN_FP32)
+    hipLaunchKernelGGL(k_noslip_rows<float>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, s, A, c.d.bnd_nodes, c.nbnd,
+                       c.d.spos, c.d.swidth, c.d.scol, c.d.Mf, c.d.Df);
+  else
+    hipLaunchKernelGGL(k_noslip_rows<double>, dim3(nblk(c.nbnd)), dim3(kBlock), 0, s, A, c.d.bnd_nodes, c.nbnd,
+                       c.d.spos, c.d.swidth, c.d.scol, c.d.Md, c.d.Dd);
+  PCFD_HIP(hipGetLastError());
+}
+
+// In reference order on the library stream: boundary part (phantom states,
+// boundary diagonal contributions), rows, then the hard-set commit.
+void launch_jacobian(Ctx& c) {
+  const JacArgs A = jac_args(c);
+  sub_begin(c);
+  jac_boundary(c, A, c.stream);
+  jac_rows(c, A, c.stream);
+  jac_commit(c, c.stream);
+  jac_noslip(c, A, c.stream);
   sub_end(c, 11);
+}
+
+// The rows on the second stream, overlapping the time step, BCs, gradient,
+// limiter and residual that follow on the library stream (they touch
+// neither the matrix nor, with the states the rows read kept apart, the rows'
+// inputs).  The rows read the states from before the hard-set commit -- a
+// snapshot when the case has hard-set nodes, q itself otherwise (then
+// nothing on the library stream writes interior states before the join) --
+// and the dt array the caller then retires (the new time step goes to the
+// other buffer).  jacobian_join() orders the library stream after the rows.
+void launch_jacobian_overlapped(Ctx& c) {
+  JacArgs A = jac_args(c);
+  jac_boundary(c, A, c.stream);
+  if (c.has_hardset && c.nbnd > 0) {
+    PCFD_HIP(hipMemcpyAsync(c.d.qjac, c.d.q, sizeof(double) * 5 * (size_t)c.ntot, hipMemcpyDeviceToDevice,
+                            c.stream));
+    A.q = c.d.qjac;
+    jac_commit(c, c.stream);
+  }
+  PCFD_HIP(hipEventRecord(c.ev_fork, c.stream));
+  PCFD_HIP(hipStreamWaitEvent(c.stream2, c.ev_fork, 0));
+  if (c.timing) PCFD_HIP(hipEventRecord(c.ev4, c.stream2));
+  jac_rows(c, A, c.stream2);
+  jac_noslip(c, A, c.stream2);
+  if (c.timing) PCFD_HIP(hipEventRecord(c.ev5, c.stream2));
+  PCFD_HIP(hipEventRecord(c.ev_join, c.stream2));
+  c.jac_pending = true;
+}
+
+void jacobian_join(Ctx& c) {
+  if (!c.jac_pending) return;
+  PCFD_HIP(hipStreamWaitEvent(c.stream, c.ev_join, 0));
+  c.jac_pending = false;
+  if (c.timing) {
+    PCFD_HIP(hipEventSynchronize(c.ev5));
+    float ms = 0.f;
+    PCFD_HIP(hipEventElapsedTime(&ms, c.ev4, c.ev5));
+    c.phase_ms[5] += ms;
+    c.phase_ms[11] += ms;
+  }
 }
 
 // In-place LU with partial pivoting of each diagonal block (LU,

@@ -9,11 +9,13 @@
 #include <chrono>
 #include <cmath>
 #include <stdexcept>
+#include <utility>
 
 #include "common.h"
 
 namespace pcfd {
 void ensure_matrix_alloc(Ctx& c);
+void ensure_overlap_buffers(Ctx& c);
 }  // namespace pcfd
 
 using namespace pcfd;
@@ -49,10 +51,18 @@ double cfl_update(const pcfd_params& p, int iter, double res, double res_nm1, do
 void iterate(Ctx& c, double* resid) {
   const pcfd_params& p = c.prm;
   // PreTimeAdvance
-  if (p.nsgs > 0 && (c.iter % p.jac_freq == 0 || c.iter == 1)) {
+  const bool do_jac = p.nsgs > 0 && (c.iter % p.jac_freq == 0 || c.iter == 1);
+  const bool overlap = do_jac && c.overlap;
+  if (do_jac) {
     ensure_matrix_alloc(c);
     phase_begin(c);
-    launch_jacobian(c);
+    if (overlap) {
+      ensure_overlap_buffers(c);
+      launch_jacobian_overlapped(c);  // rows read c.d.dt on the second stream...
+      std::swap(c.d.dt, c.d.dt_next);  // ...while the new time step goes to the other buffer
+    } else {
+      launch_jacobian(c);
+    }
     phase_end(c, 5);
     c.jac_valid = true;
     c.lu_done = false;
@@ -84,6 +94,7 @@ void iterate(Ctx& c, double* resid) {
   c.res_nm1 = c.res;  // ucs/solutionSpace.tcc:705-706
   c.res = resid[0];
   if (p.nsgs > 0) {
+    jacobian_join(c);
     if (!c.lu_done) {
       phase_begin(c);
       launch_prepare_sgs(c);
@@ -116,6 +127,7 @@ int pcfd_reset_iteration(pcfd_ctx* ctx) {
     return PCFD_ERR_STATE;
   }
   try {
+    jacobian_join(c);
     c.iter = 1;
     c.jac_valid = false;
     c.lu_done = false;
