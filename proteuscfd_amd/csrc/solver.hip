@@ -52,7 +52,8 @@ void iterate(Ctx& c, double* resid) {
   const pcfd_params& p = c.prm;
   // PreTimeAdvance
   const bool do_jac = p.nsgs > 0 && (c.iter % p.jac_freq == 0 || c.iter == 1);
-  const bool overlap = do_jac && c.overlap;
+  // with phase timing on, every phase runs alone so that its time is its own
+  const bool overlap = do_jac && c.overlap && !c.timing;
   if (do_jac) {
     ensure_matrix_alloc(c);
     phase_begin(c);
