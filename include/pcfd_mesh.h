@@ -123,10 +123,13 @@ const pcfd_mesh_data* pcfd_mesh_get(const pcfd_mesh* m);
 void pcfd_mesh_free(pcfd_mesh* m);
 const char* pcfd_mesh_error(void);
 
-/* Greedy distance-1 colouring of the node graph given by edges (used by the
+/* Distance-1 colouring of the node graph given by edges (used by the
  * multicolour SGS sweeps).  colour[nnode] receives colours 0..ncolor-1;
- * returns ncolor (<0 on error).  Deterministic: nodes visited in id order,
- * smallest free colour taken. */
+ * returns ncolor (<0 on error).  Deterministic: greedy (smallest free
+ * colour) in smallest-last order, then eight rounds of iterated greedy over
+ * whole colour classes, largest first.  Fewer colours mean fewer SGS passes
+ * (2*nsgs*ncolor - (2*nsgs - 1)).  Colour 0 is the largest class and the
+ * last colour the second largest: those are the passes the sweeps skip. */
 int pcfd_color_nodes(int nnode, int nedge, const int* edges, int* colour);
 
 #ifdef __cplusplus

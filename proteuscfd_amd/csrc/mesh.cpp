@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -748,20 +749,89 @@ int pcfd_color_nodes(int nnode, int nedge, const int* edges, int* colour) {
     adj[fill[a]++] = b;
     adj[fill[b]++] = a;
   }
-  for (int i = 0; i < nnode; ++i) colour[i] = -1;
+  if (nnode == 0) return 0;
+  // greedy colouring in a given visiting order, smallest free colour
   std::vector<int> mark;
-  int ncol = 0;
-  for (int i = 0; i < nnode; ++i) {
-    mark.assign(ncol + 1, 0);
-    for (int p = deg[i]; p < deg[i + 1]; ++p) {
-      int c = colour[adj[p]];
-      if (c >= 0) mark[c] = 1;
+  auto greedy = [&](const std::vector<int>& order) {
+    for (int i = 0; i < nnode; ++i) colour[i] = -1;
+    int ncol = 0;
+    for (int v : order) {
+      mark.assign(ncol + 1, 0);
+      for (int p = deg[v]; p < deg[v + 1]; ++p) {
+        int c = colour[adj[p]];
+        if (c >= 0) mark[c] = 1;
+      }
+      int c = 0;
+      while (c < ncol && mark[c]) ++c;
+      colour[v] = c;
+      if (c == ncol) ++ncol;
     }
-    int c = 0;
-    while (c < ncol && mark[c]) ++c;
-    colour[i] = c;
-    if (c == ncol) ++ncol;
+    return ncol;
+  };
+  // smallest-last order (Matula-Beck): repeatedly remove a node of least
+  // remaining degree (ties: lowest id), colour in reverse removal order;
+  // uses at most degeneracy + 1 colours
+  std::vector<int> d(nnode);
+  std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<std::pair<int, int>>> pq;
+  for (int i = 0; i < nnode; ++i) {
+    d[i] = deg[i + 1] - deg[i];
+    pq.push({d[i], i});
   }
+  std::vector<char> gone(nnode, 0);
+  std::vector<int> order;
+  order.reserve(nnode);
+  while (!pq.empty()) {
+    const auto [dv, v] = pq.top();
+    pq.pop();
+    if (gone[v] || d[v] != dv) continue;  // stale entry
+    gone[v] = 1;
+    order.push_back(v);
+    for (int p = deg[v]; p < deg[v + 1]; ++p) {
+      int u = adj[p];
+      if (!gone[u]) pq.push({--d[u], u});
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  int ncol = greedy(order);
+  // iterated greedy (Culberson): revisit whole colour classes, largest
+  // first; a class-contiguous order never needs more colours, and small
+  // classes tend to empty
+  for (int round = 0; round < 8; ++round) {
+    std::vector<int> cnt(ncol, 0);
+    for (int i = 0; i < nnode; ++i) cnt[colour[i]]++;
+    std::vector<int> cls(ncol);
+    std::iota(cls.begin(), cls.end(), 0);
+    std::stable_sort(cls.begin(), cls.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+    std::vector<int> rank(ncol);
+    for (int k = 0; k < ncol; ++k) rank[cls[k]] = k;
+    std::vector<int> start(ncol + 1, 0);
+    for (int c = 0; c < ncol; ++c) start[rank[c] + 1] = cnt[c];
+    for (int k = 0; k < ncol; ++k) start[k + 1] += start[k];
+    std::vector<int> o2(nnode);
+    for (int v : order) o2[start[rank[colour[v]]]++] = v;
+    order.swap(o2);
+    int nc2 = greedy(order);
+    if (nc2 > ncol) {
+      g_err = "pcfd_color_nodes: internal error (iterated greedy grew)";
+      return -1;
+    }
+    ncol = nc2;
+  }
+  // relabel: the multicolour SGS skips the pass over its last colour at each
+  // turn of a symmetric sweep and over colour 0 at the start of each later
+  // forward sweep, so the largest class becomes colour 0, the second largest
+  // the last colour, the rest (largest first) in between
+  std::vector<int> cnt(ncol, 0);
+  for (int i = 0; i < nnode; ++i) cnt[colour[i]]++;
+  std::vector<int> cls(ncol);
+  std::iota(cls.begin(), cls.end(), 0);
+  std::stable_sort(cls.begin(), cls.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+  std::vector<int> label(ncol);
+  for (int k = 0; k < ncol; ++k) {
+    int pos = k == 0 ? 0 : (k == 1 ? ncol - 1 : k - 1);
+    label[cls[k]] = pos;
+  }
+  for (int i = 0; i < nnode; ++i) colour[i] = label[colour[i]];
   return ncol;
 }
 
