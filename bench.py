@@ -240,8 +240,8 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        print("bench.py runs on one GPU only: the solver has no multi-GPU path "
+              "(a node-partitioned design with halo exchange is not built). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

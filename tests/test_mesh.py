@@ -71,7 +71,7 @@ def test_wedge_counts_volume_closure(n):
     np.testing.assert_allclose(np.linalg.norm(m.edge_a[:, :3], axis=1), 1.0, atol=1e-15)
 
 
-def test_cfg1_counts_match_survey():
+def test_wedge_60x30x2_counts():
     # 5,673 nodes, 31,232 edges, 7,920 boundary triangles, 23,760 half-edges
     m = M.Mesh.build(M.WEDGE)
     assert (m.nnode, m.nedge, m.nbface, m.nbedge) == (5673, 31232, 7920, 23760)
