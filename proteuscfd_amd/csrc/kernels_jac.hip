@@ -76,11 +76,72 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
   const double h = 1.0e-8;
   const double gamma = A.gamma;
   double Dc[5] = {0, 0, 0, 0, 0};
-  {
+  const int p0 = A.he_ptr[i], p1 = A.he_ptr[i + 1];
+  // A node whose half-edges are all of types that leave its own state alone
+  // (no noSlip / Dirichlet re-set) sees the same unperturbed phantom state
+  // and flux in its five lanes: at every fifth half-edge lane c evaluates
+  // them for half-edge p + c (and writes that phantom), and each lane reads
+  // half-edge p's from lane (p - p0) % 5 of its group.
+  bool plain = true;
+  for (int p = p0; p < p1; ++p) {
+    const int type = A.bc_type[A.he_tag[A.he_id[p]]];
+    if (type == BC_NOSLIP || is_dirichlet(type)) plain = false;
+  }
+  if (plain) {
     double QL[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) QL[k] = A.q[k * A.ntot + i];
-    for (int p = A.he_ptr[i]; p < A.he_ptr[i + 1]; ++p) {
+    const int g0 = lane64 - c;
+    double fSg[5] = {0, 0, 0, 0, 0}, QRg[5] = {0, 0, 0, 0, 0};
+    for (int p = p0; p < p1; ++p) {
+      const int k5 = (p - p0) % 5;
+      if (k5 == 0 && p + c < p1) {
+        const int hc = A.he_id[p + c];
+        const int tc = A.he_tag[hc];
+        const double ac[4] = {A.he_a[hc], A.he_a[A.nbnode + hc], A.he_a[2 * A.nbnode + hc],
+                              A.he_a[3 * A.nbnode + hc]};
+        double Qr[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+          QRg[k] = A.q[k * A.ntot + A.nnode + hc];
+          Qr[k] = A.qref[k * A.ntag + tc];
+        }
+        if (bc_state(A.bc_type[tc], QL, QRg, Qr, A.qref[5 * A.ntag + tc], ac, gamma)) {
+#pragma unroll
+          for (int k = 0; k < 5; ++k) A.q[k * A.ntot + A.nnode + hc] = QRg[k];
+        }
+        num_flux<ALG>(QL, QRg, ac, gamma, A.sym2d, fSg);
+      }
+      double fS[5], QR[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        fS[k] = __shfl(fSg[k], g0 + k5);
+        QR[k] = __shfl(QRg[k], g0 + k5);
+      }
+      const int hh = A.he_id[p];
+      const int tag = A.he_tag[hh];
+      const double a[4] = {A.he_a[hh], A.he_a[A.nbnode + hh], A.he_a[2 * A.nbnode + hh], A.he_a[3 * A.nbnode + hh]};
+      double Qr[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) Qr[k] = A.qref[k * A.ntag + tag];
+      double QPL[5];
+      perturb(QL, c, h, QPL);
+      double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
+      bc_state(A.bc_type[tag], QPL, QPR, Qr, A.qref[5 * A.ntag + tag], a, gamma);
+      double fL[5];
+      num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
+#pragma unroll
+      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
+    }
+    if (c == 0) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) Qb[(long long)t * 5 + k] = QL[k];
+    }
+  } else {
+    double QL[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) QL[k] = A.q[k * A.ntot + i];
+    for (int p = p0; p < p1; ++p) {
       const int hh = A.he_id[p];
       const int tag = A.he_tag[hh];
       const int type = A.bc_type[tag];
