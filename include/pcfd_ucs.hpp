@@ -6,6 +6,8 @@
  *                             ucs/solutionSpace.tcc:572-904)
  *   pcfd::ucs::CRS            CRS<Real> (ucs/crs.h:13-81) over the device
  *                             block matrix (CRSMatrix, ucs/crsmatrix.h:10-107)
+ *   pcfd::ucs::Mesh           a library-built mesh: Mesh<Real>::Read* readers
+ *                             (ucs/mesh.tcc) or the synthetic generator
  *   pcfd::ucs::Error          thrown where the reference would Abort
  *                             (ucs/exceptions.h:30-49); carries the ABI code
  *
@@ -45,6 +47,46 @@ struct BoundaryConditions {
   std::vector<int> type;     /* [nfactag] */
   std::vector<double> qref;  /* [nfactag*5] */
   std::vector<double> twall; /* [nfactag] noSlip wall temperature / T_ref, <0 adiabatic; empty: adiabatic */
+};
+
+/* Owner of a mesh built by the library: read from a file with the reader
+ * the reference's Mesh would use (Mesh<Type>::Read*, ucs/mesh.tcc), or
+ * generated.  data() is what Describe() takes. */
+class Mesh {
+ public:
+  static Mesh Read(const std::string& path, bool rcm = true, int format = PCFD_MESH_FMT_AUTO) {
+    pcfd_mesh* m = nullptr;
+    if (pcfd_mesh_read(path.c_str(), format, rcm ? 1 : 0, &m) != 0) throw Error(PCFD_ERR_ARG, pcfd_mesh_error());
+    return Mesh(m);
+  }
+  static Mesh ReadGMSH_Master(const std::string& path) { return Read(path, true, PCFD_MESH_FMT_GMSH); }
+  static Mesh ReadUGRID_Ascii(const std::string& path) { return Read(path, true, PCFD_MESH_FMT_UGRID); }
+  static Mesh ReadCRUNCH_Ascii(const std::string& path) { return Read(path, true, PCFD_MESH_FMT_FIELDVIEW); }
+  static Mesh ReadSU2_Ascii(const std::string& path) { return Read(path, true, PCFD_MESH_FMT_SU2); }
+  static Mesh Build(const pcfd_mesh_spec& spec) {
+    pcfd_mesh* m = nullptr;
+    if (pcfd_mesh_build(&spec, &m) != 0) throw Error(PCFD_ERR_ARG, pcfd_mesh_error());
+    return Mesh(m);
+  }
+  Mesh(Mesh&& o) noexcept : m_(o.m_) { o.m_ = nullptr; }
+  Mesh& operator=(Mesh&& o) noexcept {
+    if (this != &o) {
+      if (m_) pcfd_mesh_free(m_);
+      m_ = o.m_;
+      o.m_ = nullptr;
+    }
+    return *this;
+  }
+  Mesh(const Mesh&) = delete;
+  Mesh& operator=(const Mesh&) = delete;
+  ~Mesh() {
+    if (m_) pcfd_mesh_free(m_);
+  }
+  const pcfd_mesh_data& data() const { return *pcfd_mesh_get(m_); }
+
+ private:
+  explicit Mesh(pcfd_mesh* m) : m_(m) {}
+  pcfd_mesh* m_ = nullptr;
 };
 
 /* Mesh arrays as CalcMetrics leaves them (ucs/mesh.h:199-254): edges with

@@ -11,10 +11,13 @@
 //   f64    q[(nnode+nbnode)*5]  state after the iterations
 //   int32  order[nnode]         SGS row order
 //
-// usage: ucs_newton N ITERS OUT [precision]
+// usage: ucs_newton N|MESHFILE ITERS OUT [precision]
+//   N: the sin^2-bump N^3 mesh, slip floor; MESHFILE: a GMSH / UGRID /
+//   FieldView / SU2 file read with RCM numbering, farfield on every face tag
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "pcfd_ucs.hpp"
@@ -28,31 +31,36 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "usage: %s N ITERS OUT [precision]\n", argv[0]);
     return 2;
   }
-  const int n = std::atoi(argv[1]), iters = std::atoi(argv[2]);
+  const bool from_file = std::strspn(argv[1], "0123456789") != std::strlen(argv[1]);
+  const int n = from_file ? 0 : std::atoi(argv[1]), iters = std::atoi(argv[2]);
   const int precision = argc > 4 ? std::atoi(argv[4]) : PCFD_PLAIN_FP64;
   const double gamma = 1.4, mach = 0.5;
 
-  pcfd_mesh_spec spec;
-  pcfd_mesh_default_spec(PCFD_MESH_BUMP, &spec);
-  spec.nx = spec.ny = spec.nz = n;
-  pcfd_mesh* mh = nullptr;
-  if (pcfd_mesh_build(&spec, &mh) != 0) {
-    std::fprintf(stderr, "mesh: %s\n", pcfd_mesh_error());
-    return 1;
-  }
-  const pcfd_mesh_data& m = *pcfd_mesh_get(mh);
-
-  // freestream (ucs/compressible.tcc:823-867): rho 1, |u| = M, p = 1/gamma
-  const double p = 1.0 / gamma;
-  const double qinf[5] = {1.0, mach, 0.0, 0.0, p / (gamma - 1.0) + 0.5 * mach * mach};
-  // farfield on all sides, slip floor (face tag 5 = zmin)
-  BoundaryConditions bc;
-  bc.type = {0, 6, 6, 6, 6, 3, 6};
-  for (int t = 0; t < m.nfactag; ++t)
-    for (int k = 0; k < 5; ++k) bc.qref.push_back(qinf[k]);
-
   int rc = 0;
   try {
+    pcfd_mesh_spec spec;
+    pcfd_mesh_default_spec(PCFD_MESH_BUMP, &spec);
+    spec.nx = spec.ny = spec.nz = n;
+    const pcfd::ucs::Mesh mesh = from_file ? pcfd::ucs::Mesh::Read(argv[1]) : pcfd::ucs::Mesh::Build(spec);
+    const pcfd_mesh_data& m = mesh.data();
+
+    // freestream (ucs/compressible.tcc:823-867): rho 1, |u| = M, p = 1/gamma,
+    // along x for the bump; for a file at 5 deg incidence and 3 deg sideslip, so
+    // that no box face is exactly tangential (the farfield BC switches branch
+    // on the sign of the normal velocity)
+    const double pi = 3.141592653589793;
+    const double al = from_file ? 5.0 * pi / 180.0 : 0.0, be = from_file ? 3.0 * pi / 180.0 : 0.0;
+    const double dir[3] = {std::cos(al) * std::cos(be), std::sin(be), std::sin(al) * std::cos(be)};
+    const double p = 1.0 / gamma;
+    const double qinf[5] = {1.0, mach * dir[0], mach * dir[1], mach * dir[2], p / (gamma - 1.0) + 0.5 * mach * mach};
+    // bump: farfield on all sides, slip floor (face tag 5 = zmin); file: farfield everywhere
+    BoundaryConditions bc;
+    bc.type.assign(m.nfactag, 6);
+    bc.type[0] = 0;
+    if (!from_file) bc.type[5] = 3;
+    for (int t = 0; t < m.nfactag; ++t)
+      for (int k = 0; k < 5; ++k) bc.qref.push_back(qinf[k]);
+
     SolutionSpace space(pcfd::ucs::Describe(m, bc));
     CRS crs(space);
     pcfd_params prm{};
@@ -81,11 +89,12 @@ int main(int argc, char** argv) {
       double w = 0.0;
       if (i < m.nnode) w = 0.02 * std::sin(3.0 * m.xyz[3 * i]) * std::cos(2.0 * m.xyz[3 * i + 1]);
       const double r = 1.0 + w, u = mach * (1.0 - w), pp = p * (1.0 + 0.5 * w);
+      const double vel[3] = {u * dir[0], u * dir[1] + 0.1 * w, u * dir[2]};
       q[(size_t)i * 5 + 0] = r;
-      q[(size_t)i * 5 + 1] = r * u;
-      q[(size_t)i * 5 + 2] = r * 0.1 * w;
-      q[(size_t)i * 5 + 3] = 0.0;
-      q[(size_t)i * 5 + 4] = pp / (gamma - 1.0) + 0.5 * r * (u * u + 0.01 * w * w);
+      q[(size_t)i * 5 + 1] = r * vel[0];
+      q[(size_t)i * 5 + 2] = r * vel[1];
+      q[(size_t)i * 5 + 3] = r * vel[2];
+      q[(size_t)i * 5 + 4] = pp / (gamma - 1.0) + 0.5 * r * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
     }
 
     // one residual evaluation, phase by phase
@@ -116,12 +125,11 @@ int main(int argc, char** argv) {
     std::fwrite(qout.data(), sizeof(double), qout.size(), f);
     std::fwrite(order.data(), sizeof(int), order.size(), f);
     std::fclose(f);
-    std::printf("ucs_newton: bump %d^3, %d nodes, %d iterations, residGlobal %.6e -> %.6e\n", n, m.nnode, iters,
+    std::printf("ucs_newton: %s, %d nodes, %d iterations, residGlobal %.6e -> %.6e\n", argv[1], m.nnode, iters,
                 resid.empty() ? 0.0 : resid.front(), resid.empty() ? 0.0 : resid.back());
   } catch (const pcfd::ucs::Error& e) {
     std::fprintf(stderr, "pcfd error %d: %s\n", e.code(), e.what());
     rc = 1;
   }
-  pcfd_mesh_free(mh);
   return rc;
 }
