@@ -194,6 +194,7 @@ void launch_lsq_moments(Ctx& c);
 void launch_update_bcs(Ctx& c);
 void launch_gradient(Ctx& c, int nterms);
 void launch_limiter(Ctx& c);
+void launch_gradient_limiter(Ctx& c, int nterms);  // LSQ gradient + limiter in one pass, then the clip
 void launch_residual(Ctx& c, int higher_order);  // inviscid, then viscous / noSlip edits
 void launch_viscous(Ctx& c, int higher_order);
 void launch_norms(Ctx& c, double* host_norms6);

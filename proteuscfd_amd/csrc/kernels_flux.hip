@@ -148,14 +148,76 @@ __device__ __forceinline__ void grad_terms(const double* Q, double gm1, double g
 // symmetry-plane projection Bkernel_Symmetry_Fix (:544-565).  The per-node
 // Gram-Schmidt factors of ComputeLSQCoefficients (:140-168) depend only on
 // the node's moments and are formed once per node.
-template <int NT>
+// Limiter::Compute (ucs/limiters.tcc:52-132): neighbour min/max with both
+// bounds starting at 0 (:59-63), then the per-edge-end limiter value, the
+// minimum over the node's edges.  limit_node is the second part, given the
+// node's state, its first five gradient terms and the neighbour bounds.
+__device__ __forceinline__ void limit_node(int i, int nnode, int ntot, int type, double K3, double chi,
+                                           const Gell& g, const double* __restrict__ xyz,
+                                           const double* __restrict__ vol, const double* __restrict__ q,
+                                           const double* qi, const double* gi, const double* qmin,
+                                           const double* qmax, double* phi) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) phi[k] = 1.0;
+  const long long b = gbase_of(g, i);
+  const int w = g.width[i >> 6];
+  double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
+  const double ep2 = 6.0 * 3.141592653589793 * vol[i] * K3;
+  for (int s = 0; s < w; ++s) {
+    const int j = g.col[b + (long long)s * kW];
+    if (j < 0) break;
+    const bool left = i < j;
+    double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
+    double dx[3];
+    // dx = 0.5 (xR - xL) for the left end, its negation for the right end
+    if (left) { dx[0] = 0.5 * (xj[0] - xi[0]); dx[1] = 0.5 * (xj[1] - xi[1]); dx[2] = 0.5 * (xj[2] - xi[2]); }
+    else { dx[0] = -(0.5 * (xi[0] - xj[0])); dx[1] = -(0.5 * (xi[1] - xj[1])); dx[2] = -(0.5 * (xi[2] - xj[2])); }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      double qj = q[k * ntot + j];
+      double dQ = left ? (qj - qi[k]) : -(qi[k] - qj);
+      double Qh = extrap1(qi[k], dQ, gi[3 * k], gi[3 * k + 1], gi[3 * k + 2], dx, chi, 1.0);
+      double t;
+      if (type == 3) {
+        double DM = Qh - qi[k];
+        double DP;
+        if (left) DP = (Qh > qi[k]) ? qmax[k] - qi[k] : ((Qh < qi[k]) ? qmin[k] - qi[k] : 0.0);
+        else DP = (Qh > qi[k]) ? qmax[k] - qi[k] : qmin[k] - qi[k];
+        t = (DP * DP + ep2 + 2.0 * DM * DP) / (DP * DP + 2.0 * DM * DM + DM * DP + ep2);
+      } else {
+        t = 1.0;
+        if (Qh > qi[k]) t = (qmax[k] - qi[k]) / (Qh - qi[k]);
+        else if (Qh < qi[k]) t = (qmin[k] - qi[k]) / (Qh - qi[k]);
+        if (type == 2) {
+          t = (t * t + 2.0 * t) / (t * t + t + 2.0);
+        } else {
+          t = dmax(0.0, t);
+          t = dmin(1.0, t);
+        }
+      }
+      phi[k] = dmin(phi[k], t);
+    }
+  }
+}
+
+struct LimArgs {  // the fused limiter's inputs and output (LIM = true)
+  int type;
+  double K3, chi;
+  const double* vol;
+  double* lim2;
+};
+
+// With LIM the same kernel also forms the limiter (k_limiter's arithmetic):
+// the neighbour bounds come from the states the gradient loop reads, and the
+// per-edge limiter loop runs on the gradient still in registers.
+template <int NT, bool LIM>
 __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g, const double* __restrict__ xyz,
                                                     const double* __restrict__ sw, const double* __restrict__ q,
                                                     const int* __restrict__ he_ptr, const int* __restrict__ he_id,
                                                     const int* __restrict__ he_tag,
                                                     const double* __restrict__ he_a, int nbnode,
                                                     const int* __restrict__ bc_type, double gamma,
-                                                    double* __restrict__ grad) {
+                                                    double* __restrict__ grad, LimArgs L) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
   const double gm1 = gamma - 1.0;
@@ -179,6 +241,7 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g
   double ti[NT];
   grad_terms<NT>(Qi, gm1, gamma, ti);
   double xi = xyz[i], yi = xyz[nnode + i], zi = xyz[2 * nnode + i];
+  double qmin[5] = {0, 0, 0, 0, 0}, qmax[5] = {0, 0, 0, 0, 0};
   const long long b = gbase_of(g, i);
   const int w = g.width[i >> 6];
   for (int s = 0; s < w; ++s) {
@@ -187,6 +250,13 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g
     double Qj[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Qj[k] = q[k * ntot + j];
+    if constexpr (LIM) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        qmax[k] = dmax(qmax[k], Qj[k]);
+        qmin[k] = dmin(qmin[k], Qj[k]);
+      }
+    }
     double tj[NT];
     grad_terms<NT>(Qj, gm1, gamma, tj);
     double xj = xyz[j], yj = xyz[nnode + j], zj = xyz[2 * nnode + j];
@@ -227,6 +297,12 @@ __global__ void __launch_bounds__(kBlock) k_gradient(int nnode, int ntot, Gell g
   }
 #pragma unroll
   for (int k = 0; k < NT * 3; ++k) grad[k * nnode + i] = gr[k];
+  if constexpr (LIM) {
+    double phi[5];
+    limit_node(i, nnode, ntot, L.type, L.K3, L.chi, g, xyz, L.vol, q, Qi, gr, qmin, qmax, phi);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) L.lim2[k * nnode + i] = phi[k];
+  }
 }
 
 // Green-Gauss gradient, gradientType 1 (Kernel_/Bkernel_Green_Gauss_Gradient,
@@ -325,22 +401,17 @@ void launch_gradient(Ctx& c, int nterms) {
     PCFD_HIP(hipGetLastError());
     return;
   }
-  if (nterms == 9) {
-    hipLaunchKernelGGL(k_gradient<9>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
-                       c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
-                       c.prm.gamma, c.d.grad);
-  } else {
-    hipLaunchKernelGGL(k_gradient<5>, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c),
-                       c.d.xyz, c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type,
-                       c.prm.gamma, c.d.grad);
-  }
+  const LimArgs none{0, 0.0, 0.0, nullptr, nullptr};
+  auto kern = nterms == 9 ? k_gradient<9, false> : k_gradient<5, false>;
+  hipLaunchKernelGGL(kern, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c), c.d.xyz,
+                     c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type, c.prm.gamma,
+                     c.d.grad, none);
   PCFD_HIP(hipGetLastError());
 }
 
+
 // ---------------------------------------------------------------------------
-// Limiter::Compute (ucs/limiters.tcc:52-132): neighbour min/max with both
-// bounds starting at 0 (:59-63), then the per-edge-end limiter value, the
-// minimum over the node's edges.  Writes the pre-clip limiter to lim2.
+// Writes the pre-clip limiter to lim2.
 __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int type, double K3, double chi, Gell g,
                                                    const double* __restrict__ xyz, const double* __restrict__ vol,
                                                    const double* __restrict__ q, const double* __restrict__ grad,
@@ -353,7 +424,6 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
     qi[k] = q[k * ntot + i];
     qmin[k] = 0.0;
     qmax[k] = 0.0;
-    phi[k] = 1.0;
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
@@ -369,43 +439,7 @@ __global__ void __launch_bounds__(kBlock) k_limiter(int nnode, int ntot, int typ
       qmin[k] = dmin(qmin[k], v);
     }
   }
-  double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
-  const double ep2 = 6.0 * 3.141592653589793 * vol[i] * K3;
-  for (int s = 0; s < w; ++s) {
-    const int j = g.col[b + (long long)s * kW];
-    if (j < 0) break;
-    const bool left = i < j;
-    double xj[3] = {xyz[j], xyz[nnode + j], xyz[2 * nnode + j]};
-    double dx[3];
-    // dx = 0.5 (xR - xL) for the left end, its negation for the right end
-    if (left) { dx[0] = 0.5 * (xj[0] - xi[0]); dx[1] = 0.5 * (xj[1] - xi[1]); dx[2] = 0.5 * (xj[2] - xi[2]); }
-    else { dx[0] = -(0.5 * (xi[0] - xj[0])); dx[1] = -(0.5 * (xi[1] - xj[1])); dx[2] = -(0.5 * (xi[2] - xj[2])); }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-      double qj = q[k * ntot + j];
-      double dQ = left ? (qj - qi[k]) : -(qi[k] - qj);
-      double Qh = extrap1(qi[k], dQ, gi[3 * k], gi[3 * k + 1], gi[3 * k + 2], dx, chi, 1.0);
-      double t;
-      if (type == 3) {
-        double DM = Qh - qi[k];
-        double DP;
-        if (left) DP = (Qh > qi[k]) ? qmax[k] - qi[k] : ((Qh < qi[k]) ? qmin[k] - qi[k] : 0.0);
-        else DP = (Qh > qi[k]) ? qmax[k] - qi[k] : qmin[k] - qi[k];
-        t = (DP * DP + ep2 + 2.0 * DM * DP) / (DP * DP + 2.0 * DM * DM + DM * DP + ep2);
-      } else {
-        t = 1.0;
-        if (Qh > qi[k]) t = (qmax[k] - qi[k]) / (Qh - qi[k]);
-        else if (Qh < qi[k]) t = (qmin[k] - qi[k]) / (Qh - qi[k]);
-        if (type == 2) {
-          t = (t * t + 2.0 * t) / (t * t + t + 2.0);
-        } else {
-          t = dmax(0.0, t);
-          t = dmin(1.0, t);
-        }
-      }
-      phi[k] = dmin(phi[k], t);
-    }
-  }
+  limit_node(i, nnode, ntot, type, K3, chi, g, xyz, vol, q, qi, gi, qmin, qmax, phi);
 #pragma unroll
   for (int k = 0; k < 5; ++k) lim2[k * nnode + i] = phi[k];
 }
@@ -479,6 +513,20 @@ void launch_limiter(Ctx& c) {
   double K3 = p.venkat_K * p.venkat_K * p.venkat_K;
   hipLaunchKernelGGL(k_limiter, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.limiter, K3,
                      p.chi, gell(c), c.d.xyz, c.d.vol, c.d.q, c.d.grad, c.d.lim2);
+  PCFD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
+                     p.gamma, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
+  PCFD_HIP(hipGetLastError());
+}
+
+// weighted LSQ gradient and the limiter in one pass, then the pressure clip
+void launch_gradient_limiter(Ctx& c, int nterms) {
+  const pcfd_params& p = c.prm;
+  const LimArgs L{p.limiter, p.venkat_K * p.venkat_K * p.venkat_K, p.chi, c.d.vol, c.d.lim2};
+  auto kern = nterms == 9 ? k_gradient<9, true> : k_gradient<5, true>;
+  hipLaunchKernelGGL(kern, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, gell(c), c.d.xyz,
+                     c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type, p.gamma,
+                     c.d.grad, L);
   PCFD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
                      p.gamma, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);

@@ -77,16 +77,24 @@ void iterate(Ctx& c, double* resid) {
   launch_update_bcs(c);
   phase_end(c, 0);
   const bool ho = p.sorder > 1 && c.iter > p.nfirst;
-  if (ho || p.viscous) {  // ucs/solutionSpace.tcc:671-677
-    phase_begin(c);
-    launch_gradient(c, p.gradient_terms);
-    phase_end(c, 1);
-  }
-  if ((c.iter % p.limiter_refresh == 0) && p.sorder > 1 && p.limiter && c.iter > p.nfirst &&
-      c.iter < p.limiter_freeze) {
-    phase_begin(c);
-    launch_limiter(c);
-    phase_end(c, 2);
+  const bool grad_now = ho || p.viscous;  // ucs/solutionSpace.tcc:671-677
+  const bool lim_now = (c.iter % p.limiter_refresh == 0) && p.sorder > 1 && p.limiter && c.iter > p.nfirst &&
+                       c.iter < p.limiter_freeze;
+  if (grad_now && lim_now && p.gradient_type == 0 && !c.timing) {
+    // one pass for the weighted-LSQ gradient and the limiter (phase timing
+    // keeps them apart so that each has its own time)
+    launch_gradient_limiter(c, p.gradient_terms);
+  } else {
+    if (grad_now) {
+      phase_begin(c);
+      launch_gradient(c, p.gradient_terms);
+      phase_end(c, 1);
+    }
+    if (lim_now) {
+      phase_begin(c);
+      launch_limiter(c);
+      phase_end(c, 2);
+    }
   }
   phase_begin(c);
   launch_residual(c, ho ? 1 : 0);
