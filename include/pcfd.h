@@ -98,6 +98,10 @@ typedef struct {
                             2 (complex step) gives PCFD_ERR_UNSUPPORTED */
   int flux_type;         /* PCFD_FLUX_ROE (roeFlux, the default) or PCFD_FLUX_ALGEBRAIC (algebraicFlux,
                             ucs/eqnset.tcc:56-77); 2 (HLLC) gives PCFD_ERR_UNSUPPORTED */
+  int jac_boundary_type; /* jacobianBoundaryType: 0 one-sided FD, 1 centred FD (Bkernel_NumJac_Centered,
+                            ucs/jacobian.tcc:538-640, with jacobianBoundaryEval 0: the +h evaluation
+                            re-evaluates the boundary state, the -h one keeps the unperturbed one);
+                            2 (complex step) gives PCFD_ERR_UNSUPPORTED */
 } pcfd_params;
 
 #define PCFD_FLUX_ROE 0

@@ -33,7 +33,8 @@ class OParams(C.Structure):
                 ("clip_jacobi", C.c_int), ("viscous", C.c_int), ("re_tilde", C.c_double), ("pr", C.c_double),
                 ("prt", C.c_double), ("sutherland_s", C.c_double), ("gradient_type", C.c_int),
                 ("global_dt", C.c_int), ("fixed_dt", C.c_double), ("enable_vnn", C.c_int), ("vnn", C.c_double),
-                ("dynamic_cfl", C.c_int), ("jac_field_type", C.c_int), ("flux_type", C.c_int)]
+                ("dynamic_cfl", C.c_int), ("jac_field_type", C.c_int), ("flux_type", C.c_int),
+                ("jac_boundary_type", C.c_int)]
 
 
 class OChem(C.Structure):
@@ -151,7 +152,7 @@ class Problem:
                          int(params.viscous), params.re_tilde, params.prandtl, params.prandtl_t,
                          params.sutherland_s, params.gradient_type, params.global_dt, params.fixed_dt,
                          params.enable_vnn, params.vnn, params.dynamic_cfl, params.jac_field_type,
-                         params.flux_type)
+                         params.flux_type, params.jac_boundary_type)
 
     @property
     def bnormal(self):

@@ -1189,8 +1189,30 @@ void o_jacobian(const o_mesh* m, const o_params* p, double* q, const double* dt,
     double tw = tag_twall(m, tag);
     const double* ns = normal_state(m, q, e);
     o_bc_state_ex(bct, QL, QR, Qref, a, g, tw, ns);
-    numflux(p, QL, QR, a, fS);
     double* D = &A->M[25 * (size_t)A->iau[L]];
+    if (p->jac_boundary_type == 1) {
+      /* Bkernel_NumJac_Centered (ucs/jacobian.tcc:538-640) with jacobianBoundaryEval = 0
+       * (ucs/param.tcc:174): the +h side re-evaluates the boundary state from the
+       * perturbed node state (:577-584), the -h side uses the unperturbed boundary
+       * state (:606-608), so even a hard-set (Dirichlet) node gives half a one-sided
+       * column */
+      for (int i = 0; i < NE; i++) {
+        memcpy(QPL, QL, sizeof(QPL));
+        QPL[i] += h;
+        o_aux(QPL, g);
+        memcpy(QPR, QR, sizeof(QPR));
+        o_aux(QPR, g);
+        o_bc_state_ex(bct, QPL, QPR, Qref, a, g, tw, ns);
+        numflux(p, QPL, QPR, a, fL);
+        memcpy(QPL, QL, sizeof(QPL));
+        QPL[i] -= h;
+        o_aux(QPL, g);
+        numflux(p, QPL, QR, a, fS);
+        for (int j = 0; j < NE; j++) D[j * NE + i] += (fL[j] - fS[j]) / (2.0 * h);
+      }
+      continue;
+    }
+    numflux(p, QL, QR, a, fS);
     for (int i = 0; i < NE; i++) {
       memcpy(QPL, QL, sizeof(QPL));
       QPL[i] += h;

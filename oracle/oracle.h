@@ -69,6 +69,7 @@ typedef struct {
   int dynamic_cfl;              /* dynamicCFL */
   int jac_field_type;           /* jacobianFieldType: 0 one-sided FD, 1 centred FD */
   int flux_type;                /* 0 Roe, 1 algebraic */
+  int jac_boundary_type;        /* jacobianBoundaryType: 0 one-sided FD, 1 centred FD */
 } o_params;
 
 typedef struct {

@@ -88,6 +88,7 @@ class Params:
     dynamic_cfl: int = 0          # dynamicCFL
     jac_field_type: int = 0       # jacobianFieldType: 0 one-sided, 1 centred
     flux_type: int = 0            # 0 Roe (roeFlux), 1 algebraic (algebraicFlux)
+    jac_boundary_type: int = 0    # jacobianBoundaryType: 0 one-sided, 1 centred
 
     @property
     def re_tilde(self) -> float:  # Re / Mach (ucs/compressible.tcc:753-755)

@@ -434,17 +434,18 @@ int pcfd_set_params(pcfd_ctx* ctx, const pcfd_params* p) {
       c.err = "pcfd_set_params: parameter out of range (viscous runs need gradient_terms 9, Re/M, Pr, PrT > 0)";
       return PCFD_ERR_ARG;
     }
-    if (p->flux_type == 2 || p->jac_field_type == 2) {
-      c.err = "pcfd_set_params: the HLLC flux (fluxType 2) and the complex-step Jacobian (jacobianFieldType 2) "
-              "are not built";
+    if (p->flux_type == 2 || p->jac_field_type == 2 || p->jac_boundary_type == 2) {
+      c.err = "pcfd_set_params: the HLLC flux (fluxType 2) and the complex-step Jacobians (jacobianFieldType 2, "
+              "jacobianBoundaryType 2) are not built";
       return PCFD_ERR_UNSUPPORTED;
     }
     if ((p->gradient_type != 0 && p->gradient_type != 1) || (p->global_dt != 0 && p->global_dt != 1) ||
         (p->enable_vnn != 0 && p->enable_vnn != 1) || (p->enable_vnn && !(p->vnn > 0.0)) ||
         (p->dynamic_cfl != 0 && p->dynamic_cfl != 1) || (p->jac_field_type != 0 && p->jac_field_type != 1) ||
+        (p->jac_boundary_type != 0 && p->jac_boundary_type != 1) ||
         (p->flux_type != PCFD_FLUX_ROE && p->flux_type != PCFD_FLUX_ALGEBRAIC)) {
-      c.err = "pcfd_set_params: option out of range (gradient_type, jac_field_type, global_dt, enable_vnn, "
-              "dynamic_cfl 0/1; flux_type PCFD_FLUX_ROE/PCFD_FLUX_ALGEBRAIC; vnn > 0 when enabled)";
+      c.err = "pcfd_set_params: option out of range (gradient_type, jac_field_type, jac_boundary_type, global_dt, "
+              "enable_vnn, dynamic_cfl 0/1; flux_type PCFD_FLUX_ROE/PCFD_FLUX_ALGEBRAIC; vnn > 0 when enabled)";
       return PCFD_ERR_ARG;
     }
     c.prm = *p;

@@ -40,6 +40,7 @@ struct JacArgs {
   ViscParams vp;
   int flux_alg;            // 1: algebraic flux (generic form only)
   int centred;             // 1: centred differences, jacobianFieldType 1 (generic form only)
+  int bcentred;            // 1: centred boundary differences, jacobianBoundaryType 1
 };
 
 // One-sided differences with h = 1e-8 (ucs/jacobian.tcc:268): the quotient
@@ -64,6 +65,18 @@ __device__ __forceinline__ void perturb(const double* Q, int c, double h, double
 // lanes per node as in k_jacobian.  Writes the node's boundary part of the
 // diagonal block to Db[t*25 + r*5 + c] (t = boundary-node index) and the
 // re-evaluated phantom states (lane 0).
+//
+// With A.bcentred (Bkernel_NumJac_Centered, ucs/jacobian.tcc:538-640, at the
+// default jacobianBoundaryEval 0) column c is (F(q+h e_c, BC(q+h e_c)) -
+// F(q-h e_c, QR)) / 2h: the +h side re-evaluates the boundary state, the -h
+// side keeps the unperturbed one QR.  bnd_down() forms that -h flux.
+template <bool ALG>
+__device__ __forceinline__ void bnd_down(const double* QL, const double* QR, int c, const double* a, double gamma,
+                                         int sym2d, double* fD) {
+  double QPD[5];
+  perturb(QL, c, -1.0e-8, QPD);
+  num_flux<ALG>(QPD, QR, a, gamma, sym2d, fD);
+}
 template <bool ALG>
 __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* __restrict__ bnd_nodes, int nbnd,
                                                         double* __restrict__ Db, double* __restrict__ Qb) {
@@ -75,6 +88,8 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
   const int i = bnd_nodes[t];
   const double h = 1.0e-8;
   const double gamma = A.gamma;
+  const bool cen = A.bcentred != 0;
+  const double w = cen ? kInv2H : kInvH;
   double Dc[5] = {0, 0, 0, 0, 0};
   const int p0 = A.he_ptr[i], p1 = A.he_ptr[i + 1];
   // A node whose half-edges are all of types that leave its own state alone
@@ -110,7 +125,7 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
 #pragma unroll
           for (int k = 0; k < 5; ++k) A.q[k * A.ntot + A.nnode + hc] = QRg[k];
         }
-        num_flux<ALG>(QL, QRg, ac, gamma, A.sym2d, fSg);
+        if (!cen) num_flux<ALG>(QL, QRg, ac, gamma, A.sym2d, fSg);
       }
       double fS[5], QR[5];
 #pragma unroll
@@ -130,8 +145,9 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
       bc_state(A.bc_type[tag], QPL, QPR, Qr, A.qref[5 * A.ntag + tag], a, gamma);
       double fL[5];
       num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
+      if (cen) bnd_down<ALG>(QL, QR, c, a, gamma, A.sym2d, fS);
 #pragma unroll
-      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
+      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * w;
     }
     if (c == 0) {
 #pragma unroll
@@ -172,22 +188,31 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
           for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
         }
         double fS[5], fL[5], QPL[5];
-        num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
+        if (cen) bnd_down<ALG>(QL, QR, c, a, gamma, A.sym2d, fS);
+        else num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
         perturb(QL, c, h, QPL);
         double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
         noslip_state(QPL, QPR, tw, nQ, gamma);
         num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
-        for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
+        for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * w;
         continue;
       }
       if (is_dirichlet(type)) {
         // QL = QR = Qref and every perturbed evaluation resets both to Qref,
-        // so the reference's contribution is exactly zero
+        // so the one-sided contribution is exactly zero; the centred one keeps
+        // its -h side: (F(Qref, Qref) - F(Qref - h e_c, Qref)) / 2h
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
           QL[k] = Qr[k];
           if (c == 0) A.q[k * A.ntot + ph] = Qr[k];
+        }
+        if (cen) {
+          double fU[5], fD[5];
+          num_flux<ALG>(Qr, Qr, a, gamma, A.sym2d, fU);
+          bnd_down<ALG>(Qr, Qr, c, a, gamma, A.sym2d, fD);
+#pragma unroll
+          for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fU[rr] - fD[rr]) * w;
         }
         continue;
       }
@@ -196,7 +221,8 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
         for (int k = 0; k < 5; ++k) A.q[k * A.ntot + ph] = QR[k];
       }
       double fS[5];
-      num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
+      if (cen) bnd_down<ALG>(QL, QR, c, a, gamma, A.sym2d, fS);
+      else num_flux<ALG>(QL, QR, a, gamma, A.sym2d, fS);
       double QPL[5];
       perturb(QL, c, h, QPL);
       double QPR[5] = {QR[0], QR[1], QR[2], QR[3], QR[4]};
@@ -204,7 +230,7 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
       double fL[5];
       num_flux<ALG>(QPL, QPR, a, gamma, A.sym2d, fL);
 #pragma unroll
-      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * kInvH;
+      for (int rr = 0; rr < 5; ++rr) Dc[rr] += (fL[rr] - fS[rr]) * w;
     }
     // the node state after the re-evaluation: the viscous Jacobian, which
     // the reference runs after Bkernel_NumJac, sees it
@@ -487,7 +513,8 @@ static JacArgs jac_args(Ctx& c) {
                  c.d.gcol, c.d.gnrm, c.nslot_g * kW, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.d.bc_type,
                  c.d.qref, c.d.vol, c.d.dt, c.d.q, c.d.srow, c.d.sbase, c.d.xyz, c.d.twall, c.d.he_nrm, p.viscous,
                  ViscParams{p.gamma, p.re_tilde, p.prandtl, p.prandtl_t, p.sutherland_s, p.symmetry2d},
-                 p.flux_type == PCFD_FLUX_ALGEBRAIC ? 1 : 0, p.jac_field_type == 1 ? 1 : 0};
+                 p.flux_type == PCFD_FLUX_ALGEBRAIC ? 1 : 0, p.jac_field_type == 1 ? 1 : 0,
+                 p.jac_boundary_type == 1 ? 1 : 0};
 }
 
 static void jac_boundary(Ctx& c, const JacArgs& A, hipStream_t s) {

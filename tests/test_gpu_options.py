@@ -1,6 +1,8 @@
 """Parity of the HIP path against the oracle for the reference's non-default
 options: Green-Gauss gradient, VNN / global / fixed time steps, dynamic
-CFL, the centred finite-difference Jacobian and the algebraic flux.
+CFL, the centred finite-difference interior and boundary Jacobians and the
+algebraic flux.  The wedge's sonic inflow covers the hard-set (Dirichlet)
+branch of the centred boundary form.
 
 Bars as for the default path: FP64 phases 1e-12 relative (1e-13 for the
 time step), the finite-difference Jacobian 1e-6 (flux rounding is
@@ -102,11 +104,12 @@ def test_algebraic_residual(setup, order):
     np.testing.assert_allclose(ng, no, rtol=1e-12)
 
 
-@pytest.mark.parametrize("flux,field", [(0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("flux,field,bnd", [(0, 1, 0), (1, 0, 0), (1, 1, 0), (0, 0, 1), (1, 1, 1)])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_option_jacobian(setup, flux, field, precision):
+def test_option_jacobian(setup, flux, field, bnd, precision):
     name, case, ctx, q0 = setup
-    p = with_(case.params, flux_type=flux, jac_field_type=field, nsgs=4, precision=precision)
+    p = with_(case.params, flux_type=flux, jac_field_type=field, jac_boundary_type=bnd, nsgs=4,
+              precision=precision)
     pb = problem(case, p)
     qo = pb.update_bcs(q0)
     dto, _ = pb.timestep(qo, 5.0)
@@ -125,8 +128,8 @@ def test_option_jacobian(setup, flux, field, precision):
 
 
 RUNS = {
-    # implicit: Green-Gauss, VNN, dynamic CFL, centred Jacobian
-    "bump": dict(gradient_type=1, enable_vnn=1, vnn=4.0, dynamic_cfl=1, jac_field_type=1),
+    # implicit: Green-Gauss, VNN, dynamic CFL, centred interior and boundary Jacobians
+    "bump": dict(gradient_type=1, enable_vnn=1, vnn=4.0, dynamic_cfl=1, jac_field_type=1, jac_boundary_type=1),
     # explicit: algebraic flux, one global dt, dynamic CFL
     "wedge": dict(flux_type=1, global_dt=1, dynamic_cfl=1, sorder=1),
 }
@@ -154,7 +157,8 @@ def test_option_iterations(setup):
 
 def test_unbuilt_and_out_of_range_options_rejected(setup):
     name, case, ctx, q0 = setup
-    for kw, code in [(dict(flux_type=2), "(-4)"), (dict(jac_field_type=2), "(-4)"), (dict(gradient_type=2), "(-1)"),
+    for kw, code in [(dict(flux_type=2), "(-4)"), (dict(jac_field_type=2), "(-4)"), (dict(jac_boundary_type=2), "(-4)"),
+                     (dict(jac_boundary_type=3), "(-1)"), (dict(gradient_type=2), "(-1)"),
                      (dict(enable_vnn=1, vnn=0.0), "(-1)"), (dict(dynamic_cfl=3), "(-1)")]:
         with pytest.raises(RuntimeError) as e:
             ctx.set_params(with_(case.params, **kw))

@@ -86,13 +86,18 @@ def test_viscous_residual(setup, it):
     np.testing.assert_allclose(ng, no, rtol=1e-11)
 
 
+@pytest.mark.parametrize("bnd", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_viscous_jacobian(setup, precision):
+def test_viscous_jacobian(setup, precision, bnd):
+    """bnd 1: the centred boundary Jacobian (jacobianBoundaryType 1) on the
+    noSlip wall and farfield / outflow faces."""
     name, case, pb, ctx, q0 = setup
+    p = Params(**{**case.params.__dict__, "jac_boundary_type": bnd})
+    pb = O.Problem(case.mesh, case.bc_type, case.qref, p, clip_jacobi=True, twall=case.twall)
     qo = pb.update_bcs(q0)
     dto, _ = pb.timestep(qo, 5.0)
     ia, ja, iau, Mo, qafter = pb.jacobian(qo, dto)
-    ctx.set_params(Params(**{**case.params.__dict__, "precision": precision}))
+    ctx.set_params(Params(**{**p.__dict__, "precision": precision}))
     ctx.upload_q(qo)
     ctx.set_timestep(dto)
     ctx.jacobian()

@@ -10,6 +10,8 @@ flux for the Jacobian blocks, freestream preservation and discrete
 conservation with a textbook flux, and a matrix-free derivative of the
 residual.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -316,3 +318,64 @@ def test_reference_sine_gradient_on_structured_cube(gradient_type, tmp_path):
         assert np.abs(g[:, 3:]).max() <= 1e-15
         exact = 4 * np.pi * np.cos(4 * np.pi * m.xyz[:, d])
         assert np.all(np.abs(g[:, d] - exact) <= 0.485 * np.abs(exact))
+
+
+def _boundary_case(kind):
+    if kind == "wedge":
+        return cases.wedge_case(nx=6, ny=4, nz=2)
+    case = cases.bump_case(n=4)
+    if kind == "bump_noslip":
+        bct = case.bc_type.copy()
+        bct[5] = cases.BC_NOSLIP
+        tw = np.full(len(bct), -1.0)
+        tw[5] = 1.0
+        case = dataclasses.replace(case, bc_type=bct, twall=tw)
+    return case
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+@pytest.mark.parametrize("kind", ["bump", "wedge", "bump_noslip"])
+def test_centred_boundary_jacobian_is_half_reevaluated_half_frozen(kind, flux):
+    """jacobianBoundaryType 1 with jacobianBoundaryEval 0 differences the +h
+    side through the re-evaluated boundary state and the -h side against
+    the unperturbed one, so its diagonal block is the mean of the one-sided
+    block (the total derivative, pinned by the matrix-free test above) and
+    the derivative of the residual with the phantom states frozen, taken
+    here by finite differences of the residual one node state at a time.
+    Hard-set nodes (sonic inflow, noSlip) are included: their one-sided
+    boundary part is zero or hard-set, their centred one is not."""
+    case = _boundary_case(kind)
+    m = case.mesh
+    out = []
+    for bnd in (0, 1):
+        p = Params(sorder=1, nsgs=2, flux_type=flux, jac_boundary_type=bnd)
+        pb = O.Problem(m, case.bc_type, case.qref, p, twall=case.twall)
+        q = pb.update_bcs(cases.perturbed_state(case, amp=0.1))
+        dt, _ = pb.timestep(q, 1.0)
+        out.append((pb, dt) + tuple(pb.jacobian(q, dt)))
+    (pb, dt, ia, ja, iau, Mo, qj), (_, _, _, _, _, Mc, qjc) = out
+    np.testing.assert_array_equal(qj, qjc)
+    b0, _ = pb.residual(qj)
+    bnodes = np.unique(m.bedges[:, 0])
+    wall = set(m.bedges[case.bc_type[m.bfactag] == cases.BC_NOSLIP, 0].tolist())
+    rng = np.random.default_rng(11)
+    checked = 0
+    for i in rng.choice(bnodes, min(24, len(bnodes)), replace=False):
+        Jf = np.empty((5, 5))
+        for k in range(5):
+            eps = 1e-6 * max(abs(qj[i, k]), 0.1)
+            qp = qj.copy()
+            qp[i, k] += eps
+            qp = cases.aux(qp, G)
+            bp, _ = pb.residual(qp)
+            Jf[:, k] = -(bp[i] - b0[i]) / eps
+        vdt = m.vol[i] / dt[i]
+        Do = Mo[iau[i]] - vdt * np.eye(5)
+        Dc = Mc[iau[i]] - vdt * np.eye(5)
+        want = 0.5 * (Do + Jf)
+        # noSlip rows 1-4 of an isothermal wall node are replaced after the
+        # fill (Bkernel_BC_Jac_Modify) in both forms
+        rows = [0] if i in wall else slice(None)
+        np.testing.assert_allclose(Dc[rows], want[rows], atol=2e-5 * np.abs(want).max(), err_msg=f"node {i}")
+        checked += np.abs(Dc - Do).max() > 1e-3 * np.abs(Do).max()
+    assert checked > 0
