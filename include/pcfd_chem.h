@@ -58,6 +58,23 @@ const char* pcfd_chem_last_error(const pcfd_chem* c);
  * T[n] K -> wdot[n*ns] kg/(m^3 s). */
 int pcfd_chem_wdot(pcfd_chem* c, int n, const double* rhoi, const double* T, double* wdot);
 
+/* CompressibleFREqnSet::ComputeAuxiliaryVariables (ucs/compressibleFR.tcc:754-814) for n states,
+ * ideal-gas EOS per species (ucs/EOS.tcc:36-38, 79-82).  q [n*(ns+4)]: rho_1..rho_ns, u, v, w, T
+ * (non-dimensional); aux [n*(2+2ns)]: p, rho, cv_1..cv_ns, conc_1..conc_ns with
+ *   p = sum_i (rho_i rho_ref) R_i (T T_ref) / ref_pressure,  rho = sum_i rho_i,
+ *   cv_i = (Cp_i(T T_ref) - R_i) / (ref_velocity^2 / T_ref),  conc_i = rho_i / MW_i / 1000
+ * (MassToMole, ucs/chem.tcc:980-986, applied to the non-dimensional rho_i as the reference does).
+ * ref_pressure (Pa) and ref_velocity (m/s) are the case's Param scales (ucs/param.tcc:355-375). */
+int pcfd_chem_fr_aux(pcfd_chem* c, int n, const double* q, double ref_pressure, double ref_velocity, double* aux);
+
+/* CompressibleFREqnSet::NewtonFindTGivenP (ucs/compressibleFR.tcc:2342-2378): the temperature at
+ * which the species densities rhoi [n*ns] give the pressure p [n] (both non-dimensional), by
+ * Newton iteration from Tinit [n] with a forward difference (1e-8 K) for dP/dT, at most 28 steps,
+ * stopping when |dT| / T_ref < 1e-15.  T [n] non-dimensional; iters [n] (may be NULL): the step
+ * at which it stopped, 28 when it did not converge (the reference warns and returns the last T). */
+int pcfd_chem_t_from_p(pcfd_chem* c, int n, const double* rhoi, const double* p, const double* Tinit,
+                       double ref_pressure, double* T, int* iters);
+
 /* SourceTerm (and SourceTermJacobian when jac != NULL) for n cells (host
  * arrays): q[n*neqn] FR state, vol[n] -> src[n*neqn], jac[n*neqn*neqn]
  * (row-major dS_j/dQ_i at [j*neqn + i], one-sided h = 1e-8). */

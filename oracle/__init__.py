@@ -109,6 +109,10 @@ def lib():
         L.o_chem_source_jacobian.argtypes = [C.POINTER(OChem), dp, C.c_double, dp]
         L.o_chem_cp.argtypes = [C.POINTER(OChem), C.c_int, C.c_double]
         L.o_chem_cp.restype = C.c_double
+        L.o_fr_aux.argtypes = [C.POINTER(OChem), dp, C.c_double, C.c_double, dp]
+        L.o_fr_t_from_p.argtypes = [C.POINTER(OChem), dp, C.c_double, C.c_double, C.c_double,
+                                    C.POINTER(C.c_int)]
+        L.o_fr_t_from_p.restype = C.c_double
         _lib = L
     return _lib
 
@@ -370,3 +374,16 @@ class Chem:
 
     def cp(self, s, T):
         return lib().o_chem_cp(C.byref(self.c), s, T)
+
+    def fr_aux(self, Q, ref_pressure, ref_velocity):
+        """p, rho, cv_1..cv_ns, conc_1..conc_ns of one FR state."""
+        a = np.zeros(2 + 2 * self.model.ns)
+        lib().o_fr_aux(C.byref(self.c), _d(_f64(Q)), float(ref_pressure), float(ref_velocity), _d(a))
+        return a
+
+    def t_from_p(self, rhoi, p, Tinit, ref_pressure):
+        """(T, Newton steps) for one state."""
+        it = C.c_int(0)
+        T = lib().o_fr_t_from_p(C.byref(self.c), _d(_f64(rhoi)), float(p), float(Tinit), float(ref_pressure),
+                                C.byref(it))
+        return T, it.value

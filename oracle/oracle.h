@@ -148,6 +148,8 @@ void o_chem_source(const o_chem* c, const double* Q, double vol, double* source)
 /* SourceTermJacobian by one-sided differences, h = 1e-8 */
 void o_chem_source_jacobian(const o_chem* c, const double* Q, double vol, double* A);
 double o_chem_cp(const o_chem* c, int s, double T);
+void o_fr_aux(const o_chem* c, const double* Q, double ref_pressure, double ref_velocity, double* aux);
+double o_fr_t_from_p(const o_chem* c, const double* rhoi, double p, double Tinit, double ref_pressure, int* iters);
 
 /* Full solver state for the steady Newton / explicit loop. */
 typedef struct {

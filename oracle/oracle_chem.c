@@ -151,3 +151,53 @@ double o_chem_cp(const o_chem* c, int s, double T) {
   double cp_R = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
   return cp_R * (UNIV_R / c->mw[s]);
 }
+
+/* ChemModel::GetP (ucs/chem.tcc:988-997) with IdealGasEOS::GetP
+ * (ucs/EOS.tcc:36-38): P = sum_i (rho_i R_i) T, SI */
+static double fr_p(const o_chem* c, const double* rhoiDim, double TDim) {
+  double P = 0.0;
+  for (int i = 0; i < c->ns; i++) P += rhoiDim[i] * (UNIV_R / c->mw[i]) * TDim;
+  return P;
+}
+
+/* CompressibleFREqnSet::ComputeAuxiliaryVariables (ucs/compressibleFR.tcc:
+ * 754-814): aux = p, rho, cv_1..cv_ns (Mayer, ucs/EOS.tcc:79-82, scaled by
+ * ref_velocity^2 / T_ref), conc_1..conc_ns (MassToMole, ucs/chem.tcc:980-986,
+ * on the non-dimensional rho_i as the reference passes them) */
+void o_fr_aux(const o_chem* c, const double* Q, double ref_pressure, double ref_velocity, double* aux) {
+  int ns = c->ns;
+  double rho = 0.0, rd[O_CHEM_MAXS];
+  for (int i = 0; i < ns; i++) {
+    rho += Q[i];
+    rd[i] = Q[i] * c->ref_density;
+  }
+  double TDim = Q[ns + 3] * c->ref_temperature;
+  aux[0] = fr_p(c, rd, TDim) / ref_pressure;
+  aux[1] = rho;
+  double s_ref = ref_velocity * ref_velocity / c->ref_temperature;
+  for (int i = 0; i < ns; i++) {
+    aux[2 + i] = (o_chem_cp(c, i, TDim) - UNIV_R / c->mw[i]) / s_ref;
+    aux[2 + ns + i] = Q[i] / c->mw[i] / 1000.0;
+  }
+}
+
+/* CompressibleFREqnSet::NewtonFindTGivenP (ucs/compressibleFR.tcc:2342-2378) */
+double o_fr_t_from_p(const o_chem* c, const double* rhoi, double p, double Tinit, double ref_pressure, int* iters) {
+  const int maxit = 28;
+  const double tol = 1.0e-15, Tref = c->ref_temperature;
+  double rd[O_CHEM_MAXS];
+  for (int i = 0; i < c->ns; i++) rd[i] = rhoi[i] * c->ref_density;
+  double TDim = Tinit * Tref, PgoalDim = p * ref_pressure;
+  int j;
+  for (j = 0; j < maxit; j++) {
+    double TpDim = TDim + 1.0e-8;
+    double z = PgoalDim - fr_p(c, rd, TDim);
+    double zp = PgoalDim - fr_p(c, rd, TpDim);
+    double dzdT = (zp - z) / (TpDim - TDim);
+    double dT = -z / dzdT;
+    if (fabs(dT / Tref) < tol) break;
+    TDim += dT;
+  }
+  if (iters) *iters = j;
+  return TDim / Tref;
+}

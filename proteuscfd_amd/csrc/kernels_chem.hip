@@ -217,6 +217,125 @@ const char* pcfd_chem_last_error(const pcfd_chem* c) { return c ? c->err.c_str()
 
 double pcfd_chem_last_kernel_ms(const pcfd_chem* c) { return c ? (double)c->last_ms : 0.0; }
 
+// Species::GetCp (ucs/species.tcc:43-53) with the coefficient set of
+// GetThermoCoeff (T > 1000 K: high), J/(kg K)
+__device__ __forceinline__ double cp_of(const ChemDev& m, int s, double T) {
+  const double* a = T > 1000.0 ? m.hi[s] : m.lo[s];
+  const double cp_R = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])));
+  return cp_R * (kUnivR / m.mw[s]);
+}
+
+// ChemModel::GetP (ucs/chem.tcc:988-997): Dalton's law over the species'
+// ideal-gas EOS, P += (rho_i R_i) T in species order, SI units
+__device__ __forceinline__ double p_of(const ChemDev& m, const double* rhoiDim, double TDim) {
+  double P = 0.0;
+  for (int i = 0; i < m.ns; ++i) P += rhoiDim[i] * (kUnivR / m.mw[i]) * TDim;
+  return P;
+}
+
+// ComputeAuxiliaryVariables (ucs/compressibleFR.tcc:754-814), one thread per state
+__global__ void __launch_bounds__(kChemBlock) k_fr_aux(const ChemDev* __restrict__ mp, int n, const double* __restrict__ q, double pref,
+                                                      double s_ref, double* __restrict__ aux) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const ChemDev& m = *mp;
+  const int ns = m.ns, neqn = ns + 4, naux = 2 + 2 * ns;
+  const double* Q = q + (long long)t * neqn;
+  double* X = aux + (long long)t * naux;
+  double rho = 0.0, rd[PCFD_CHEM_MAXS];
+  for (int i = 0; i < ns; ++i) {
+    rho += Q[i];
+    rd[i] = Q[i] * m.ref_density;
+  }
+  const double TDim = Q[ns + 3] * m.ref_temperature;
+  X[0] = p_of(m, rd, TDim) / pref;
+  X[1] = rho;
+  for (int i = 0; i < ns; ++i) {
+    X[2 + i] = (cp_of(m, i, TDim) - kUnivR / m.mw[i]) / s_ref;
+    X[2 + ns + i] = Q[i] / m.mw[i] / 1000.0;
+  }
+}
+
+// NewtonFindTGivenP (ucs/compressibleFR.tcc:2342-2378), one thread per state
+__global__ void __launch_bounds__(kChemBlock) k_fr_t_from_p(const ChemDev* __restrict__ mp, int n, const double* __restrict__ rhoi,
+                                                           const double* __restrict__ p,
+                                                           const double* __restrict__ Tinit, double pref,
+                                                           double* __restrict__ T, int* __restrict__ iters) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const ChemDev& m = *mp;
+  const int ns = m.ns, maxit = 28;
+  const double tol = 1.0e-15, Tref = m.ref_temperature;
+  double rd[PCFD_CHEM_MAXS];
+  for (int i = 0; i < ns; ++i) rd[i] = rhoi[(long long)t * ns + i] * m.ref_density;
+  double TDim = Tinit[t] * Tref;
+  const double PgoalDim = p[t] * pref;
+  int j = 0;
+  for (j = 0; j < maxit; ++j) {
+    const double TpDim = TDim + 1.0e-8;
+    const double z = PgoalDim - p_of(m, rd, TDim);
+    const double zp = PgoalDim - p_of(m, rd, TpDim);
+    const double dzdT = (zp - z) / (TpDim - TDim);
+    const double dT = -z / dzdT;
+    if (fabs(dT / Tref) < tol) break;
+    TDim += dT;
+  }
+  T[t] = TDim / Tref;
+  if (iters) iters[t] = j;
+}
+
+int pcfd_chem_fr_aux(pcfd_chem* c, int n, const double* q, double ref_pressure, double ref_velocity, double* aux) {
+  return guarded_chem(c, [&](pcfd_chem& cc) -> int {
+    if (n < 0 || (n > 0 && (!q || !aux)) || !(ref_pressure > 0.0) || !(ref_velocity > 0.0)) {
+      cc.err = "pcfd_chem_fr_aux: invalid arguments (n >= 0, buffers, ref_pressure > 0, ref_velocity > 0)";
+      return PCFD_ERR_ARG;
+    }
+    if (n == 0) return PCFD_OK;
+    const int ns = cc.host.ns;
+    const size_t nq = (size_t)n * (ns + 4), na = (size_t)n * (2 + 2 * ns);
+    DevBuf dq(sizeof(double) * nq), da(sizeof(double) * na);
+    check(hipMemcpyAsync(dq.p, q, sizeof(double) * nq, hipMemcpyHostToDevice, cc.stream), "copy");
+    check(hipEventRecord(cc.e0, cc.stream), "hipEventRecord");
+    const double s_ref = ref_velocity * ref_velocity / cc.host.ref_temperature;
+    hipLaunchKernelGGL(k_fr_aux, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, cc.stream, cc.dev, n,
+                       dq.as<double>(), ref_pressure, s_ref, da.as<double>());
+    check(hipGetLastError(), "k_fr_aux");
+    check(hipEventRecord(cc.e1, cc.stream), "hipEventRecord");
+    check(hipMemcpyAsync(aux, da.p, sizeof(double) * na, hipMemcpyDeviceToHost, cc.stream), "copy");
+    check(hipStreamSynchronize(cc.stream), "hipStreamSynchronize");
+    check(hipEventElapsedTime(&cc.last_ms, cc.e0, cc.e1), "hipEventElapsedTime");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_chem_t_from_p(pcfd_chem* c, int n, const double* rhoi, const double* p, const double* Tinit,
+                       double ref_pressure, double* T, int* iters) {
+  return guarded_chem(c, [&](pcfd_chem& cc) -> int {
+    if (n < 0 || (n > 0 && (!rhoi || !p || !Tinit || !T)) || !(ref_pressure > 0.0)) {
+      cc.err = "pcfd_chem_t_from_p: invalid arguments (n >= 0, buffers, ref_pressure > 0)";
+      return PCFD_ERR_ARG;
+    }
+    if (n == 0) return PCFD_OK;
+    const int ns = cc.host.ns;
+    DevBuf dr(sizeof(double) * (size_t)n * ns), dp(sizeof(double) * (size_t)n), dt0(sizeof(double) * (size_t)n),
+        dT(sizeof(double) * (size_t)n), di(sizeof(int) * (size_t)n);
+    check(hipMemcpyAsync(dr.p, rhoi, sizeof(double) * (size_t)n * ns, hipMemcpyHostToDevice, cc.stream), "copy");
+    check(hipMemcpyAsync(dp.p, p, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, cc.stream), "copy");
+    check(hipMemcpyAsync(dt0.p, Tinit, sizeof(double) * (size_t)n, hipMemcpyHostToDevice, cc.stream), "copy");
+    check(hipEventRecord(cc.e0, cc.stream), "hipEventRecord");
+    hipLaunchKernelGGL(k_fr_t_from_p, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, cc.stream,
+                       cc.dev, n, dr.as<double>(), dp.as<double>(), dt0.as<double>(), ref_pressure, dT.as<double>(),
+                       di.as<int>());
+    check(hipGetLastError(), "k_fr_t_from_p");
+    check(hipEventRecord(cc.e1, cc.stream), "hipEventRecord");
+    check(hipMemcpyAsync(T, dT.p, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, cc.stream), "copy");
+    if (iters) check(hipMemcpyAsync(iters, di.p, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, cc.stream), "copy");
+    check(hipStreamSynchronize(cc.stream), "hipStreamSynchronize");
+    check(hipEventElapsedTime(&cc.last_ms, cc.e0, cc.e1), "hipEventElapsedTime");
+    return PCFD_OK;
+  });
+}
+
 int pcfd_chem_wdot(pcfd_chem* c, int n, const double* rhoi, const double* T, double* wdot) {
   return guarded_chem(c, [&](pcfd_chem& cc) -> int {
     if (n < 0 || (n > 0 && (!rhoi || !T || !wdot))) {

@@ -289,7 +289,8 @@ class ChemDesc(C.Structure):
 
 
 CHEM_SYMBOLS = ["pcfd_chem_create", "pcfd_chem_destroy", "pcfd_chem_last_error", "pcfd_chem_wdot",
-                "pcfd_chem_source", "pcfd_chem_source_device", "pcfd_chem_last_kernel_ms"]
+                "pcfd_chem_source", "pcfd_chem_source_device", "pcfd_chem_last_kernel_ms", "pcfd_chem_fr_aux",
+                "pcfd_chem_t_from_p"]
 
 
 def _chem_lib():
@@ -304,6 +305,8 @@ def _chem_lib():
         L.pcfd_chem_source.argtypes = [vp, C.c_int, dp, dp, dp, dp]
         L.pcfd_chem_source_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.pcfd_chem_last_kernel_ms.argtypes = [vp]
+        L.pcfd_chem_fr_aux.argtypes = [vp, C.c_int, dp, C.c_double, C.c_double, dp]
+        L.pcfd_chem_t_from_p.argtypes = [vp, C.c_int, dp, dp, dp, C.c_double, dp, C.POINTER(C.c_int)]
         L.pcfd_chem_last_kernel_ms.restype = C.c_double
         L._chem_ready = True
     return L
@@ -352,6 +355,25 @@ class ChemContext:
         w = np.zeros_like(rhoi)
         self._chk(_chem_lib().pcfd_chem_wdot(self.h, len(T), _d(rhoi), _d(T), _d(w)), "pcfd_chem_wdot")
         return w
+
+    def fr_aux(self, q, ref_pressure, ref_velocity):
+        """[n, 2+2ns]: p, rho, cv_1..cv_ns, conc_1..conc_ns per FR state."""
+        q = _f64(q)
+        n = q.shape[0]
+        a = np.zeros((n, 2 + 2 * (q.shape[1] - 4)))
+        self._chk(_chem_lib().pcfd_chem_fr_aux(self.h, n, _d(q), float(ref_pressure), float(ref_velocity), _d(a)),
+                  "pcfd_chem_fr_aux")
+        return a
+
+    def t_from_p(self, rhoi, p, Tinit, ref_pressure):
+        """(T [n], Newton steps [n]) for n states."""
+        rhoi, p, Tinit = _f64(rhoi), _f64(p), _f64(Tinit)
+        n = len(p)
+        T = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        self._chk(_chem_lib().pcfd_chem_t_from_p(self.h, n, _d(rhoi), _d(p), _d(Tinit), float(ref_pressure), _d(T),
+                                                 it.ctypes.data_as(C.POINTER(C.c_int))), "pcfd_chem_t_from_p")
+        return T, it
 
     def source(self, q, vol, jacobian=True):
         q = _f64(q)

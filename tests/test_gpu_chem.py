@@ -59,3 +59,33 @@ def test_source_and_jacobian_match_oracle():
         tol = 1e-6 * jc + 1e-14 * sc / 1e-8
         assert np.all(np.abs(Jg[i] - Jo) <= tol), (i, np.abs(Jg[i] - Jo).max(), jc, sc)
     ctx.close()
+
+
+def test_fr_aux_and_t_from_p_match_oracle():
+    from proteuscfd_amd import gpu
+    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, [0, 0.767, 0.233, 0, 0])
+    m = MODEL.with_reference(rho_ref, t_ref, T_ref)
+    ctx = gpu.ChemContext(m)
+    rhoi, T = states(2000, 7)
+    rng = np.random.default_rng(8)
+    q = np.concatenate([rhoi / rho_ref, rng.uniform(-1, 1, (len(T), 3)), (T / T_ref)[:, None]], axis=1)
+    pref, vref = 101325.0, 347.0
+    print(f"[chem] FR aux / T from p n={len(T)}", flush=True)
+    ag = ctx.fr_aux(q, pref, vref)
+    oc = O.Chem(m)
+    ao = np.array([oc.fr_aux(q[i], pref, vref) for i in range(len(T))])
+    # FP64 bar as for the other phases: the NASA-7 Horner sums cancel at
+    # high temperature, so fused multiply-adds move the last bits (1.9e-13
+    # seen on one cv of 24000 entries)
+    np.testing.assert_allclose(ag, ao, rtol=1e-12, atol=0)
+    T0 = q[:, -1] * rng.uniform(0.5, 2.0, len(T))
+    Tg, ig = ctx.t_from_p(q[:, :5], ag[:, 0], T0, pref)
+    # the step count is not compared: the stop test (|dT| / T_ref < 1e-15)
+    # sits at the rounding level of dT, so it can trip a step apart
+    for i in range(len(T)):
+        To, io = oc.t_from_p(q[i, :5], ag[i, 0], T0[i], pref)
+        assert abs(Tg[i] - To) <= 1e-13 * To
+    assert np.all((ig >= 1) & (ig <= 28))
+    with pytest.raises(RuntimeError):
+        ctx.fr_aux(q, -1.0, vref)
+    ctx.close()

@@ -111,3 +111,40 @@ def test_source_term_scaling_and_jacobian():
         qm[i] -= h
         fd = (c.source(qp, vol) - c.source(qm, vol)) / (2 * h)
         np.testing.assert_allclose(A[:5, i], fd[:5], rtol=2e-3, atol=1e-6 * np.abs(fd).max())
+
+
+def test_fr_aux_closed_forms_and_t_from_p_round_trip():
+    """ComputeAuxiliaryVariables of the FR set against the ideal-gas law in
+    mixture form (p = rho R_u sum(Y_i / MW_i) T), Mayer's relation on the
+    NASA-7 Cp, MassToMole, and a diatomic gamma of 7/5 at room temperature;
+    NewtonFindTGivenP inverts the pressure back to the temperature.  Its
+    stop test |dT| / T_ref < 1e-15 lies below the rounding of dT at high
+    temperatures, where the reference's loop runs all 28 steps (and warns);
+    the temperature is then as exact as at the low ones."""
+    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, [0, 0.767, 0.233, 0, 0])
+    m = MODEL.with_reference(rho_ref, t_ref, T_ref)
+    oc = O.Chem(m)
+    mw = np.asarray(m.mw, dtype=np.float64)
+    pref, vref, Ru = 101325.0, 347.0, 8.31447215
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        Y = rng.dirichlet(np.ones(5))
+        rho = 10.0 ** rng.uniform(-3.0, 0.5)
+        T = rng.uniform(250.0, 9000.0)
+        rhoi = rho * Y / rho_ref
+        Q = np.concatenate([rhoi, rng.uniform(-1.0, 1.0, 3), [T / T_ref]])
+        a = oc.fr_aux(Q, pref, vref)
+        assert abs(a[0] - rho * Ru * np.sum(Y / mw) * T / pref) <= 1e-13 * a[0]
+        assert abs(a[1] - rhoi.sum()) <= 1e-15 * a[1]
+        cp = np.array([oc.cp(i, T) for i in range(5)])
+        np.testing.assert_allclose(a[2:7] * vref ** 2 / T_ref, cp - Ru / mw, rtol=1e-13)
+        np.testing.assert_allclose(a[7:12], rhoi / mw / 1000.0, rtol=1e-15)
+        Tn, it = oc.t_from_p(rhoi, a[0], 1.7 * T / T_ref, pref)
+        assert abs(Tn * T_ref - T) <= 1e-12 * T
+        assert 1 <= it <= 28
+        if T < 600.0:
+            assert it < 28
+    a = oc.fr_aux(np.concatenate([np.full(5, 0.2), [0.0, 0.0, 0.0], [300.0 / T_ref]]), pref, vref)
+    for i in (list(m.symbols).index("N2"), list(m.symbols).index("O2")):
+        cv = a[2 + i] * vref ** 2 / T_ref
+        assert 1.39 < (cv + Ru / mw[i]) / cv < 1.41
