@@ -233,27 +233,42 @@ __device__ __forceinline__ double p_of(const ChemDev& m, const double* rhoiDim, 
   return P;
 }
 
-// ComputeAuxiliaryVariables (ucs/compressibleFR.tcc:754-814), one thread per state
-__global__ void __launch_bounds__(kChemBlock) k_fr_aux(const ChemDev* __restrict__ mp, int n, const double* __restrict__ q, double pref,
-                                                      double s_ref, double* __restrict__ aux) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
+// ComputeAuxiliaryVariables (ucs/compressibleFR.tcc:754-814), one thread per
+// state.  The states are AoS (ns + 4 doubles each, as the reference stores
+// them): a block stages its slice of q and of aux through LDS so that the
+// HBM reads and writes are contiguous across the block.
+__global__ void __launch_bounds__(kChemBlock) k_fr_aux(const ChemDev* __restrict__ mp, int n,
+                                                      const double* __restrict__ q, double pref, double s_ref,
+                                                      double* __restrict__ aux) {
+  __shared__ double sh[kChemBlock * (2 + 2 * PCFD_CHEM_MAXS)];
   const ChemDev& m = *mp;
   const int ns = m.ns, neqn = ns + 4, naux = 2 + 2 * ns;
-  const double* Q = q + (long long)t * neqn;
-  double* X = aux + (long long)t * naux;
-  double rho = 0.0, rd[PCFD_CHEM_MAXS];
-  for (int i = 0; i < ns; ++i) {
-    rho += Q[i];
-    rd[i] = Q[i] * m.ref_density;
+  const long long b0 = (long long)blockIdx.x * blockDim.x;
+  const int nb = (int)(n - b0 < (long long)blockDim.x ? n - b0 : (long long)blockDim.x);
+  for (int k = threadIdx.x; k < nb * neqn; k += blockDim.x) sh[k] = q[b0 * neqn + k];
+  __syncthreads();
+  const int t = threadIdx.x;
+  double X[2 + 2 * PCFD_CHEM_MAXS];
+  if (t < nb) {
+    const double* Q = sh + t * neqn;
+    double rho = 0.0, rd[PCFD_CHEM_MAXS];
+    for (int i = 0; i < ns; ++i) {
+      rho += Q[i];
+      rd[i] = Q[i] * m.ref_density;
+    }
+    const double TDim = Q[ns + 3] * m.ref_temperature;
+    X[0] = p_of(m, rd, TDim) / pref;
+    X[1] = rho;
+    for (int i = 0; i < ns; ++i) {
+      X[2 + i] = (cp_of(m, i, TDim) - kUnivR / m.mw[i]) / s_ref;
+      X[2 + ns + i] = Q[i] / m.mw[i] / 1000.0;
+    }
   }
-  const double TDim = Q[ns + 3] * m.ref_temperature;
-  X[0] = p_of(m, rd, TDim) / pref;
-  X[1] = rho;
-  for (int i = 0; i < ns; ++i) {
-    X[2 + i] = (cp_of(m, i, TDim) - kUnivR / m.mw[i]) / s_ref;
-    X[2 + ns + i] = Q[i] / m.mw[i] / 1000.0;
-  }
+  __syncthreads();
+  if (t < nb)
+    for (int k = 0; k < naux; ++k) sh[t * naux + k] = X[k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < nb * naux; k += blockDim.x) aux[b0 * naux + k] = sh[k];
 }
 
 // NewtonFindTGivenP (ucs/compressibleFR.tcc:2342-2378), one thread per state
