@@ -25,7 +25,7 @@ HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_visc.hip", "kernels_jac.hip",
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics", "-fno-gpu-rdc",
+    "-fno-gpu-rdc",
 ]
 
 

@@ -48,126 +48,11 @@ PCFD_DEV void phys_flux(const double* Q, const double* a, double gm1, double* f)
 PCFD_DEV double dmax(double x, double y) { return x > y ? x : y; }
 PCFD_DEV double dmin(double x, double y) { return x < y ? x : y; }
 
-// Roe flux with the Harten-Hyman type-2 entropy fix, RoeVariables and the
-// explicit right/left eigenvector matrices applied as two 5x5 mat-vecs
-// (ucs/compressible.tcc:92-230, 533-684), NaN components zeroed
-// (ucs/eqnset.tcc:74-88).  F is the flux through a[3]*a[0..2] from L to R.
-PCFD_DEV void roe_flux(const double* QL, const double* QR, const double* a, double gamma, int sym2d,
-                       double* F) {
-  const double gm1 = gamma - 1.0;
-  // RoeVariables
-  double rhoL = QL[0], rhoR = QR[0];
-  double uL = QL[1] / QL[0], uR = QR[1] / QR[0];
-  double vL = QL[2] / QL[0], vR = QR[2] / QR[0];
-  double wL = QL[3] / QL[0], wR = QR[3] / QR[0];
-  double v2L = uL * uL + vL * vL + wL * wL;
-  double v2R = uR * uR + vR * vR + wR * wR;
-  double PL = gm1 * (QL[4] - 0.5 * rhoL * v2L);
-  double PR = gm1 * (QR[4] - 0.5 * rhoR * v2R);
-  double hL = (QL[4] + PL) / rhoL;
-  double hR = (QR[4] + PR) / rhoR;
-  double rr = sqrt(rhoL * rhoR);
-  double sigma = rr / (rhoL + rr);
-  double ur = uL + sigma * (uR - uL);
-  double vr = vL + sigma * (vR - vL);
-  double wr = wL + sigma * (wR - wL);
-  double hr = hL + sigma * (hR - hL);
-  double v2hr = 0.5 * (ur * ur + vr * vr + wr * wr);
-  double Qr[5] = {rr, rr * ur, rr * vr, rr * wr, rr / gamma * (hr + gm1 * v2hr)};
-  // Eigensystem on the Roe state
-  const double nx = a[0], ny = a[1], nz = a[2];
-  double rho = Qr[0];
-  double u = Qr[1] / rho, v = Qr[2] / rho, w = Qr[3] / rho;
-  double thetaf = u * nx + v * ny + w * nz;
-  double theta = thetaf + 0.0;
-  double v2h = 0.5 * (u * u + v * v + w * w);
-  double P = gm1 * (Qr[4] - rho * v2h);
-  double c2 = gamma * P / rho;
-  double c = sqrt(c2);
-  double e0 = theta, e3 = theta + c, e4 = theta - c;
-  // entropy fix (uses the L/R states' own pressures, as the reference does)
-  double thetaL = uL * nx + vL * ny + wL * nz + 0.0;
-  double thetaR = uR * nx + vR * ny + wR * nz + 0.0;
-  double cR = sqrt(gamma * PR / rhoR);
-  double cL = sqrt(gamma * PL / rhoL);
-  double eps;
-  eps = dmax(0.0, dmax(e0 - thetaL, thetaR - e0));
-  e0 = (fabs(e0) < eps) ? 0.5 * (e0 * e0 / eps + eps) : fabs(e0);
-  eps = dmax(0.0, dmax(e3 - (thetaL + cL), (thetaR + cR) - e3));
-  e3 = (fabs(e3) < eps) ? 0.5 * (e3 * e3 / eps + eps) : fabs(e3);
-  eps = dmax(0.0, dmax(e4 - (thetaL - cL), (thetaR - cR) - e4));
-  e4 = (fabs(e4) < eps) ? 0.5 * (e4 * e4 / eps + eps) : fabs(e4);
-  double dQ[5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) dQ[i] = QR[i] - QL[i];
-  // dv = Tinv * dQ, row by row (MatVecMult order, ucs/matrix.h:63-74)
-  double Ti[5];
-  double dv[5];
-  Ti[0] = nx - nz * v / rho + ny * w / rho - nx / c2 * v2h * gm1;
-  Ti[1] = nx / c2 * u * gm1;
-  Ti[2] = nz / rho + nx / c2 * v * gm1;
-  Ti[3] = -ny / rho + nx / c2 * w * gm1;
-  Ti[4] = -nx / c2 * gm1;
-  dv[0] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
-  Ti[0] = ny + nz * u / rho - nx * w / rho - ny / c2 * v2h * gm1;
-  Ti[1] = -nz / rho + ny / c2 * u * gm1;
-  Ti[2] = ny / c2 * v * gm1;
-  Ti[3] = nx / rho + ny / c2 * w * gm1;
-  Ti[4] = -ny / c2 * gm1;
-  dv[1] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
-  Ti[0] = nz - ny * u / rho + nx * v / rho - nz / c2 * v2h * gm1;
-  Ti[1] = ny / rho + nz / c2 * u * gm1;
-  Ti[2] = -nx / rho + nz / c2 * v * gm1;
-  Ti[3] = nz / c2 * w * gm1;
-  Ti[4] = -nz / c2 * gm1;
-  dv[2] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
-  Ti[0] = -0.5 / rho * (thetaf - gm1 * v2h / c);
-  Ti[1] = 0.5 / rho * (nx - gm1 * u / c);
-  Ti[2] = 0.5 / rho * (ny - gm1 * v / c);
-  Ti[3] = 0.5 / rho * (nz - gm1 * w / c);
-  Ti[4] = 0.5 / rho * (gm1 / c);
-  dv[3] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
-  Ti[0] = 0.5 / rho * (thetaf + gm1 * v2h / c);
-  Ti[1] = -0.5 / rho * (nx + gm1 * u / c);
-  Ti[2] = -0.5 / rho * (ny + gm1 * v / c);
-  Ti[3] = -0.5 / rho * (nz + gm1 * w / c);
-  Ti[4] = +0.5 / rho * (gm1 / c);
-  dv[4] = Ti[0] * dQ[0] + Ti[1] * dQ[1] + Ti[2] * dQ[2] + Ti[3] * dQ[3] + Ti[4] * dQ[4];
-  dv[0] *= fabs(e0);
-  dv[1] *= fabs(e0);
-  dv[2] *= fabs(e0);
-  dv[3] *= fabs(e3);
-  dv[4] *= fabs(e4);
-  // dr = T * dv, row by row
-  double dr[5];
-  dr[0] = nx * dv[0] + ny * dv[1] + nz * dv[2] + rho / c * dv[3] + rho / c * dv[4];
-  dr[1] = u * nx * dv[0] + (u * ny - rho * nz) * dv[1] + (u * nz + rho * ny) * dv[2] +
-          rho * (u / c + nx) * dv[3] + rho * (u / c - nx) * dv[4];
-  dr[2] = (v * nx + rho * nz) * dv[0] + v * ny * dv[1] + (v * nz - rho * nx) * dv[2] +
-          rho * (v / c + ny) * dv[3] + rho * (v / c - ny) * dv[4];
-  dr[3] = (w * nx - rho * ny) * dv[0] + (w * ny + rho * nx) * dv[1] + w * nz * dv[2] +
-          rho * (w / c + nz) * dv[3] + rho * (w / c - nz) * dv[4];
-  dr[4] = (v2h * nx + rho * (v * nz - w * ny)) * dv[0] + (v2h * ny + rho * (w * nx - u * nz)) * dv[1] +
-          (v2h * nz + rho * (u * ny - v * nx)) * dv[2] + rho * (v2h / c + thetaf + c / gm1) * dv[3] +
-          rho * (v2h / c - thetaf + c / gm1) * dv[4];
-  double fL[5], fR[5];
-  phys_flux(QL, a, gm1, fL);
-  phys_flux(QR, a, gm1, fR);
-  const double area = a[3];
-#pragma unroll
-  for (int i = 0; i < 5; ++i) F[i] = 0.5 * area * (fL[i] + fR[i] - dr[i]);
-  if (sym2d == 1) F[1] = 0.0;
-  else if (sym2d == 2) F[2] = 0.0;
-  else if (sym2d == 3) F[3] = 0.0;
-#pragma unroll
-  for (int i = 0; i < 5; ++i)
-    if (__builtin_isnan(F[i])) F[i] = 0.0;
-}
-
 // ---------------------------------------------------------------------------
 // Roe flux in wave-strength form.  Same Roe average, eigenvalues,
 // Harten-Hyman type-2 entropy fix (ucs/compressible.tcc:92-230) and NaN
-// zeroing as roe_flux above, but the dissipation T*|L|*Tinv*dQ is assembled
+// zeroing (ucs/eqnset.tcc:74-88) as the reference's RoeFlux, whose dissipation
+// T*|L|*Tinv*dQ is two explicit 5x5 mat-vecs; here it is assembled
 // from the characteristic wave strengths of the primitive jumps (the Roe
 // average makes the two forms identical; results differ by rounding only)
 // and every division by rho, c, c^2 is a multiplication by a reciprocal.

@@ -349,9 +349,22 @@ class ChemContext:
         except Exception:
             pass
 
+    def _check_states(self, name, a, ncols, n=None):
+        """The C side strides host arrays by the model's species count: a mismatched
+        shape would read or write past the buffers, so it is refused here."""
+        if a.ndim != 2 or a.shape[1] != ncols or (n is not None and a.shape[0] != n):
+            want = f"({n if n is not None else 'n'}, {ncols})"
+            raise ValueError(f"{name} has shape {a.shape}; this model ({self.model.ns} species) needs {want}")
+
+    def _check_vector(self, name, a, n):
+        if a.ndim != 1 or a.shape[0] != n:
+            raise ValueError(f"{name} has shape {a.shape}; expected ({n},)")
+
     def wdot(self, rhoi, T):
         rhoi = _f64(rhoi)
         T = _f64(T)
+        self._check_vector("T", T, len(T))
+        self._check_states("rhoi", rhoi, self.model.ns, len(T))
         w = np.zeros_like(rhoi)
         self._chk(_chem_lib().pcfd_chem_wdot(self.h, len(T), _d(rhoi), _d(T), _d(w)), "pcfd_chem_wdot")
         return w
@@ -360,7 +373,8 @@ class ChemContext:
         """[n, 2+2ns]: p, rho, cv_1..cv_ns, conc_1..conc_ns per FR state."""
         q = _f64(q)
         n = q.shape[0]
-        a = np.zeros((n, 2 + 2 * (q.shape[1] - 4)))
+        self._check_states("q", q, self.model.ns + 4, n)
+        a = np.zeros((n, 2 + 2 * self.model.ns))
         self._chk(_chem_lib().pcfd_chem_fr_aux(self.h, n, _d(q), float(ref_pressure), float(ref_velocity), _d(a)),
                   "pcfd_chem_fr_aux")
         return a
@@ -369,6 +383,9 @@ class ChemContext:
         """(T [n], Newton steps [n]) for n states."""
         rhoi, p, Tinit = _f64(rhoi), _f64(p), _f64(Tinit)
         n = len(p)
+        self._check_vector("p", p, n)
+        self._check_vector("Tinit", Tinit, n)
+        self._check_states("rhoi", rhoi, self.model.ns, n)
         T = np.zeros(n)
         it = np.zeros(n, dtype=np.int32)
         self._chk(_chem_lib().pcfd_chem_t_from_p(self.h, n, _d(rhoi), _d(p), _d(Tinit), float(ref_pressure), _d(T),
@@ -378,7 +395,10 @@ class ChemContext:
     def source(self, q, vol, jacobian=True):
         q = _f64(q)
         vol = _f64(vol)
-        n, neqn = q.shape
+        neqn = self.model.ns + 4
+        self._check_states("q", q, neqn)
+        n = q.shape[0]
+        self._check_vector("vol", vol, n)
         s = np.zeros((n, neqn))
         J = np.zeros((n, neqn, neqn)) if jacobian else None
         self._chk(_chem_lib().pcfd_chem_source(self.h, n, _d(q), _d(vol), _d(s), _d(J) if jacobian else None),

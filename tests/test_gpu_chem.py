@@ -85,7 +85,19 @@ def test_fr_aux_and_t_from_p_match_oracle():
     for i in range(len(T)):
         To, io = oc.t_from_p(q[i, :5], ag[i, 0], T0[i], pref)
         assert abs(Tg[i] - To) <= 1e-13 * To
+    # the pressure was formed from the true temperature q[:, -1]: Newton must land on it
+    np.testing.assert_allclose(Tg, q[:, -1], rtol=1e-12, atol=0)
     assert np.all((ig >= 1) & (ig <= 28))
+    # 28 means "did not stop"; below 600 K the stop test trips well before that
+    cold = q[:, -1] * T_ref < 600.0
+    assert cold.sum() > 10
+    assert np.all(ig[cold] < 28), ig[cold].max()
+    with pytest.raises(ValueError):
+        ctx.fr_aux(q[:, :-1], pref, vref)
+    with pytest.raises(ValueError):
+        ctx.t_from_p(q[:, :4], ag[:, 0], T0, pref)
+    with pytest.raises(ValueError):
+        ctx.t_from_p(q[:, :5], ag[:, 0], T0[:-1], pref)
     with pytest.raises(RuntimeError):
         ctx.fr_aux(q, -1.0, vref)
     ctx.close()
