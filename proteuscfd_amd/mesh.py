@@ -76,6 +76,35 @@ def lib():
     return _lib
 
 
+_h5lib = None
+
+
+def h5lib():
+    """libpcfd_h5.so (partitioned HDF5 mesh files).  HDF5 itself is loaded first by
+    absolute path, so the library's dependency on it resolves without an rpath."""
+    global _h5lib
+    if _h5lib is None:
+        from . import build as B
+        path = os.path.join(HERE, "libpcfd_h5.so")
+        hdf5 = B.hdf5_library()
+        if hdf5 is None or not os.path.exists(path):
+            raise RuntimeError("libpcfd_h5.so is not built (HDF5 not found); run __graft_entry__.build()")
+        C.CDLL(hdf5, mode=C.RTLD_GLOBAL)
+        L = C.CDLL(path)
+        ipt, dpt = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.pcfd_mesh_write_h5.argtypes = [C.c_char_p, C.c_int, ipt, C.c_int, dpt, C.c_int, ipt, ipt, ipt, C.c_int,
+                                         ipt, ipt, ipt, ipt, C.c_int]
+        L.pcfd_mesh_write_h5.restype = C.c_int
+        L.pcfd_mesh_read_h5.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]
+        L.pcfd_mesh_read_h5.restype = C.c_int
+        L.pcfd_mesh_get.argtypes = [C.c_void_p]
+        L.pcfd_mesh_get.restype = C.POINTER(MeshData)
+        L.pcfd_mesh_free.argtypes = [C.c_void_p]
+        L.pcfd_mesh_error.restype = C.c_char_p
+        _h5lib = L
+    return _h5lib
+
+
 def default_spec(kind: int, **kw) -> MeshSpec:
     s = MeshSpec()
     lib().pcfd_mesh_default_spec(kind, C.byref(s))
@@ -93,15 +122,16 @@ def _arr(ptr, n, dtype):
 class Mesh:
     """Numpy view of a generated mesh (all arrays copied, C object freed)."""
 
-    def __init__(self, spec: MeshSpec | None, _handle=None):
+    def __init__(self, spec: MeshSpec | None, _handle=None, _lib=None):
+        L = _lib or lib()
         if _handle is None:
             h = C.c_void_p()
-            if lib().pcfd_mesh_build(C.byref(spec), C.byref(h)) != 0:
-                raise RuntimeError(lib().pcfd_mesh_error().decode())
+            if L.pcfd_mesh_build(C.byref(spec), C.byref(h)) != 0:
+                raise RuntimeError(L.pcfd_mesh_error().decode())
         else:
             h = _handle
         try:
-            d = lib().pcfd_mesh_get(h).contents
+            d = L.pcfd_mesh_get(h).contents
             self.spec = spec
             self.nnode, self.nelem, self.nedge = d.nnode, d.nelem, d.nedge
             self.nbface, self.nbedge, self.nfactag = d.nbface, d.nbedge, d.nfactag
@@ -124,7 +154,7 @@ class Mesh:
             self.total_volume = d.total_volume
             self.elem_volume = d.elem_volume
         finally:
-            lib().pcfd_mesh_free(h)
+            L.pcfd_mesh_free(h)
 
     @classmethod
     def build(cls, kind: int, **kw) -> "Mesh":
@@ -159,6 +189,37 @@ class Mesh:
         if rc != 0:
             raise RuntimeError(lib().pcfd_mesh_error().decode())
         return cls(None, _handle=h)
+
+    @classmethod
+    def read_h5(cls, casename: str, rcm=False) -> "Mesh":
+        """Read the partition files <casename>.<rank>.h5 (ucs.x's format) and join
+        the partitions into one mesh."""
+        L = h5lib()
+        h = C.c_void_p()
+        if L.pcfd_mesh_read_h5(os.fsencode(casename), int(rcm), C.byref(h)) != 0:
+            raise RuntimeError(L.pcfd_mesh_error().decode())
+        return cls(None, _handle=h, _lib=L)
+
+    def write_h5(self, casename: str, np_parts: int = 1, part=None, reordered=False) -> None:
+        """Write the mesh as np_parts partition files <casename>.<rank>.h5;
+        part[nnode] gives each node's rank (required when np_parts > 1)."""
+        L = h5lib()
+        ip = lambda a: np.ascontiguousarray(a, dtype=np.int32).ctypes.data_as(C.POINTER(C.c_int))
+        keep = [np.ascontiguousarray(a, dtype=np.int32) for a in
+                (self.elem_type, self.elem_ptr, self.elem_nodes, self.bface_type, self.bface_ptr, self.bface_nodes,
+                 self.bface_tag)]
+        pp = None
+        if part is not None:
+            part = np.ascontiguousarray(part, dtype=np.int32)
+            if part.shape != (self.nnode,):
+                raise ValueError(f"part has shape {part.shape}; expected ({self.nnode},)")
+            pp = ip(part)
+        xyz = np.ascontiguousarray(self.xyz, dtype=np.float64)
+        rc = L.pcfd_mesh_write_h5(os.fsencode(casename), int(np_parts), pp, self.nnode,
+                                  xyz.ctypes.data_as(C.POINTER(C.c_double)), self.nelem, *[ip(a) for a in keep[:3]],
+                                  self.nbface, *[ip(a) for a in keep[3:]], int(bool(reordered)))
+        if rc != 0:
+            raise RuntimeError(L.pcfd_mesh_error().decode())
 
     def colour(self) -> tuple[np.ndarray, int]:
         col = np.empty(self.nnode, dtype=np.int32)
