@@ -119,22 +119,6 @@ int  pcfd_mesh_build_from(int nnode, const double* xyz, int nelem, const int* el
 #define PCFD_MESH_FMT_FIELDVIEW 3
 #define PCFD_MESH_FMT_SU2       4
 int  pcfd_mesh_read(const char* path, int format, int rcm, pcfd_mesh** out);
-/* Partitioned HDF5 mesh files <casename>.<rank>.h5, the format ucs.x reads
- * (in libpcfd_h5.so, built where HDF5 is installed; the other libraries do
- * not need HDF5).
- * pcfd_mesh_write_h5 replaces udecomp.x's writer (ucs/decomp.cpp:281-535) and
- * Mesh<Type>::WriteParallelMesh (ucs/mesh.tcc:2750-2818): np files from the
- * element and boundary-face lists of a mesh (pcfd_mesh_data's elem_* and
- * bface_* arrays) and a partition vector part[nnode] (NULL when np == 1).
- * pcfd_mesh_read_h5 replaces Mesh<Type>::ReadPartedMesh
- * (ucs/mesh.tcc:2820-2990): it reads every rank's file, joins the partitions
- * into one mesh (rank r's local node j is node offset[r] + j; elements stored
- * by several ranks are kept once) and builds it as pcfd_mesh_build_from. */
-int  pcfd_mesh_write_h5(const char* casename, int np, const int* part, int nnode, const double* xyz, int nelem,
-                        const int* elem_type, const int* elem_ptr, const int* elem_nodes, int nbface,
-                        const int* bface_type, const int* bface_ptr, const int* bface_nodes, const int* bface_tag,
-                        int reordered);
-int  pcfd_mesh_read_h5(const char* casename, int rcm, pcfd_mesh** out);
 const pcfd_mesh_data* pcfd_mesh_get(const pcfd_mesh* m);
 void pcfd_mesh_free(pcfd_mesh* m);
 const char* pcfd_mesh_error(void);

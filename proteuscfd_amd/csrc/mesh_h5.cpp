@@ -40,7 +40,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pcfd_mesh.h"
+#include "../../include/pcfd_mesh_h5.h"
 
 void pcfd_mesh_set_error(const std::string& msg);  // mesh.cpp
 

@@ -17,11 +17,18 @@ def declared(header):
 
 
 @pytest.mark.parametrize("lib,headers", [("libpcfd.so", ["pcfd.h", "pcfd_mesh.h", "pcfd_chem.h"]),
-                                         ("libpcfd_mesh.so", ["pcfd_mesh.h"])])
+                                         ("libpcfd_mesh.so", ["pcfd_mesh.h"]),
+                                         ("libpcfd_h5.so", ["pcfd_mesh.h", "pcfd_mesh_h5.h"])])
 def test_library_exports_declared_symbols(lib, headers):
     path = os.path.join(ROOT, "proteuscfd_amd", lib)
-    assert os.path.exists(path), f"{lib} not built"
-    L = ctypes.CDLL(path)
+    if lib == "libpcfd_h5.so":
+        from proteuscfd_amd import build, mesh
+        if build.hdf5_library() is None:
+            pytest.skip("HDF5 is not installed: libpcfd_h5.so is not built")
+        L = mesh.h5lib()
+    else:
+        assert os.path.exists(path), f"{lib} not built"
+        L = ctypes.CDLL(path)
     names = [n for h in headers for n in declared(h)]
     assert len(names) > 5
     missing = [n for n in names if not hasattr(L, n)]
