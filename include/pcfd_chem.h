@@ -135,6 +135,20 @@ typedef struct {
 int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n,
                                     const double* rhoi, const double* T, double* wdot, int* nfallback, double* ms);
 
+/* The same as pcfd_chem_surrogate_wdot_device without allocating device
+ * memory, for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  The fallback list, per-pass counts and their offsets live
+ * in a fixed library-owned device array (one per GPU, 17.3 MB, held until the
+ * process ends) that limits a call to PCFD_SURROGATE_NOALLOC_MAX_CELLS cells;
+ * larger n is refused with PCFD_ERR_ARG.  Every call on one GPU shares that
+ * array: calls that overlap on one GPU (on different mlp objects or streams)
+ * race and must be serialised by the caller. */
+#define PCFD_SURROGATE_NOALLOC_MAX_CELLS (1 << 22)
+int pcfd_chem_surrogate_wdot_device_noalloc(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n,
+                                            const double* rhoi, const double* T, double* wdot, int* nfallback,
+                                            double* ms);
+
 /* the same on host arrays (copied in and out; *ms is device time only) */
 int pcfd_chem_surrogate_wdot(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n, const double* rhoi,
                              const double* T, double* wdot, int* nfallback, double* ms);

@@ -5,6 +5,7 @@
 
 #include "common.h"
 #include "physics.h"
+#include "scan.h"
 
 namespace pcfd {
 
@@ -61,6 +62,58 @@ __global__ void __launch_bounds__(kBlock) k_explicit(int nnode, int ntot, double
   apply_dq(dq, Q, gamma);
 #pragma unroll
   for (int k = 0; k < 5; ++k) q[k * ntot + i] = Q[k];
+}
+
+// Exclusive prefix sum of per-pass counts (one workgroup of 1024, fixed
+// order): off[p] = sum of cnt[0..p-1], off[npass] = total.  Used to compact
+// lists in order without atomics (the surrogate's fallback cells, the
+// pressure clip's candidate edges).
+// Counts are read in coalesced tiles of 1024, up to kScanTiles tiles at a
+// time so their loads are in flight together; each tile is scanned with
+// wavefront shuffles plus one pass over the 16 wavefront totals.
+constexpr int kScanBlock = 1024;
+constexpr int kScanTiles = 16;
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_counts(int npass, const int* __restrict__ cnt,
+                                                                int* __restrict__ off) {
+  __shared__ int wsum[kScanBlock / 64];
+  __shared__ int carry_sh;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) carry_sh = 0;
+  __syncthreads();
+  for (int base = 0; base < npass; base += kScanBlock * kScanTiles) {
+    int v[kScanTiles];
+#pragma unroll
+    for (int k = 0; k < kScanTiles; ++k) {
+      const int p = base + k * kScanBlock + t;
+      v[k] = p < npass ? cnt[p] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanTiles; ++k) {
+      // inclusive scan within the wavefront
+      int x = v[k];
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+      }
+      if (lane == 63) wsum[wv] = x;
+      __syncthreads();
+      int before = carry_sh;
+      for (int u = 0; u < wv; ++u) before += wsum[u];
+      const int p = base + k * kScanBlock + t;
+      if (p < npass) off[p] = before + x - v[k];
+      __syncthreads();
+      if (t == kScanBlock - 1) carry_sh = before + x;
+      __syncthreads();
+    }
+  }
+  if (t == 0) off[npass] = carry_sh;
+}
+
+void launch_scan_counts(hipStream_t stream, int n, const int* cnt, int* off) {
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kScanBlock), 0, stream, n, cnt, off);
+  PCFD_HIP(hipGetLastError());
 }
 
 __global__ void k_zero_int(int* p) { *p = 0; }

@@ -21,12 +21,14 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/pcfd_chem.h"
 #include "chem_model.h"
+#include "scan.h"
 
 namespace pcfd {
 
@@ -475,6 +477,7 @@ struct pcfd_mlp {
   float* part2 = nullptr;  // group totals of the partials
   float* loss2 = nullptr;
   int grid = 0;  // blocks of the gradient kernel
+  int* scratch = nullptr;  // device address of the no-allocation scratch on this mlp's device
   long long adam_t = 0;
   float last_ms = 0.f;
   std::string err;
@@ -544,6 +547,10 @@ uint64_t mix(uint64_t z) {
 }
 }  // namespace
 
+namespace pcfd {
+int* scratch_on_current_device();
+}
+
 extern "C" {
 
 int pcfd_mlp_create(unsigned long long seed, pcfd_mlp** out) {
@@ -557,6 +564,7 @@ int pcfd_mlp_create(unsigned long long seed, pcfd_mlp** out) {
     mcheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
     mcheck(hipEventCreate(&c->e0), "hipEventCreate");
     mcheck(hipEventCreate(&c->e1), "hipEventCreate");
+    c->scratch = pcfd::scratch_on_current_device();
     c->grid = 256;
     const size_t P = sizeof(float) * kNparam;
     mcheck(hipMalloc((void**)&c->p, P), "hipMalloc");
@@ -843,51 +851,6 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
   }
 }
 
-// Exclusive prefix sum of the per-pass fallback counts (one workgroup of
-// 1024, fixed order): off[p] = sum of cnt[0..p-1], off[npass] = total.
-// Counts are read in coalesced tiles of 1024, up to kScanTiles tiles at a
-// time so their loads are in flight together; each tile is scanned with
-// wavefront shuffles plus one pass over the 16 wavefront totals.
-constexpr int kScanBlock = 1024;
-constexpr int kScanTiles = 16;
-
-__global__ void __launch_bounds__(kScanBlock) k_fallback_offsets(int npass, const int* __restrict__ cnt,
-                                                                int* __restrict__ off) {
-  __shared__ int wsum[kScanBlock / 64];
-  __shared__ int carry_sh;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  if (t == 0) carry_sh = 0;
-  __syncthreads();
-  for (int base = 0; base < npass; base += kScanBlock * kScanTiles) {
-    int v[kScanTiles];
-#pragma unroll
-    for (int k = 0; k < kScanTiles; ++k) {
-      const int p = base + k * kScanBlock + t;
-      v[k] = p < npass ? cnt[p] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kScanTiles; ++k) {
-      // inclusive scan within the wavefront
-      int x = v[k];
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d);
-        if (lane >= d) x += y;
-      }
-      if (lane == 63) wsum[wv] = x;
-      __syncthreads();
-      int before = carry_sh;
-      for (int u = 0; u < wv; ++u) before += wsum[u];
-      const int p = base + k * kScanBlock + t;
-      if (p < npass) off[p] = before + x - v[k];
-      __syncthreads();
-      if (t == kScanBlock - 1) carry_sh = before + x;
-      __syncthreads();
-    }
-  }
-  if (t == 0) off[npass] = carry_sh;
-}
-
 // Exact rates of the fallback cells: entry t of the compacted list lives in
 // the pass p with off[p] <= t < off[p+1] (binary search).
 __global__ void __launch_bounds__(kChemBlock) k_wdot_list(const ChemDev* __restrict__ md, int npass,
@@ -914,6 +877,42 @@ const ChemDev* chem_device_model(const pcfd_chem* c);
 int chem_nspecies(const pcfd_chem* c);
 const double* chem_mw(const pcfd_chem* c);
 
+// Scratch of pcfd_chem_surrogate_wdot_device_noalloc: the fallback list (64
+// slots per 64-cell pass), the per-pass counts and their offsets, for at most
+// PCFD_SURROGATE_NOALLOC_MAX_CELLS cells (17.3 MB per device, held until the
+// process ends).  Every entry a call reads is written earlier in the same
+// call (each pass writes its count, the scan writes every offset), so
+// nothing needs clearing.  Host code never passes the array's name to a
+// kernel: it is a host-side stand-in; the device address comes from
+// hipGetSymbolAddress, once per device.
+constexpr int kNoAllocPasses = PCFD_SURROGATE_NOALLOC_MAX_CELLS / 64;
+constexpr size_t kNoAllocInts = (size_t)kNoAllocPasses * 64 + kNoAllocPasses + (kNoAllocPasses + 1);
+static __device__ __align__(256) int scratch[kNoAllocInts];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+int* scratch_on_current_device() {
+  int dev = 0;
+  mcheck(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device id out of range for the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int* ptr = nullptr;
+    mcheck(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
+// Surrogate pass + fixed-order scan of the per-pass fallback counts + exact
+// rates of the fallback list, on the mlp's stream; lst/cnt/off hold at least
+// npass*64, npass and npass+1 ints.
+int surrogate_launch(pcfd_chem* chem, pcfd_mlp& m, const pcfd_surrogate_desc* d, int n, const double* rhoi,
+                     const double* T, double* wdot, int* lst, int* cnt, int* off, int* nfallback, double* ms);
+
 }  // namespace pcfd
 
 extern "C" {
@@ -932,52 +931,93 @@ int pcfd_chem_surrogate_wdot_device(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_s
     }
     if (nfallback) *nfallback = 0;
     if (n == 0) return PCFD_OK;
-    SurrogateDev sd{};
-    sd.lnT_mid = (float)d->lnT_mid;
-    sd.ilnT_half = (float)(1.0 / d->lnT_half);
-    sd.lnrho_mid = (float)d->lnrho_mid;
-    sd.ilnrho_half = (float)(1.0 / d->lnrho_half);
-    sd.y_mid = (float)d->y_mid;
-    sd.iy_scale = (float)(1.0 / d->y_scale);
-    sd.y_eps = (float)d->y_eps;
-    sd.s0 = d->rate_scale;
-    for (int k = 0; k < 5; ++k) {
-      sd.sigma[k] = d->sigma[k];
-      sd.imw[k] = 1.0 / chem_mw(chem)[k];
-    }
-    sd.T_min = d->T_min;
-    sd.T_max = d->T_max;
-    sd.rho_min = d->rho_min;
-    sd.rho_max = d->rho_max;
-    sd.tol = d->tol;
-    sd.floor = d->floor;
-    // each wavefront loads its weight fragments once and loops over 64-sample
-    // passes: about four blocks per CU
     const int npass = (n + 63) / 64;
-    const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
     // fallback list: 64 slots per pass, per-pass counts, their offsets
     DBuf dlst(sizeof(int) * (size_t)npass * 64), dcnt(sizeof(int) * (size_t)npass),
         doff(sizeof(int) * ((size_t)npass + 1));
-    mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
-    hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
-                       dlst.as<int>(), dcnt.as<int>());
-    mcheck(hipGetLastError(), "k_surrogate");
-    hipLaunchKernelGGL(k_fallback_offsets, dim3(1), dim3(kScanBlock), 0, m.stream, npass, dcnt.as<int>(),
-                       doff.as<int>());
-    mcheck(hipGetLastError(), "k_fallback_offsets");
-    hipLaunchKernelGGL(k_wdot_list, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, m.stream,
-                       chem_device_model(chem), npass, dlst.as<int>(), doff.as<int>(), rhoi, T, wdot);
-    mcheck(hipGetLastError(), "k_wdot_list");
-    mcheck(hipEventRecord(m.e1, m.stream), "hipEventRecord");
-    int nf = 0;
-    mcheck(hipMemcpyAsync(&nf, doff.as<int>() + npass, sizeof(int), hipMemcpyDeviceToHost, m.stream), "copy");
-    mcheck(hipStreamSynchronize(m.stream), "hipStreamSynchronize");
-    mcheck(hipEventElapsedTime(&m.last_ms, m.e0, m.e1), "hipEventElapsedTime");
-    if (nfallback) *nfallback = nf;
-    if (ms) *ms = m.last_ms;
-    return PCFD_OK;
+    return surrogate_launch(chem, m, d, n, rhoi, T, wdot, dlst.as<int>(), dcnt.as<int>(), doff.as<int>(), nfallback,
+                            ms);
   });
 }
+
+int pcfd_chem_surrogate_wdot_device_noalloc(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n,
+                                            const double* rhoi, const double* T, double* wdot, int* nfallback,
+                                            double* ms) {
+  return guarded_mlp(mlp, [&](pcfd_mlp& m) -> int {
+    if (!chem || !d || n < 0 || (n > 0 && (!rhoi || !T || !wdot))) {
+      m.err = "pcfd_chem_surrogate_wdot_device_noalloc: invalid arguments";
+      return PCFD_ERR_ARG;
+    }
+    if (n > PCFD_SURROGATE_NOALLOC_MAX_CELLS) {
+      m.err = "pcfd_chem_surrogate_wdot_device_noalloc: " + std::to_string(n) +
+              " cells exceed the fixed scratch of PCFD_SURROGATE_NOALLOC_MAX_CELLS = " +
+              std::to_string(PCFD_SURROGATE_NOALLOC_MAX_CELLS) + " (split the call)";
+      return PCFD_ERR_ARG;
+    }
+    if (chem_nspecies(chem) != kNout) {
+      m.err = "pcfd_chem_surrogate_wdot_device_noalloc: the surrogate is built for 5 species (N, N2, O2, NO, O)";
+      return PCFD_ERR_UNSUPPORTED;
+    }
+    if (nfallback) *nfallback = 0;
+    if (n == 0) return PCFD_OK;
+    int* const base = m.scratch;
+    int* const lst = base;
+    int* const cnt = base + (size_t)kNoAllocPasses * 64;
+    int* const off = cnt + kNoAllocPasses;
+    return surrogate_launch(chem, m, d, n, rhoi, T, wdot, lst, cnt, off, nfallback, ms);
+  });
+}
+
+}  // extern "C"
+
+namespace pcfd {
+
+int surrogate_launch(pcfd_chem* chem, pcfd_mlp& m, const pcfd_surrogate_desc* d, int n, const double* rhoi,
+                     const double* T, double* wdot, int* lst, int* cnt, int* off, int* nfallback, double* ms) {
+  SurrogateDev sd{};
+  sd.lnT_mid = (float)d->lnT_mid;
+  sd.ilnT_half = (float)(1.0 / d->lnT_half);
+  sd.lnrho_mid = (float)d->lnrho_mid;
+  sd.ilnrho_half = (float)(1.0 / d->lnrho_half);
+  sd.y_mid = (float)d->y_mid;
+  sd.iy_scale = (float)(1.0 / d->y_scale);
+  sd.y_eps = (float)d->y_eps;
+  sd.s0 = d->rate_scale;
+  for (int k = 0; k < 5; ++k) {
+    sd.sigma[k] = d->sigma[k];
+    sd.imw[k] = 1.0 / chem_mw(chem)[k];
+  }
+  sd.T_min = d->T_min;
+  sd.T_max = d->T_max;
+  sd.rho_min = d->rho_min;
+  sd.rho_max = d->rho_max;
+  sd.tol = d->tol;
+  sd.floor = d->floor;
+  // each wavefront loads its weight fragments once and loops over 64-sample
+  // passes: about four blocks per CU
+  const int npass = (n + 63) / 64;
+  const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
+  mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
+  hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
+                     lst, cnt);
+  mcheck(hipGetLastError(), "k_surrogate");
+  launch_scan_counts(m.stream, npass, cnt, off);
+  hipLaunchKernelGGL(k_wdot_list, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, m.stream,
+                     chem_device_model(chem), npass, lst, off, rhoi, T, wdot);
+  mcheck(hipGetLastError(), "k_wdot_list");
+  mcheck(hipEventRecord(m.e1, m.stream), "hipEventRecord");
+  int nf = 0;
+  mcheck(hipMemcpyAsync(&nf, off + npass, sizeof(int), hipMemcpyDeviceToHost, m.stream), "copy");
+  mcheck(hipStreamSynchronize(m.stream), "hipStreamSynchronize");
+  mcheck(hipEventElapsedTime(&m.last_ms, m.e0, m.e1), "hipEventElapsedTime");
+  if (nfallback) *nfallback = nf;
+  if (ms) *ms = m.last_ms;
+  return PCFD_OK;
+}
+
+}  // namespace pcfd
+
+extern "C" {
 
 int pcfd_chem_surrogate_wdot(pcfd_chem* chem, pcfd_mlp* mlp, const pcfd_surrogate_desc* d, int n,
                              const double* rhoi, const double* T, double* wdot, int* nfallback, double* ms) {

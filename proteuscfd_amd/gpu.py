@@ -279,6 +279,59 @@ class Context:
 
 
 # ---------------------------------------------------------------------------
+# device buffers for the *_device entry points, through the HIP runtime that
+# libpcfd.so itself links (no second runtime in the process)
+
+_hip = None
+
+
+def _hip_rt():
+    global _hip
+    if _hip is None:
+        lib()  # loads libamdhip64.so.7 as libpcfd.so's dependency
+        H = C.CDLL("libamdhip64.so.7")
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipDeviceSynchronize.argtypes = []
+        _hip = H
+    return _hip
+
+
+class DeviceArray:
+    """A device copy of a numpy array (hipMalloc'd; freed by close())."""
+
+    def __init__(self, a):
+        a = np.ascontiguousarray(a)
+        self.shape, self.dtype, self.nbytes = a.shape, a.dtype, a.nbytes
+        H = _hip_rt()
+        p = C.c_void_p()
+        if H.hipMalloc(C.byref(p), max(self.nbytes, 1)) != 0:
+            raise RuntimeError("hipMalloc failed")
+        self.ptr = p.value
+        if self.nbytes and H.hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1) != 0:  # host -> device
+            raise RuntimeError("hipMemcpy failed")
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        H = _hip_rt()
+        if H.hipDeviceSynchronize() != 0 or (self.nbytes and H.hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2)):
+            raise RuntimeError("hipMemcpy failed")
+        return out
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            _hip_rt().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
 # finite-rate chemistry (include/pcfd_chem.h)
 
 class ChemDesc(C.Structure):
@@ -419,7 +472,8 @@ MLP_NPARAM = 6288
 MLP_SYMBOLS = ["pcfd_mlp_create", "pcfd_mlp_destroy", "pcfd_mlp_last_error", "pcfd_mlp_last_kernel_ms",
                "pcfd_mlp_set_params", "pcfd_mlp_get_params", "pcfd_mlp_forward", "pcfd_mlp_forward_device",
                "pcfd_mlp_loss_grad", "pcfd_mlp_train", "pcfd_chem_surrogate_wdot_device", "pcfd_chem_wdot_device",
-               "pcfd_chem_surrogate_wdot"]
+               "pcfd_chem_surrogate_wdot", "pcfd_chem_surrogate_wdot_device_noalloc"]
+SURROGATE_NOALLOC_MAX_CELLS = 1 << 22
 CHEM_SYMBOLS += MLP_SYMBOLS
 
 fp = C.POINTER(C.c_float)
@@ -450,6 +504,8 @@ def _mlp_lib():
         L.pcfd_mlp_train.argtypes = [vp, C.c_int, fp, fp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, fp]
         L.pcfd_chem_surrogate_wdot_device.argtypes = [vp, vp, C.POINTER(SurrogateDesc), C.c_int, vp, vp, vp, ip,
                                                       dp]
+        L.pcfd_chem_surrogate_wdot_device_noalloc.argtypes = [vp, vp, C.POINTER(SurrogateDesc), C.c_int, vp, vp,
+                                                              vp, ip, dp]
         L.pcfd_chem_wdot_device.argtypes = [vp, C.c_int, vp, vp, vp]
         L.pcfd_chem_surrogate_wdot.argtypes = [vp, vp, C.POINTER(SurrogateDesc), C.c_int, dp, dp, dp, ip, dp]
         L._mlp_ready = True
