@@ -128,7 +128,7 @@ def cpu_baseline(case, budget_s):
     one full implicit iteration of the same mesh and parameters, timed per
     phase.  Returns seconds per iteration and the residual time."""
     import oracle as O
-    pb = O.Problem.from_case(case, clip_jacobi=True)
+    pb = O.Problem.from_case(case)
     t0 = time.time()
     s = O.Solver(pb, case.initial_state())
     init = time.time() - t0

@@ -1400,7 +1400,10 @@ int o_solver_init(o_solver* s, const o_mesh* m, const o_params* p, const double*
   s->dt = (double*)calloc((size_t)m->nnode, sizeof(double));
   o_lsq_moments(m, s->sw);
   if (p->nsgs > 0) o_crs_init(m, &s->A);
-  s->iter = 1;
+  /* the iteration counter starts at 0 (SolutionSpaceBase, ucs/solutionSpaceBase.h:45):
+   * the first Newton iteration runs at iter 0 -- CFL = cflStart and, since
+   * iter > firstOrderSteps fails, first order without gradient or limiter */
+  s->iter = 0;
   /* PreIterate: UpdateCFL + ComputeTimesteps, ucs/solutionSpace.tcc:509-569 */
   s->cfl = o_cfl_update(p, s->iter, 0.0, 0.0, p->cfl_start); /* PreIterate (ucs/solutionSpace.tcc:546-551) */
   s->res = s->res_nm1 = 0.0;

@@ -19,15 +19,18 @@ constexpr int kW = 64;       // wavefront width = interleave width of the ELL/SE
 //
 // gELL (node-gather lists): nodes are cut into chunks of 64 consecutive ids;
 // chunk c holds gwidth[c] slots, slot s of lane l (node 64c+l) at
-// index (gbase[c]+s)*64 + l.  Slots list the node's neighbours in ascending
-// id, -1 after the last one.  Consecutive lanes read consecutive addresses.
+// index (gbase[c]+s)*64 + l.  Slots list the node's neighbours in edge order
+// (the order the reference's edge loop adds their fluxes to the node: lower
+// neighbours ascending, then the node's own edges in the point-to-point
+// map's order), -1 after the last one.  Consecutive lanes read consecutive
+// addresses.
 //
 // SELL (block matrix): rows grouped by colour, within a colour sorted by
 // degree (descending) then id, padded to whole chunks of 64 rows (srow = -1
 // for padding).  Chunk k stores its off-diagonal blocks as
 // [slot][entry 0..24][lane] from sbase[k]*25*64, the diagonal block as
-// [entry][lane] at k*25*64; slot s of a row is its s-th neighbour in
-// ascending id, scol = -1 for padding.
+// [entry][lane] at k*25*64; slot s of a row is its s-th gELL neighbour,
+// scol = -1 for padding.
 struct Dev {
   // geometry
   double* xyz = nullptr;    // 3 x nnode
@@ -59,6 +62,12 @@ struct Dev {
   double* grad = nullptr;   // 27 x nnode (term*3+dir)
   double* lim = nullptr;    // 5 x nnode
   double* lim2 = nullptr;   // 5 x nnode (pre-clip copy)
+  // pressure clip in edge order
+  int* edges = nullptr;            // 2 x nedge, the edge list (n0, n1) in the reference's numbering
+  unsigned char* clip_flag = nullptr;  // nnode
+  int* clip_cnt = nullptr;         // per 64-edge pass
+  int* clip_off = nullptr;         // passes + 1
+  int* clip_list = nullptr;        // nedge (the flagged edges, in order)
   double* b = nullptr;      // 5 x nnode
   double* dt = nullptr;     // nnode
   double* dt_next = nullptr;  // nnode: the time step being computed while the Jacobian rows read dt

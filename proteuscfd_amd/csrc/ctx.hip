@@ -43,7 +43,8 @@ void free_all(Ctx& c) {
   dfree(d.xyz); dfree(d.vol); dfree(d.vol23); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
   dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
   dfree(d.jdb); dfree(d.jqb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
-  dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.b); dfree(d.dt);
+  dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.edges); dfree(d.clip_flag); dfree(d.clip_cnt);
+  dfree(d.clip_off); dfree(d.clip_list); dfree(d.b); dfree(d.dt);
   dfree(d.srow); dfree(d.swidth); dfree(d.sbase); dfree(d.scol); dfree(d.Md); dfree(d.Mf); dfree(d.Dd);
   dfree(d.Df); dfree(d.piv); dfree(d.bsd); dfree(d.bsf); dfree(d.xd); dfree(d.xf); dfree(d.red);
   dfree(d.red_out); dfree(d.icount); dfree(d.dt_next); dfree(d.qjac);
@@ -197,7 +198,9 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     c.ntot = nn + nb;
     c.ntag = nt;
     c.nnz = 2 * ne;
-    // ---- node adjacency, neighbours ascending, with the canonical edge id
+    // ---- node adjacency in edge order, with the edge id: a node's slots
+    //      hold its neighbours in the order the reference's edge loop adds
+    //      their fluxes to it (Driver, ucs/driver.tcc:5-67)
     std::vector<int> row_ptr(nn + 1, 0);
     for (int e = 0; e < ne; ++e) {
       row_ptr[m->edges[2 * e] + 1]++;
@@ -213,10 +216,13 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
         ent[fill[b]++] = {a, e};
       }
     }
-    for (int i = 0; i < nn; ++i) {
-      std::sort(ent.begin() + row_ptr[i], ent.begin() + row_ptr[i + 1]);
-      for (int p = row_ptr[i] + 1; p < row_ptr[i + 1]; ++p)
-        if (ent[p].first == ent[p - 1].first) throw std::runtime_error("pcfd_create: duplicate edge between nodes");
+    {
+      std::vector<int> stamp(nn, -1);
+      for (int i = 0; i < nn; ++i)
+        for (int p = row_ptr[i]; p < row_ptr[i + 1]; ++p) {
+          if (stamp[ent[p].first] == i) throw std::runtime_error("pcfd_create: duplicate edge between nodes");
+          stamp[ent[p].first] = i;
+        }
     }
     auto deg = [&](int i) { return row_ptr[i + 1] - row_ptr[i]; };
     // ---- gELL: chunks of 64 consecutive nodes
@@ -388,6 +394,15 @@ int pcfd_create(const pcfd_mesh_desc* m, pcfd_ctx** out) {
     dalloc(c, &d.grad, 27 * (size_t)nn);
     dalloc(c, &d.lim, 5 * (size_t)nn);
     dalloc(c, &d.lim2, 5 * (size_t)nn);
+    {
+      std::vector<int> ev(m->edges, m->edges + 2 * (size_t)ne);
+      dput(c, &d.edges, ev);
+      const size_t npass = ((size_t)ne + 63) / 64;
+      dalloc(c, &d.clip_flag, (size_t)nn);
+      dalloc(c, &d.clip_cnt, std::max<size_t>(npass, 1));
+      dalloc(c, &d.clip_off, npass + 1);
+      dalloc(c, &d.clip_list, std::max<size_t>((size_t)ne, 1));
+    }
     dalloc(c, &d.b, 5 * (size_t)nn);
     dalloc(c, &d.dt, (size_t)nn);
     dput(c, &d.srow, srow);
@@ -657,8 +672,8 @@ int pcfd_download_jacobian(pcfd_ctx* ctx, const int* ia, const int* ja, double* 
         }
         const int* cb = c.h_col.data() + c.h_row_ptr[i];
         const int* ce = c.h_col.data() + c.h_row_ptr[i + 1];
-        const int* f = std::lower_bound(cb, ce, j);
-        if (f == ce || *f != j) {
+        const int* f = std::find(cb, ce, j);  // a row's slots follow the edge order
+        if (f == ce) {
           c.err = "pcfd_download_jacobian: block (" + std::to_string(i) + "," + std::to_string(j) +
                   ") is not in the mesh graph";
           return PCFD_ERR_ARG;

@@ -13,6 +13,7 @@
 
 #include "common.h"
 #include "physics.h"
+#include "scan.h"
 
 namespace pcfd {
 
@@ -452,18 +453,31 @@ __device__ __forceinline__ bool bad_state_r(const double* Q, double gamma) {
   return (p < 1.0e-10) || (r < 0.0) || (E < 1.0e-10);
 }
 
-// Kernel_PressureClip (ucs/limiters.tcc:736-815) with every edge seeing the
-// pre-clip limiter (the reference's sequential edge loop lets an earlier
-// edge's zeroing change a later edge's extrapolation; the two agree whenever
-// no clip chain forms), then negative limiters -> 0 (:118-124).
-__global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, double chi, double gamma, Gell g,
-                                                         const double* __restrict__ xyz,
-                                                         const double* __restrict__ q,
-                                                         const double* __restrict__ grad,
-                                                         const double* __restrict__ lim2, double* __restrict__ lim) {
+// Kernel_PressureClip (ucs/limiters.tcc:736-815) in the reference's edge
+// order, then negative limiters -> 0 (:118-124).  The reference walks the
+// edges in order and zeroes a node's limiters as soon as one of its
+// extrapolations, or the Roe state of the pair, is unphysical, so a later
+// edge sees the zeroed values.  An edge can only zero anything when one of
+// its ends extrapolates to an unphysical state: the Roe state of two
+// physical states is physical (its pressure is at least the smaller end
+// pressure), and a zeroed node extrapolates to its own state.  Hence:
+//  1. k_clip_flag: every node checks its edges with the pre-clip limiters and
+//     is flagged when one of its own extrapolations or a Roe state is
+//     unphysical, or its own state is; it writes its clamped pre-clip
+//     limiter as the result for the case no edge touches it;
+//  2. k_clip_count / k_scan_counts / k_clip_list: the edges with a flagged
+//     end, in edge order (per-64-edge ballots and a fixed-order scan: no
+//     atomics);
+//  3. k_clip_replay: one thread runs the reference's loop over that list on
+//     the current limiters.  Edges off the list cannot fire, so the result
+//     is the reference's full loop.
+__global__ void __launch_bounds__(kBlock) k_clip_flag(int nnode, int ntot, double chi, double gamma, Gell g,
+                                                     const double* __restrict__ xyz, const double* __restrict__ q,
+                                                     const double* __restrict__ grad,
+                                                     const double* __restrict__ lim2, double* __restrict__ lim,
+                                                     unsigned char* __restrict__ flag) {
   int i = xcd_block() * blockDim.x + threadIdx.x;
   if (i >= nnode) return;
-  bool zero = false;
   // The node's own state, gradient and limiter stay in registers; only the
   // neighbour side is gathered.  Both extrapolations are written relative to
   // node i: dQ = qj - qi and dx = (xj - xi)/2 give bit-identical states to
@@ -477,10 +491,11 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
   }
 #pragma unroll
   for (int k = 0; k < 15; ++k) gi[k] = grad[k * nnode + i];
+  bool fire = bad_state(qi, gamma);
   const double xi[3] = {xyz[i], xyz[nnode + i], xyz[2 * nnode + i]};
   const long long b = gbase_of(g, i);
   const int w = g.width[i >> 6];
-  for (int s = 0; s < w && !zero; ++s) {
+  for (int s = 0; s < w && !fire; ++s) {
     const int j = g.col[b + (long long)s * kW];
     if (j < 0) break;
     const double dx[3] = {0.5 * (xyz[j] - xi[0]), 0.5 * (xyz[nnode + j] - xi[1]), 0.5 * (xyz[2 * nnode + j] - xi[2])};
@@ -494,18 +509,103 @@ __global__ void __launch_bounds__(kBlock) k_pressure_clip(int nnode, int ntot, d
       Qj[k] = extrap1(qj, -dQ, grad[(3 * k) * nnode + j], grad[(3 * k + 1) * nnode + j],
                       grad[(3 * k + 2) * nnode + j], mdx, chi, lim2[k * nnode + j]);
     }
-    if (bad_state_r(Qi, gamma)) zero = true;
+    if (bad_state(Qi, gamma)) fire = true;
     double Qr[5];
     if (i < j) roe_state(Qi, Qj, gamma, Qr);
     else roe_state(Qj, Qi, gamma, Qr);
-    if (bad_state_r(Qr, gamma)) zero = true;
+    if (bad_state(Qr, gamma)) fire = true;
   }
+  flag[i] = fire ? 1 : 0;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    double v = zero ? 0.0 : li[k];
-    if (v < 0.0) v = 0.0;
-    lim[k * nnode + i] = v;
+  for (int k = 0; k < 5; ++k) lim[k * nnode + i] = li[k] < 0.0 ? 0.0 : li[k];
+}
+
+// cnt[p] = number of edges p*64 .. p*64+63 with a flagged end
+__global__ void __launch_bounds__(kBlock) k_clip_count(int nedge, int npass, const int* __restrict__ edges,
+                                                      const unsigned char* __restrict__ flag,
+                                                      int* __restrict__ cnt) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int pass = (int)(t >> 6), lane = threadIdx.x & 63;
+  const long long e = t;
+  bool cand = false;
+  if (e < nedge) cand = (flag[edges[2 * e]] | flag[edges[2 * e + 1]]) != 0;
+  const unsigned long long m = __ballot(cand);
+  if (lane == 0 && pass < npass) cnt[pass] = __popcll(m);
+}
+
+// list[off[p] + r] = the r-th flagged edge of pass p (edge order)
+__global__ void __launch_bounds__(kBlock) k_clip_list(int nedge, int npass, const int* __restrict__ edges,
+                                                     const unsigned char* __restrict__ flag,
+                                                     const int* __restrict__ cnt, const int* __restrict__ off,
+                                                     int* __restrict__ list) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int pass = (int)(t >> 6), lane = threadIdx.x & 63;
+  if (pass >= npass || cnt[pass] == 0) return;  // uniform across the wavefront
+  const long long e = t;
+  bool cand = false;
+  if (e < nedge) cand = (flag[edges[2 * e]] | flag[edges[2 * e + 1]]) != 0;
+  const unsigned long long m = __ballot(cand);
+  if (cand) list[off[pass] + __popcll(m & ((1ull << lane) - 1ull))] = (int)e;
+}
+
+// The reference's PressureClip loop over the listed edges, in order, on the
+// current (raw) limiters lim2; zeroed nodes are written to lim2 and lim.
+// One thread: the list is empty unless a clip fires, and short when it does.
+__global__ void __launch_bounds__(64) k_clip_replay(int nnode, int ntot, int npass, double chi, double gamma,
+                                                   const int* __restrict__ edges, const int* __restrict__ off,
+                                                   const int* __restrict__ list, const double* __restrict__ xyz,
+                                                   const double* __restrict__ q, const double* __restrict__ grad,
+                                                   double* lim2, double* lim) {
+  if (threadIdx.x != 0) return;
+  const int total = off[npass];
+  for (int t = 0; t < total; ++t) {
+    const int e = list[t];
+    const int L = edges[2 * e], R = edges[2 * e + 1];
+    double dx[3], mdx[3], dQ[5], QL[5], QR[5];
+    for (int d = 0; d < 3; ++d) {
+      dx[d] = 0.5 * (xyz[d * nnode + R] - xyz[d * nnode + L]);
+      mdx[d] = -dx[d];
+    }
+    for (int k = 0; k < 5; ++k) {
+      dQ[k] = q[k * ntot + R] - q[k * ntot + L];
+      QL[k] = extrap1(q[k * ntot + L], dQ[k], grad[(3 * k) * nnode + L], grad[(3 * k + 1) * nnode + L],
+                      grad[(3 * k + 2) * nnode + L], dx, chi, lim2[k * nnode + L]);
+    }
+    bool zl = bad_state(QL, gamma), zr = false;
+    if (zl)
+      for (int k = 0; k < 5; ++k) lim2[k * nnode + L] = 0.0;
+    for (int k = 0; k < 5; ++k)
+      QR[k] = extrap1(q[k * ntot + R], -dQ[k], grad[(3 * k) * nnode + R], grad[(3 * k + 1) * nnode + R],
+                      grad[(3 * k + 2) * nnode + R], mdx, chi, lim2[k * nnode + R]);
+    if (bad_state(QR, gamma)) zr = true;
+    double Qr[5];
+    roe_state(QL, QR, gamma, Qr);
+    if (bad_state(Qr, gamma)) zl = zr = true;
+    for (int k = 0; k < 5; ++k) {
+      if (zl) lim2[k * nnode + L] = lim[k * nnode + L] = 0.0;
+      if (zr) lim2[k * nnode + R] = lim[k * nnode + R] = 0.0;
+    }
   }
+}
+
+void launch_pressure_clip(Ctx& c) {
+  const pcfd_params& p = c.prm;
+  hipLaunchKernelGGL(k_clip_flag, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi, p.gamma,
+                     gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim, c.d.clip_flag);
+  PCFD_HIP(hipGetLastError());
+  if (c.nedge == 0) return;
+  const int npass = (c.nedge + 63) / 64;
+  const int nb = nblk((long long)npass * 64);
+  hipLaunchKernelGGL(k_clip_count, dim3(nb), dim3(kBlock), 0, c.stream, c.nedge, npass, c.d.edges, c.d.clip_flag,
+                     c.d.clip_cnt);
+  PCFD_HIP(hipGetLastError());
+  launch_scan_counts(c.stream, npass, c.d.clip_cnt, c.d.clip_off);
+  hipLaunchKernelGGL(k_clip_list, dim3(nb), dim3(kBlock), 0, c.stream, c.nedge, npass, c.d.edges, c.d.clip_flag,
+                     c.d.clip_cnt, c.d.clip_off, c.d.clip_list);
+  PCFD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_clip_replay, dim3(1), dim3(64), 0, c.stream, c.nnode, c.ntot, npass, p.chi, p.gamma,
+                     c.d.edges, c.d.clip_off, c.d.clip_list, c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
+  PCFD_HIP(hipGetLastError());
 }
 
 void launch_limiter(Ctx& c) {
@@ -514,9 +614,7 @@ void launch_limiter(Ctx& c) {
   hipLaunchKernelGGL(k_limiter, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.limiter, K3,
                      p.chi, gell(c), c.d.xyz, c.d.vol, c.d.q, c.d.grad, c.d.lim2);
   PCFD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
-                     p.gamma, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
-  PCFD_HIP(hipGetLastError());
+  launch_pressure_clip(c);
 }
 
 // weighted LSQ gradient and the limiter in one pass, then the pressure clip
@@ -528,9 +626,7 @@ void launch_gradient_limiter(Ctx& c, int nterms) {
                      c.d.sw, c.d.q, c.d.he_ptr, c.d.he_id, c.d.he_tag, c.d.he_a, c.nbnode, c.d.bc_type, p.gamma,
                      c.d.grad, L);
   PCFD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_pressure_clip, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, p.chi,
-                     p.gamma, gell(c), c.d.xyz, c.d.q, c.d.grad, c.d.lim2, c.d.lim);
-  PCFD_HIP(hipGetLastError());
+  launch_pressure_clip(c);
 }
 
 // ---------------------------------------------------------------------------

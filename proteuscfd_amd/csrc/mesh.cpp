@@ -2,7 +2,8 @@
 // include/pcfd_mesh.h).  Plain C++17, built with g++; no GPU code.
 //
 // Reference behaviour followed (ucs/ = reference solver sources):
-//  * edges n[0]<n[1], one interior edge per node pair    ucs/mesh.tcc:1077-1091
+//  * edges n[0]<n[1], one interior edge per node pair, numbered in the
+//    order of the point-to-point map                      ucs/mesh.tcc:517-660,1077-1091
 //  * boundary half-edges: one per (boundary face, node), phantom id nnode+k
 //                                                          ucs/mesh.tcc:1102-1133
 //  * dual facet of edge (p,q) inside an element: the loop edge-midpoint,
@@ -335,6 +336,74 @@ std::vector<uint64_t> edge_keys(const Work& w) {
   return keys;
 }
 
+// Neighbour order of each local node of an element, in the order the
+// reference's point-to-point map visits them (the connection table of
+// Mesh::BuildPsp, ucs/mesh.tcc:538-545), indexed by element type
+// (PCFD_TRI .. PCFD_HEX) and local node; -1 ends a list.
+const int kPspOrder[6][8][5] = {
+    {{1, 2, -1}, {2, 0, -1}, {0, 1, -1}},
+    {{1, 3, -1}, {2, 0, -1}, {3, 1, -1}, {0, 2, -1}},
+    {{1, 2, 3, -1}, {2, 0, 3, -1}, {0, 1, 3, -1}, {0, 1, 2, -1}},
+    {{1, 3, 4, -1}, {2, 0, 4, -1}, {3, 1, 4, -1}, {0, 2, 4, -1}, {0, 1, 2, 3, -1}},
+    {{1, 2, 3, -1}, {2, 0, 4, -1}, {0, 1, 5, -1}, {4, 5, 0, -1}, {5, 3, 1, -1}, {3, 4, 2, -1}},
+    {{1, 3, 4, -1}, {2, 0, 5, -1}, {3, 1, 6, -1}, {0, 2, 7, -1}, {5, 7, 0, -1}, {6, 4, 1, -1}, {7, 5, 2, -1},
+     {4, 6, 3, -1}}};
+
+// Edges in the order ucs.x numbers them (Mesh::BuildPsp + BuildEdges,
+// ucs/mesh.tcc:517-660,1077-1091): node i's neighbours come from the
+// elements holding i in element-list order -- boundary faces first, then
+// volume elements, the order a partition file stores them -- each adding the
+// neighbours of i's local node in the table's order, first occurrence kept;
+// then edge (i, j) for every neighbour j > i, in that order, i ascending.
+// Returns the edges as (n0 << 32 | n1) keys in that order.
+std::vector<uint64_t> edges_in_psp_order(const Work& w, int nn) {
+  const size_t nbf = w.btype.size(), ne = w.etype.size();
+  std::vector<int> cnt(nn + 1, 0);
+  auto nodes_of = [&](size_t el, const int*& n, int& type) -> int {
+    if (el < nbf) {
+      n = &w.bnodes[w.bptr[el]];
+      type = w.btype[el];
+      return w.bptr[el + 1] - w.bptr[el];
+    }
+    const size_t e = el - nbf;
+    n = &w.enodes[w.eptr[e]];
+    type = w.etype[e];
+    return w.eptr[e + 1] - w.eptr[e];
+  };
+  for (size_t el = 0; el < nbf + ne; ++el) {
+    const int* n;
+    int t;
+    const int k = nodes_of(el, n, t);
+    for (int j = 0; j < k; ++j) cnt[n[j] + 1]++;
+  }
+  for (int i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> elsp(cnt[nn]), fill(cnt.begin(), cnt.end() - 1);
+  for (size_t el = 0; el < nbf + ne; ++el) {
+    const int* n;
+    int t;
+    const int k = nodes_of(el, n, t);
+    for (int j = 0; j < k; ++j) elsp[fill[n[j]]++] = (int)el;
+  }
+  std::vector<uint64_t> out;
+  std::vector<int> stamp(nn, -1);
+  for (int i = 0; i < nn; ++i) {
+    for (int p = cnt[i]; p < cnt[i + 1]; ++p) {
+      const int* n;
+      int t;
+      const int k = nodes_of((size_t)elsp[p], n, t);
+      int ln = 0;
+      while (ln < k && n[ln] != i) ++ln;
+      for (const int* c = kPspOrder[t][ln]; *c >= 0; ++c) {
+        const int j = n[*c];
+        if (stamp[j] == i) continue;
+        stamp[j] = i;
+        if (j > i) out.push_back(((uint64_t)i << 32) | (uint32_t)j);
+      }
+    }
+  }
+  return out;
+}
+
 // reverse Cuthill-McKee: BFS from a pseudo-peripheral minimum-degree node,
 // unvisited neighbours appended in ascending degree, then the order reversed.
 std::vector<int> rcm_order(int nn, const std::vector<uint64_t>& keys) {
@@ -541,13 +610,23 @@ static int finish(Work& w, int rcm, pcfd_mesh** out) {
   }
 
   pcfd_mesh* m = new pcfd_mesh();
-  std::vector<uint64_t> keys = edge_keys(w);
-  const size_t ned = keys.size();
+  const std::vector<uint64_t> order = edges_in_psp_order(w, nn);
+  const size_t ned = order.size();
   m->edges.resize(2 * ned);
   for (size_t e = 0; e < ned; ++e) {
-    m->edges[2 * e] = (int)(keys[e] >> 32);
-    m->edges[2 * e + 1] = (int)(keys[e] & 0xffffffffu);
+    m->edges[2 * e] = (int)(order[e] >> 32);
+    m->edges[2 * e + 1] = (int)(order[e] & 0xffffffffu);
   }
+  // sorted keys -> edge number, for the metric accumulation below
+  std::vector<std::pair<uint64_t, int>> keys(ned);
+  for (size_t e = 0; e < ned; ++e) keys[e] = {order[e], (int)e};
+  std::sort(keys.begin(), keys.end());
+  for (size_t e = 1; e < ned; ++e)
+    if (keys[e].first == keys[e - 1].first) {
+      g_err = "mesh: duplicate edge in the point-to-point map";
+      delete m;
+      return -1;
+    }
   std::vector<double> A(3 * ned, 0.0);
   m->vol.assign(nn, 0.0);
   double elem_vol = 0.0;
@@ -606,7 +685,7 @@ static int finish(Work& w, int rcm, pcfd_mesh** out) {
         sgn = -1.0;
       }
       uint64_t key = ((uint64_t)a << 32) | b;
-      size_t idx = std::lower_bound(keys.begin(), keys.end(), key) - keys.begin();
+      const size_t idx = (size_t)std::lower_bound(keys.begin(), keys.end(), std::make_pair(key, -1))->second;
       A[3 * idx + 0] += sgn * Apq.x;
       A[3 * idx + 1] += sgn * Apq.y;
       A[3 * idx + 2] += sgn * Apq.z;

@@ -137,7 +137,9 @@ int pcfd_reset_iteration(pcfd_ctx* ctx) {
   }
   try {
     jacobian_join(c);
-    c.iter = 1;
+    // the counter starts at 0 (ucs/solutionSpaceBase.h:45): the first
+    // iteration uses cflStart and, as iter > firstOrderSteps fails, first order
+    c.iter = 0;
     c.jac_valid = false;
     c.lu_done = false;
     // Limiter ctor fills 1.0 (ucs/limiters.tcc:39-43); PreIterate computes dt

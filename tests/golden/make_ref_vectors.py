@@ -1,0 +1,294 @@
+"""Golden vectors from the reference solver itself.
+
+Runs oracle/_ref/ref_harness -- the reference's ucs solver core compiled from
+its own sources by oracle/build_ref.py against the build container's MPICH
+and HDF5 -- on small meshes from this repository's generator, written as
+partition files <case>.0.h5 (ucs.x's format) with a .param and a .bc file,
+and stores every input and output as tests/golden/ref_<name>.npz.
+
+Run in the build container (it needs /root/reference and /opt/conda):
+    python tests/golden/make_ref_vectors.py [name ...]
+The fixtures are data; the tests that read them never run the reference.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import build_ref  # noqa: E402
+from proteuscfd_amd import cases  # noqa: E402
+
+# BCTypes names as the reference's .bc parser knows them (ucs/bc_defines.h:31-53)
+BC_NAMES = {1: "dirichlet", 2: "neumann", 3: "impermeableWall", 4: "noSlip", 6: "farField", 7: "sonicInflow",
+            8: "sonicOutflow", 9: "symmetry"}
+
+MESH_KEYS = ["edges", "edge_a", "bedges", "bedge_a", "bfactag", "xyz", "vol"]
+
+
+def param_text(name, keys, nsteps=1):
+    lines = [f"{k} = {v}" for k, v in keys.items()]
+    return (f"<<<BEGIN TEMPORAL CONTROL>>>\nnumTimeSteps = {nsteps}\nnewtonIterations = 1\n"
+            f"<<<END TEMPORAL CONTROL>>>\n\n<<<BEGIN SOLUTION ORDERING>>>\nIterate {name}\n"
+            f"<<<END SOLUTION ORDERING>>>\n\n<<<BEGIN SPACE {name}>>>\n" + "\n".join(lines) +
+            "\n<<<END SPACE>>>\n")
+
+
+def euler_keys(**kw):
+    k = {"equationSet": "compressibleEuler", "timeOrder": 0, "reorderMesh": 0, "refPressure": 101325,
+         "flowDirection": "[1.0, 0.0, 0.0]"}
+    k.update(kw)
+    return k
+
+
+class Run:
+    """One harness invocation in a scratch directory."""
+
+    def __init__(self, case, keys, bc, nsteps=1, name="c"):
+        self.case, self.name = case, name
+        self.dir = tempfile.mkdtemp(prefix="pcfd_ref_") + "/"
+        case.mesh.write_h5(self.dir + name, 1)
+        with open(self.dir + name + ".param", "w") as f:
+            f.write(param_text(name, keys, nsteps))
+        with open(self.dir + name + ".bc", "w") as f:
+            for tag, typ in bc.items():
+                f.write(f"surface #{tag} = {BC_NAMES[typ]}\n")
+
+    def __call__(self, mode, *args):
+        exe = build_ref.harness_path()
+        r = subprocess.run([exe, self.dir, self.name, mode, *map(str, args)], capture_output=True, text=True,
+                           cwd=self.dir)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout[-3000:] + r.stderr[-3000:])
+            raise RuntimeError(f"ref_harness {mode} failed ({r.returncode})")
+        self.meta = {}
+        for line in open(self.dir + "ref_meta.txt"):
+            k, v = line.split()
+            self.meta[k] = float(v)
+        return self
+
+    def arr(self, name, dtype=np.float64):
+        return np.fromfile(self.dir + "ref_" + name + ".bin", dtype=dtype)
+
+    def write_q(self, q):
+        np.ascontiguousarray(q, dtype=np.float64).tofile(self.dir + "q_in.bin")
+
+    def mesh_arrays(self):
+        m = self.case.mesh
+        out = {"edges": self.arr("edges", np.int32).reshape(-1, 2), "edge_a": self.arr("edge_a").reshape(-1, 4),
+               "bedges": self.arr("bedges", np.int32).reshape(-1, 2), "bedge_a": self.arr("bedge_a").reshape(-1, 4),
+               "bfactag": self.arr("bfactag", np.int32), "xyz": self.arr("xyz").reshape(-1, 3),
+               "vol": self.arr("vol")}
+        assert out["xyz"].shape == m.xyz.shape
+        return {"mesh_" + k: v for k, v in out.items()}
+
+
+def phases(run, q, it, nsgs, nvars=10):
+    """The harness's phase-by-phase implicit step on state q (all rows)."""
+    m = run.case.mesh
+    run.write_q(q)
+    run("phases", it, nsgs)
+    nn, nt = m.nnode, m.nnode + m.nbedge
+    out = {"q_in": q, "q_bc": run.arr("q_bc").reshape(nt, nvars), "grad": run.arr("grad").reshape(nn, -1),
+           "residual": run.arr("residual").reshape(nn, -1), "residual_norms": run.arr("residual_norms"),
+           "dt": run.arr("dt"), "ia": run.arr("ia", np.int32), "ja": run.arr("ja", np.int32),
+           "jacobian": run.arr("jacobian").reshape(-1, 5, 5), "dq": run.arr("dq").reshape(nn, -1),
+           "q_new": run.arr("q_new").reshape(nt, nvars)}
+    if os.path.exists(run.dir + "ref_limiter.bin"):
+        out["limiter"] = run.arr("limiter").reshape(nn, -1)
+    for k in ["gamma", "cfl", "dtmin", "sgs_delta", "Re", "ref_temperature"]:
+        out[k] = np.float64(run.meta[k])
+    out["iter"] = np.int32(it)
+    out["nsgs"] = np.int32(nsgs)
+    out["qinf"] = run.arr("qinf")
+    return out
+
+
+def hot_state(case, gamma, seed, radius):
+    """Freestream plus small smooth noise, with strongly disturbed states
+    (random density, velocity, pressure) inside a ball at the domain centre,
+    where the extrapolated states turn unphysical and the pressure clip
+    zeroes limiters in chains."""
+    rng = np.random.default_rng(seed)
+    m = case.mesh
+    q = cases.perturbed_state(case, amp=0.05, seed=seed)
+    x = np.concatenate([m.xyz, m.xyz[m.bedges[:, 0]]])
+    hot = np.linalg.norm(x - 0.5, axis=1) < radius
+    rho, vel, p = q[:, 0].copy(), q[:, 1:4] / q[:, :1], q[:, 6].copy()
+    k = int(hot.sum())
+    rho[hot] = np.exp(rng.normal(0.0, 0.5, k))
+    vel[hot] = rng.normal(0.0, 1.5, (k, 3))
+    p[hot] = np.exp(rng.normal(np.log(0.3), 1.0, k))
+    out = np.zeros_like(q)
+    out[:, 0] = rho
+    out[:, 1:4] = rho[:, None] * vel
+    out[:, 4] = p / (gamma - 1.0) + 0.5 * rho * (vel ** 2).sum(axis=1)
+    return cases.aux(out, gamma)
+
+
+def bump_bc():
+    return {1: 6, 2: 6, 3: 6, 4: 6, 5: 3, 6: 6}
+
+
+def make_bump5_phases():
+    """Second-order Euler on a 5^3 bump (216 nodes): BCs, weighted-LSQ
+    gradient, modified Venkat, residual, time step, FD Jacobian, 8 SGS
+    (natural row order) and ApplyDQ, on a perturbed state."""
+    case = cases.bump_case(n=5, rcm=False)
+    run = Run(case, euler_keys(spatialOrder=2, limiter=3, numberSGS=8, CFL=50.0, startingCFL=1.0, rampCFL=50,
+                               velocity=0.5), bump_bc())
+    # the reference's gamma comes from its default Cp and MW: build the state with it
+    gamma = ref_gamma(run)
+    q = cases.aux(cases.perturbed_state(case, amp=0.05, seed=7).copy(), gamma)
+    out = phases(run, q, 10, 8)
+    out.update(run.mesh_arrays())
+    return out
+
+
+_GAMMA = {}
+
+
+def ref_gamma(run):
+    """gamma of the reference's default gas (Cp, MW), from a pointwise call."""
+    if "g" not in _GAMMA:
+        np.zeros((0, 32)).tofile(run.dir + "pw_in.bin")
+        run("pointwise")
+        _GAMMA["g"] = run.meta["gamma"]
+    return _GAMMA["g"]
+
+
+def make_bump6_clip():
+    """Limiters 1 (Barth), 2 (Venkat), 3 (modified Venkat) on a 6^3 bump
+    whose centre holds strongly disturbed states: the pressure clip zeroes
+    node limiters, in chains where an earlier edge's zeroing changes a later
+    edge's extrapolation (ucs/limiters.tcc:736-815)."""
+    case = cases.bump_case(n=6, rcm=False)
+    out = {}
+    for lim in (1, 2, 3):
+        run = Run(case, euler_keys(spatialOrder=2, limiter=lim, numberSGS=2, CFL=5.0, startingCFL=5.0,
+                                   velocity=0.5), bump_bc())
+        gamma = ref_gamma(run)
+        q = hot_state(case, gamma, seed=4, radius=0.3)
+        r = phases(run, q, 5, 2)
+        for k in ["limiter", "residual"]:
+            out[f"{k}_{lim}"] = r[k]
+        out["q_in"], out["q_bc"], out["grad"] = q, r["q_bc"], r["grad"]
+        out["gamma"] = r["gamma"]
+        out["qinf"] = r["qinf"]
+    return out
+
+
+def make_wedge_phases():
+    """Second-order Euler on a small 15-degree wedge channel at M 2.5: sonic
+    inflow (hard-set interior nodes), sonic outflow, slip walls, symmetry
+    planes; Venkat limiter."""
+    case = cases.wedge_case(nx=8, ny=4, nz=2, rcm=False)
+    bc = {1: 7, 2: 8, 3: 3, 4: 3, 5: 9, 6: 9}
+    run = Run(case, euler_keys(spatialOrder=2, limiter=2, numberSGS=4, CFL=2.0, startingCFL=2.0, velocity=2.5),
+              bc)
+    gamma = ref_gamma(run)
+    q = cases.aux(cases.perturbed_state(case, amp=0.05, seed=11).copy(), gamma)
+    out = phases(run, q, 3, 4)
+    out.update(run.mesh_arrays())
+    return out
+
+
+def make_pointwise():
+    """Farfield and slip-wall boundary states and ApplyDQ, evaluated by the
+    reference's eqnset on states that reach every branch: farfield
+    subsonic/supersonic inflow and outflow with the interior far from the
+    freestream, slip walls with non-zero normal velocity, ApplyDQ with
+    density, energy and pressure driven below zero."""
+    case = cases.bump_case(n=2, rcm=False)
+    run = Run(case, euler_keys(velocity=0.5), bump_bc())
+    gamma = ref_gamma(run)
+    rng = np.random.default_rng(21)
+    recs = []
+
+    def state(rho, vel, p):
+        q = np.zeros(10)
+        q[0] = rho
+        q[1:4] = rho * np.asarray(vel)
+        q[4] = p / (gamma - 1.0) + 0.5 * rho * np.dot(vel, vel)
+        return cases.aux(q, gamma)
+
+    for k in range(400):
+        kind = 0 if k < 240 else 1
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        a = np.concatenate([n, [rng.uniform(0.01, 0.1)]])
+        c = np.sqrt(gamma * 1.0 / gamma)  # freestream sound speed (p/rho = 1/gamma)
+        # normal Mach from -2.5 to 2.5 (all four characteristic branches), plus tangential flow
+        mn = rng.uniform(-2.5, 2.5) if kind == 0 else rng.uniform(-0.6, 0.6)
+        t = rng.normal(size=3)
+        t -= np.dot(t, n) * n
+        rho = np.exp(rng.uniform(-0.7, 0.7))
+        p = np.exp(rng.uniform(-1.0, 1.0)) / gamma
+        cl = np.sqrt(gamma * p / rho)
+        vel = mn * cl * n + 0.5 * c * t
+        QL = state(rho, vel, p)
+        # the phantom on entry: a different state (the reference iterates from it)
+        QR = state(rho * np.exp(rng.normal(0, 0.2)), vel * rng.uniform(0.5, 1.5), p * np.exp(rng.normal(0, 0.2)))
+        rec = np.zeros(32)
+        rec[0] = kind
+        rec[1:11], rec[11:21], rec[21:25], rec[25] = QL, QR, a, 0.0
+        recs.append(rec)
+    for k in range(200):
+        Q = state(np.exp(rng.uniform(-1, 1)), rng.normal(0, 1.0, 3), np.exp(rng.uniform(-2, 0)))
+        dQ = rng.normal(0, 1.0, 5) * np.abs(Q[:5])
+        mode = k % 4
+        if mode == 0:
+            dQ[0] = -1.5 * Q[0]            # density below zero
+        elif mode == 1:
+            dQ[4] = -1.5 * Q[4]            # energy below zero
+        elif mode == 2:
+            dQ[1:4] = 3.0 * Q[1:4]         # kinetic energy above the total: pressure clip
+            dQ[4] = -0.5 * Q[4]
+        rec = np.zeros(32)
+        rec[0] = 2
+        rec[1:6], rec[6:16] = dQ, Q
+        recs.append(rec)
+    recs = np.array(recs)
+    recs.tofile(run.dir + "pw_in.bin")
+    run("pointwise")
+    out = run.arr("pointwise").reshape(-1, 10)
+    return {"records": recs, "out": out, "gamma": np.float64(gamma), "qinf": run.arr("qinf")}
+
+
+def make_bump6_solve():
+    """The reference's own steady loop (natural-order SGS) from freestream to
+    an 8-order drop of residGlobal on a 6^3 bump, second order, modified
+    Venkat: the residual history and the converged field."""
+    case = cases.bump_case(n=6, rcm=False)
+    run = Run(case, euler_keys(spatialOrder=2, limiter=3, numberSGS=8, CFL=20.0, startingCFL=1.0, rampCFL=20,
+                               velocity=0.5), bump_bc(), nsteps=2000)
+    run("solve", 2000, 1e-8)
+    m = case.mesh
+    out = {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 10),
+           "gamma": np.float64(run.meta["gamma"]), "qinf": run.arr("qinf")}
+    return out
+
+
+MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
+          "pointwise": make_pointwise, "bump6_solve": make_bump6_solve}
+
+
+def main(names):
+    if build_ref.build() is None:
+        raise SystemExit("the reference sources or the conda MPI/HDF5 installation are missing")
+    for name in names or MAKERS:
+        out = MAKERS[name]()
+        path = os.path.join(HERE, f"ref_{name}.npz")
+        np.savez_compressed(path, **out)
+        print(f"{path}: {os.path.getsize(path)} bytes")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

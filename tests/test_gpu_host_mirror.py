@@ -68,7 +68,7 @@ def run_and_check(tmp_path, arg, case, iters=3, resid_rtol=1e-8, alpha=0.0, beta
 
     assert (nnode, nbnode) == (case.mesh.nnode, case.mesh.nbedge)
     q0 = initial_state(case, alpha, beta)
-    pb = O.Problem.from_case(case, clip_jacobi=True)
+    pb = O.Problem.from_case(case)
     qo = pb.update_bcs(q0)
     go = pb.gradient(qo)
     lo = pb.limiter(qo, go)

@@ -48,7 +48,7 @@ def setup(request, tmp_path_factory):
         bc = {1: cases.BC_SYMMETRY, 2: cases.BC_SYMMETRY, 3: cases.BC_FARFIELD, 4: cases.BC_IMPERMEABLE}
         case = cases.file_case(path, bc, mach=0.3, params=Params(sorder=1, cfl=1.0, cfl_start=1.0, nsgs=8))
         q0 = case.initial_state()
-    pb = O.Problem.from_case(case, clip_jacobi=True)
+    pb = O.Problem.from_case(case)
     ctx = gpu.Context(case)
     print(f"[meshfile] {request.param} nnode={case.mesh.nnode} nedge={case.mesh.nedge} nbedge={case.mesh.nbedge}",
           flush=True)

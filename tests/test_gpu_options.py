@@ -48,7 +48,7 @@ def setup(request):
 
 
 def problem(case, p):
-    return O.Problem(case.mesh, case.bc_type, case.qref, p, clip_jacobi=True, twall=case.twall)
+    return O.Problem(case.mesh, case.bc_type, case.qref, p, twall=case.twall)
 
 
 def test_green_gauss_gradient(setup):

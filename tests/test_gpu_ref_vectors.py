@@ -1,0 +1,120 @@
+"""The HIP path against golden vectors written by the reference solver itself
+(tests/golden/ref_*.npz, made by tests/golden/make_ref_vectors.py): every
+phase of an implicit step on the reference's own arrays, the pressure clip
+with chains in the reference's edge order, and the converged steady field of
+both precision forms against the reference's natural-order run."""
+import os
+
+import numpy as np
+import pytest
+
+from proteuscfd_amd import cases
+from proteuscfd_amd.cases import Params
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def with_ref_gas(case, d, **kw):
+    """The case with the reference run's gamma and freestream."""
+    case.params = Params(gamma=float(d["gamma"]), venkat_K=1.0, **kw)
+    case.qref = np.tile(d["qinf"], (len(case.bc_type), 1))
+    case.qinf = d["qinf"].copy()
+    return case
+
+
+PHASE_CASES = {
+    "bump5_phases": (lambda: cases.bump_case(n=5, rcm=False),
+                     dict(sorder=2, limiter=3, cfl=50.0, cfl_start=1.0, cfl_ramp=50, nsgs=8)),
+    "wedge_phases": (lambda: cases.wedge_case(nx=8, ny=4, nz=2, rcm=False),
+                     dict(sorder=2, limiter=2, cfl=2.0, cfl_start=2.0, nsgs=4)),
+}
+
+
+@pytest.mark.parametrize("name", list(PHASE_CASES))
+def test_phases_match_reference(name):
+    from proteuscfd_amd import gpu
+    d = load(name)
+    build, kw = PHASE_CASES[name]
+    case = with_ref_gas(build(), d, **kw)
+    nn = case.mesh.nnode
+    print(f"[ref] {name} nnode={nn}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(d["q_in"])
+    ctx.update_bcs()
+    assert rel(ctx.download_q(), d["q_bc"]) <= 1e-13
+    ctx.upload_q(d["q_bc"])
+    assert rel(ctx.gradient(), d["grad"]) <= 1e-12
+    assert rel(ctx.limiter(), d["limiter"]) <= 1e-12
+    ctx.set_limiter(d["limiter"])
+    b, norms = ctx.residual(it=int(d["iter"]))
+    assert np.linalg.norm(b - d["residual"]) <= 1e-12 * np.linalg.norm(d["residual"])
+    assert rel(norms, d["residual_norms"]) <= 1e-12
+    dt, dtmin = ctx.timestep(float(d["cfl"]))
+    assert rel(dt, d["dt"]) <= 1e-13
+    ctx.set_timestep(d["dt"])
+    ctx.jacobian()
+    M = ctx.download_jacobian(d["ia"], d["ja"])
+    ia = d["ia"]
+    for i in range(nn):
+        scale = np.abs(d["jacobian"][ia[i]:ia[i + 1]]).max()
+        assert np.abs(M[ia[i]:ia[i + 1]] - d["jacobian"][ia[i]:ia[i + 1]]).max() <= 1e-6 * scale, i
+    ctx.close()
+
+
+@pytest.mark.parametrize("lim", [1, 2, 3])
+def test_pressure_clip_chains_match_reference(lim):
+    """The limiter field after the reference's sequential pressure clip
+    (ucs/limiters.tcc:736-815) on a state where the clip zeroes nodes in
+    chains: equal node for node, zeroed set included."""
+    from proteuscfd_amd import gpu
+    d = load("bump6_clip")
+    case = with_ref_gas(cases.bump_case(n=6, rcm=False), d, sorder=2, limiter=lim)
+    print(f"[ref] clip limiter={lim}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(d["q_bc"])
+    g = ctx.gradient()
+    assert rel(g, d["grad"]) <= 1e-12
+    lg = ctx.limiter()
+    ref = d[f"limiter_{lim}"]
+    np.testing.assert_array_equal((lg == 0).all(axis=1), (ref == 0).all(axis=1))
+    assert (ref == 0).all(axis=1).sum() >= 5
+    assert rel(lg, ref) <= 1e-12
+    ctx.set_limiter(ref)
+    b, _ = ctx.residual(it=5)
+    assert np.linalg.norm(b - d[f"residual_{lim}"]) <= 1e-12 * np.linalg.norm(d[f"residual_{lim}"])
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_converged_field_matches_reference(precision):
+    """Both precision forms from freestream to an 8-order drop on the 6^3
+    bump (multicolour SGS), against the reference's own converged field
+    (natural-order SGS): per-variable error <= 1e-6 relative, and the
+    number of Newton iterations within a few of the reference's."""
+    from proteuscfd_amd import gpu
+    d = load("bump6_solve")
+    case = with_ref_gas(cases.bump_case(n=6, rcm=False), d, sorder=2, limiter=3, cfl=20.0, cfl_start=1.0,
+                        cfl_ramp=20, nsgs=8, precision=precision)
+    print(f"[ref] converged bump 6^3 precision={precision}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(case.initial_state())
+    ctx.reset_iteration()
+    hist, nit, _ = ctx.solve(400, 1e-8)
+    q = ctx.download_q()[:case.mesh.nnode, :5]
+    qr = d["q_final"][:case.mesh.nnode, :5]
+    err = [float(np.abs(q[:, k] - qr[:, k]).max() / np.abs(qr[:, k]).max()) for k in range(5)]
+    print(f"[ref] iterations gpu {nit} reference {len(d['history'])}, field error per variable {err}", flush=True)
+    assert hist[-1] <= 1e-8 * hist[0]
+    assert abs(nit - len(d["history"])) <= 5
+    assert max(err) <= 1e-6
+    ctx.close()

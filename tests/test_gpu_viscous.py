@@ -39,7 +39,7 @@ CASES = {"isothermal": plate(1.0), "adiabatic": plate(-1.0)}
 def setup(request):
     from proteuscfd_amd import gpu
     case = CASES[request.param]
-    pb = O.Problem.from_case(case, clip_jacobi=True)
+    pb = O.Problem.from_case(case)
     ctx = gpu.Context(case)
     q0 = cases.perturbed_state(case, amp=0.03)
     print(f"[viscous] case={request.param} nnode={case.mesh.nnode} nbedge={case.mesh.nbedge}", flush=True)
@@ -65,7 +65,7 @@ def test_viscous_residual(setup, it):
     second order; the noSlip rows are hard-set in both."""
     name, case, pb, ctx, q0 = setup
     p = Params(**{**case.params.__dict__, "nfirst": 1})
-    pbo = O.Problem(case.mesh, case.bc_type, case.qref, p, clip_jacobi=True, twall=case.twall)
+    pbo = O.Problem(case.mesh, case.bc_type, case.qref, p, twall=case.twall)
     qo = pbo.update_bcs(q0)
     go = pbo.gradient(qo)
     lo = np.ones((case.mesh.nnode, 5))
@@ -80,7 +80,7 @@ def test_viscous_residual(setup, it):
     assert rel(bg, bo) < 1e-12
     # per-node bound against the size of the node's fluxes
     bi, _ = O.Problem(case.mesh, case.bc_type, case.qref, Params(**{**p.__dict__, "viscous": 0}),
-                      clip_jacobi=True, twall=case.twall).residual(qo, go, lo, it=it)
+                      twall=case.twall).residual(qo, go, lo, it=it)
     scale = np.abs(bo - bi).max() + np.abs(bi).max()
     assert np.abs(bg - bo).max() <= 1e-12 * scale
     np.testing.assert_allclose(ng, no, rtol=1e-11)
@@ -93,7 +93,7 @@ def test_viscous_jacobian(setup, precision, bnd):
     noSlip wall and farfield / outflow faces."""
     name, case, pb, ctx, q0 = setup
     p = Params(**{**case.params.__dict__, "jac_boundary_type": bnd})
-    pb = O.Problem(case.mesh, case.bc_type, case.qref, p, clip_jacobi=True, twall=case.twall)
+    pb = O.Problem(case.mesh, case.bc_type, case.qref, p, twall=case.twall)
     qo = pb.update_bcs(q0)
     dto, _ = pb.timestep(qo, 5.0)
     ia, ja, iau, Mo, qafter = pb.jacobian(qo, dto)

@@ -239,8 +239,11 @@ def write_su2(path, xyz, elems, bfaces):
 
 def same_mesh(a, b, tag_map=lambda g: g):
     assert (a.nnode, a.nelem, a.nedge, a.nbface, a.nbedge) == (b.nnode, b.nelem, b.nedge, b.nbface, b.nbedge)
-    np.testing.assert_array_equal(a.edges, b.edges)
-    np.testing.assert_allclose(a.edge_a, b.edge_a, rtol=0, atol=1e-14)
+    # edges are numbered in the point-to-point map order, which follows the
+    # file's element order: compare them as a set, each with its area
+    oa, ob = np.lexsort(a.edges.T[::-1]), np.lexsort(b.edges.T[::-1])
+    np.testing.assert_array_equal(a.edges[oa], b.edges[ob])
+    np.testing.assert_allclose(a.edge_a[oa], b.edge_a[ob], rtol=0, atol=1e-14)
     np.testing.assert_allclose(a.vol, b.vol, rtol=1e-14)
     # half-edges follow the file's face order: compare them as a set
     def rows(m, tags):

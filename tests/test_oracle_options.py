@@ -161,7 +161,8 @@ def test_dynamic_cfl_rule():
 
 
 def test_dynamic_cfl_follows_residual_history():
-    """Iteration n uses the residuals of iterations n-1 and n-2.  The
+    """Iteration n uses the residuals of iterations n-1 and n-2; the
+    counter starts at 0, so iterations 0 and 1 run at cflStart.  The
     explicit wedge run overshoots its stability limit, so the residual
     rises and falls and the CFL is capped, cut and raised again."""
     p = Params(sorder=1, cfl=3.0, cfl_start=1.5, dynamic_cfl=1, nsgs=0)
@@ -175,10 +176,9 @@ def test_dynamic_cfl_follows_residual_history():
         hist.append(norms[0])
         cfls.append(so.s.cfl)
     so.close()
-    expect = [1.5]
-    for n in range(2, nit + 1):
-        r1 = hist[n - 2]
-        r2 = hist[n - 3] if n >= 3 else 0.0
+    expect = [1.5, 1.5]
+    for n in range(2, nit):
+        r1, r2 = hist[n - 1], hist[n - 2]
         delta = (r2 - r1) / r2 if r2 > 1e-21 else 9999.0
         prev = expect[-1]
         expect.append(min(max(0.0, prev * (1.1 if delta > 0 else 0.9)), 3.0))

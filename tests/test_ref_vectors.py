@@ -1,0 +1,200 @@
+"""The oracle against golden vectors written by the reference solver itself
+(tests/golden/ref_*.npz, made by tests/golden/make_ref_vectors.py from the
+reference's ucs sources compiled in the build container): mesh metrics,
+every phase of an implicit step, the pressure clip with chains, boundary
+states and ApplyDQ on all their branches, and a converged steady solve.
+Runs on the CPU; the reference is not needed here."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import cases
+from proteuscfd_amd.cases import Params
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def bump(n):
+    return cases.bump_case(n=n, rcm=False)
+
+
+def wedge():
+    return cases.wedge_case(nx=8, ny=4, nz=2, rcm=False)
+
+
+CASES = {
+    # name: (case builder, params of the fixture's run)
+    "bump5_phases": (lambda: bump(5), dict(sorder=2, limiter=3, cfl=50.0, cfl_start=1.0, cfl_ramp=50, nsgs=8)),
+    "wedge_phases": (wedge, dict(sorder=2, limiter=2, cfl=2.0, cfl_start=2.0, nsgs=4)),
+}
+
+
+def problem(case, d, clip_jacobi=False, **kw):
+    p = Params(gamma=float(d["gamma"]), venkat_K=1.0, **kw)
+    qref = np.tile(d["qinf"], (len(case.bc_type), 1))
+    return O.Problem(case.mesh, case.bc_type, qref, p, clip_jacobi=clip_jacobi)
+
+
+@pytest.mark.parametrize("name", ["bump5_phases", "wedge_phases"])
+def test_mesh_matches_reference(name):
+    """Edges (in the reference's point-to-point order), half-edges, dual
+    areas and volumes of the generator's mesh against those the reference
+    builds from the same partition file (ucs/mesh.tcc:517-2333)."""
+    d = load(name)
+    m = CASES[name][0]().mesh
+    np.testing.assert_array_equal(m.edges, d["mesh_edges"])
+    np.testing.assert_array_equal(m.bedges, d["mesh_bedges"])
+    np.testing.assert_array_equal(m.bfactag, d["mesh_bfactag"])
+    np.testing.assert_array_equal(m.xyz, d["mesh_xyz"])
+    assert np.abs(m.edge_a - d["mesh_edge_a"]).max() <= 1e-14 * max(1.0, np.abs(d["mesh_edge_a"]).max())
+    assert np.abs(m.bedge_a - d["mesh_bedge_a"]).max() <= 1e-14
+    assert rel(m.vol, d["mesh_vol"]) <= 1e-14
+
+
+@pytest.mark.parametrize("name", ["bump5_phases", "wedge_phases"])
+def test_phases_match_reference(name):
+    """One implicit step phase by phase (UpdateBCs, gradient, limiter,
+    residual, time step, Jacobian, SGS, ApplyDQ; ucs/solutionSpace.tcc:
+    639-804) against the reference's own arrays on the same state."""
+    d = load(name)
+    build, kw = CASES[name]
+    case = build()
+    m = case.mesh
+    pb = problem(case, d, **kw)
+    nn = m.nnode
+    # boundary phantoms (and hard-set interior nodes)
+    qbc = pb.update_bcs(d["q_in"])
+    assert rel(qbc, d["q_bc"]) <= 1e-13
+    q = d["q_bc"]
+    assert rel(pb.gradient(q), d["grad"]) <= 1e-12
+    lim = pb.limiter(q, d["grad"])
+    assert rel(lim, d["limiter"]) <= 1e-12
+    b, norms = pb.residual(q, d["grad"], d["limiter"], it=int(d["iter"]))
+    assert np.linalg.norm(b - d["residual"]) <= 1e-12 * np.linalg.norm(d["residual"])
+    assert rel(norms, d["residual_norms"]) <= 1e-12
+    dt, dtmin = pb.timestep(q, float(d["cfl"]))
+    assert rel(dt, d["dt"]) <= 1e-13 and abs(dtmin - float(d["dtmin"])) <= 1e-13 * float(d["dtmin"])
+    # Jacobian blocks by (row, column): the reference stores a row's
+    # neighbours in point-to-point order, the oracle in ascending order
+    ia, ja, _, M, _ = pb.jacobian(q, d["dt"])
+    where = {(i, int(ja[k])): k for i in range(nn) for k in range(ia[i], ia[i + 1])}
+    ia_r, ja_r, M_r = d["ia"], d["ja"], d["jacobian"]
+    assert ia_r[-1] == len(ja) == len(ja_r)
+    # FD with h = 1e-8 keeps about 8 digits of an entry; blocks that vanish
+    # in exact arithmetic (supersonic upwinding) hold rounding noise, so the
+    # bar is taken on the row's scale
+    for i in range(nn):
+        scale = np.abs(M_r[ia_r[i]:ia_r[i + 1]]).max()
+        for k in range(ia_r[i], ia_r[i + 1]):
+            blk = M[where[(i, int(ja_r[k]))]]
+            assert np.abs(blk - M_r[k]).max() <= 1e-6 * scale, (i, ja_r[k])
+    # SGS (natural row order) on the reference's own matrix and right-hand side
+    x = O.sgs_on_matrix(ia_r, ja_r, ia_r[:-1].copy(), M_r, d["residual"], int(d["nsgs"]))
+    assert rel(x, d["dq"]) <= 1e-12
+    # the update with the reference's correction
+    qn = q.copy()
+    for i in range(nn):
+        row = np.ascontiguousarray(qn[i])
+        dq = np.ascontiguousarray(d["dq"][i])
+        O.lib().o_apply_dq(dq.ctypes.data_as(O.dp), row.ctypes.data_as(O.dp), float(d["gamma"]))
+        qn[i] = row
+    assert rel(qn[:nn], d["q_new"][:nn]) <= 1e-15
+
+
+@pytest.mark.parametrize("lim", [1, 2, 3])
+def test_pressure_clip_matches_reference(lim):
+    """Barth, Venkat and modified Venkat on a state whose centre is strongly
+    disturbed: the pressure clip zeroes limiters edge by edge in the
+    reference's edge order, an earlier zeroing changing a later edge's
+    extrapolation (ucs/limiters.tcc:736-815).  The oracle's sequential form
+    reproduces the reference's limiter field exactly; its Jacobi form (every
+    edge seeing the pre-clip limiter) differs, which shows the chains are
+    there."""
+    d = load("bump6_clip")
+    case = bump(6)
+    pb = problem(case, d, sorder=2, limiter=lim)
+    q, grad = d["q_bc"], d["grad"]
+    assert rel(pb.update_bcs(d["q_in"]), q) <= 1e-13
+    assert rel(pb.gradient(q), grad) <= 1e-12
+    ref = d[f"limiter_{lim}"]
+    l_seq = pb.limiter(q, grad)
+    zeroed = (ref == 0).all(axis=1)
+    assert zeroed.sum() >= 5
+    np.testing.assert_array_equal((l_seq == 0).all(axis=1), zeroed)
+    assert rel(l_seq, ref) <= 1e-12
+    b, _ = pb.residual(q, grad, ref, it=5)
+    assert np.linalg.norm(b - d[f"residual_{lim}"]) <= 1e-12 * np.linalg.norm(d[f"residual_{lim}"])
+    if lim == 3:
+        l_jac = problem(case, d, clip_jacobi=True, sorder=2, limiter=lim).limiter(q, grad)
+        assert (l_jac == 0).all(axis=1).sum() > zeroed.sum()
+
+
+def test_boundary_states_and_apply_dq_match_reference():
+    """GetFarfieldBoundaryVariables on all four characteristic branches with
+    the interior far from the freestream and a phantom that starts elsewhere,
+    GetInviscidWallBoundaryVariables with non-zero normal velocity
+    (ucs/compressible.tcc:1245-1472), and ApplyDQ with density, energy and
+    pressure driven below zero (:928-993), against the reference's eqnset."""
+    d = load("pointwise")
+    g = float(d["gamma"])
+    recs, out = d["records"], d["out"]
+    branches = set()
+    lib = O.lib()
+    for r, o in zip(recs, out):
+        kind = int(r[0])
+        if kind in (0, 1):
+            QL, QR = r[1:11].copy(), r[11:21].copy()
+            a = r[21:25].copy()
+            bct = cases.BC_FARFIELD if kind == 0 else cases.BC_IMPERMEABLE
+            lib.o_bc_state(bct, QL.ctypes.data_as(O.dp), QR.ctypes.data_as(O.dp), d["qinf"].ctypes.data_as(O.dp),
+                           a.ctypes.data_as(O.dp), g)
+            assert np.isfinite(o[:5]).all() and rel(QR[:5], o[:5]) <= 1e-13, r
+            if kind == 0:
+                avg = 0.5 * (r[1:6] + r[11:16])
+                theta = np.dot(avg[1:4] / avg[0], a[:3])
+                p = (g - 1.0) * (avg[4] - 0.5 * np.dot(avg[1:4], avg[1:4]) / avg[0])
+                branches.add((theta > 0, abs(theta) >= np.sqrt(g * p / avg[0])))
+        else:
+            Q, dQ = r[6:16].copy(), r[1:6].copy()
+            lib.o_apply_dq(dQ.ctypes.data_as(O.dp), Q.ctypes.data_as(O.dp), g)
+            assert rel(Q, o) <= 1e-15, r
+    assert len(branches) == 4
+    # the ApplyDQ records reach each clip: density, energy and pressure
+    q_ap = out[recs[:, 0] == 2]
+    assert (q_ap[:, 0] == 1e-10).any() and (q_ap[:, 4] == 1e-10).any()
+
+
+def test_steady_solve_matches_reference():
+    """The oracle's steady loop, SGS in natural row order, against the
+    reference's own run from freestream to an 8-order drop on a 6^3 bump:
+    the same number of Newton iterations, the residual history and the
+    converged field."""
+    d = load("bump6_solve")
+    case = bump(6)
+    g = float(d["gamma"])
+    pb = problem(case, d, sorder=2, limiter=3, cfl=20.0, cfl_start=1.0, cfl_ramp=20, nsgs=8)
+    q0 = np.tile(d["qinf"], (case.mesh.nnode + case.mesh.nbedge, 1))
+    s = O.Solver(pb, q0)
+    hist = []
+    for _ in range(len(d["history"])):
+        hist.append(s.step()[0])
+    href = d["history"]
+    hist = np.array(hist)
+    # the same iterate, up to the rounding of the block-row order inside SGS
+    assert np.all(np.abs(hist - href) <= 1e-6 * href)
+    assert hist[-1] <= 1e-8 * hist[0]
+    qn, qr = s.q[:case.mesh.nnode, :5], d["q_final"][:case.mesh.nnode, :5]
+    for k in range(5):
+        assert np.abs(qn[:, k] - qr[:, k]).max() <= 1e-10 * np.abs(qr[:, k]).max(), k
+    s.close()
+    assert g > 1.39
