@@ -51,7 +51,7 @@ def euler_keys(**kw):
 class Run:
     """One harness invocation in a scratch directory."""
 
-    def __init__(self, case, keys, bc, nsteps=1, name="c"):
+    def __init__(self, case, keys, bc, nsteps=1, name="c", extra=None):
         self.case, self.name = case, name
         self.dir = tempfile.mkdtemp(prefix="pcfd_ref_") + "/"
         case.mesh.write_h5(self.dir + name, 1)
@@ -59,7 +59,7 @@ class Run:
             f.write(param_text(name, keys, nsteps))
         with open(self.dir + name + ".bc", "w") as f:
             for tag, typ in bc.items():
-                f.write(f"surface #{tag} = {BC_NAMES[typ]}\n")
+                f.write(f"surface #{tag} = {BC_NAMES[typ]}{(extra or {}).get(tag, '')}\n")
 
     def __call__(self, mode, *args):
         exe = build_ref.harness_path()
@@ -103,7 +103,7 @@ def phases(run, q, it, nsgs, nvars=10):
            "q_new": run.arr("q_new").reshape(nt, nvars)}
     if os.path.exists(run.dir + "ref_limiter.bin"):
         out["limiter"] = run.arr("limiter").reshape(nn, -1)
-    for k in ["gamma", "cfl", "dtmin", "sgs_delta", "Re", "ref_temperature"]:
+    for k in ["gamma", "cfl", "dtmin", "sgs_delta", "Re", "ref_temperature", "ref_velocity"]:
         out[k] = np.float64(run.meta[k])
     out["iter"] = np.int32(it)
     out["nsgs"] = np.int32(nsgs)
@@ -276,8 +276,66 @@ def make_bump6_solve():
     return out
 
 
+def plate(twall, nx=12, ny=3, nz=10, nlayers=4):
+    """The laminar flat-plate parity channel: nx x ny x nz hexes, prism
+    layers on the noSlip floor (isothermal at twall T_ref, adiabatic for
+    twall < 0), farfield inflow and top at 1 degree of incidence, sonic
+    outflow, symmetry sides."""
+    return cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=nlayers, first_h=0.01, growth=1.3, twall=twall,
+                            incidence_deg=1.0, outflow_bc=cases.BC_SONIC_OUTFLOW)
+
+
+def plate_bc(twall):
+    bc = {1: 6, 2: 8, 3: 9, 4: 9, 5: 4, 6: 6}
+    # twall in K; a negative value is the adiabatic wall (the default, 1 K, is isothermal)
+    return bc, {5: f" twall = [{300.0 * twall if twall > 0 else -1.0:.17g}]"}
+
+
+def ns_keys(**kw):
+    a = np.deg2rad(1.0)
+    # refViscosity sets Re = rho_inf rho_ref u_ref V L / mu_ref near 1e4 at M 0.3
+    return euler_keys(equationSet="compressibleNS", flowDirection=f"[{np.cos(a):.17g}, 0.0, {np.sin(a):.17g}]",
+                      initialTemperature=300, refViscosity=0.01225, velocity=0.3, **kw)
+
+
+def make_plate_phases():
+    """Laminar Navier-Stokes, second order, unlimited, on the plate channel
+    with an isothermal and an adiabatic noSlip floor: BCs (strong no-slip
+    state), gradient, viscous + inviscid residual, time step, FD + analytic
+    viscous Jacobian with the hard-set wall rows, SGS and ApplyDQ."""
+    out = {}
+    for tag, tw in (("iso", 1.0), ("adi", -1.0)):
+        case = plate(tw, nx=6, ny=2, nz=6, nlayers=3)
+        bc, extra = plate_bc(tw)
+        run = Run(case, ns_keys(spatialOrder=2, limiter=0, numberSGS=4, CFL=10.0, startingCFL=10.0), bc, extra=extra)
+        gamma = ref_gamma(run)
+        q = cases.aux(cases.perturbed_state(case, amp=0.03, seed=5).copy(), gamma)
+        r = phases(run, q, 3, 4)
+        for k, v in r.items():
+            out[f"{tag}_{k}"] = v
+        if tag == "iso":
+            out.update(run.mesh_arrays())
+    return out
+
+
+def make_plate_solve():
+    """The reference's own steady loop on the isothermal plate channel (laminar
+    NS, second order, unlimited MUSCL, 8 SGS, CFL 1 -> 50 over 30 iterations)
+    to an 8-order drop."""
+    case = plate(1.0)
+    bc, extra = plate_bc(1.0)
+    run = Run(case, ns_keys(spatialOrder=2, limiter=0, numberSGS=8, CFL=50.0, startingCFL=1.0, rampCFL=30), bc,
+              nsteps=3000, extra=extra)
+    run("solve", 3000, 1e-8)
+    m = case.mesh
+    return {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 10),
+            "gamma": np.float64(run.meta["gamma"]), "qinf": run.arr("qinf"), "Re": np.float64(run.meta["Re"]),
+            "ref_temperature": np.float64(run.meta["ref_temperature"])}
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
-          "pointwise": make_pointwise, "bump6_solve": make_bump6_solve}
+          "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
+          "plate_solve": make_plate_solve}
 
 
 def main(names):

@@ -118,3 +118,69 @@ def test_converged_field_matches_reference(precision):
     assert abs(nit - len(d["history"])) <= 5
     assert max(err) <= 1e-6
     ctx.close()
+
+
+def plate(twall, nx=12, ny=3, nz=10, nlayers=4):
+    return cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=nlayers, first_h=0.01, growth=1.3, twall=twall,
+                            incidence_deg=1.0, outflow_bc=cases.BC_SONIC_OUTFLOW)
+
+
+def with_ref_ns(case, d, pre="", **kw):
+    case.params = Params(gamma=float(d[pre + "gamma"]), viscous=1, reynolds=float(d[pre + "Re"]), mach_ref=0.3,
+                         sutherland_s=110.4 / float(d[pre + "ref_temperature"]), **kw)
+    case.qref = np.tile(d[pre + "qinf"], (len(case.bc_type), 1))
+    case.qinf = d[pre + "qinf"].copy()
+    return case
+
+
+@pytest.mark.parametrize("tag,twall", [("iso", 1.0), ("adi", -1.0)])
+def test_viscous_phases_match_reference(tag, twall):
+    from proteuscfd_amd import gpu
+    d = load("plate_phases")
+    g = lambda k: d[f"{tag}_{k}"]
+    case = with_ref_ns(plate(twall, nx=6, ny=2, nz=6, nlayers=3), d, pre=f"{tag}_", sorder=2, limiter=0, cfl=10.0,
+                       cfl_start=10.0, nsgs=4)
+    nn = case.mesh.nnode
+    print(f"[ref] plate {tag} nnode={nn}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(g("q_in"))
+    ctx.update_bcs()
+    assert rel(ctx.download_q(), g("q_bc")) <= 1e-13
+    ctx.upload_q(g("q_bc"))
+    assert rel(ctx.gradient(), g("grad")) <= 1e-12
+    ctx.set_limiter(np.ones((nn, 5)))
+    b, _ = ctx.residual(it=int(g("iter")))
+    assert np.linalg.norm(b - g("residual")) <= 1e-12 * np.linalg.norm(g("residual"))
+    dt, _ = ctx.timestep(float(g("cfl")))
+    assert rel(dt, g("dt")) <= 1e-13
+    ctx.set_timestep(g("dt"))
+    ctx.jacobian()
+    M = ctx.download_jacobian(g("ia"), g("ja"))
+    ia, Mr = g("ia"), g("jacobian")
+    for i in range(nn):
+        scale = np.abs(Mr[ia[i]:ia[i + 1]]).max()
+        assert np.abs(M[ia[i]:ia[i + 1]] - Mr[ia[i]:ia[i + 1]]).max() <= 1e-6 * scale, i
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_viscous_converged_field_matches_reference(precision):
+    """Laminar NS plate channel, both precision forms, to an 8-order drop,
+    against the reference's converged field: <= 1e-6 per variable."""
+    from proteuscfd_amd import gpu
+    d = load("plate_solve")
+    case = with_ref_ns(plate(1.0), d, sorder=2, limiter=0, cfl=50.0, cfl_start=1.0, cfl_ramp=30, nsgs=8,
+                       precision=precision)
+    print(f"[ref] converged plate precision={precision}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(case.initial_state())
+    ctx.reset_iteration()
+    hist, nit, _ = ctx.solve(2000, 1e-8)
+    q = ctx.download_q()[:case.mesh.nnode, :5]
+    qr = d["q_final"][:case.mesh.nnode, :5]
+    err = [float(np.abs(q[:, k] - qr[:, k]).max() / np.abs(qr[:, k]).max()) for k in range(5)]
+    print(f"[ref] plate iterations gpu {nit} reference {len(d['history'])}, field error per variable {err}",
+          flush=True)
+    assert hist[-1] <= 1e-8 * hist[0]
+    assert max(err) <= 1e-6
+    ctx.close()

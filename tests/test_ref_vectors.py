@@ -198,3 +198,67 @@ def test_steady_solve_matches_reference():
         assert np.abs(qn[:, k] - qr[:, k]).max() <= 1e-10 * np.abs(qr[:, k]).max(), k
     s.close()
     assert g > 1.39
+
+
+def plate(twall, nx=12, ny=3, nz=10, nlayers=4):
+    return cases.plate_case(nx=nx, ny=ny, nz=nz, nlayers=nlayers, first_h=0.01, growth=1.3, twall=twall,
+                            incidence_deg=1.0, outflow_bc=cases.BC_SONIC_OUTFLOW)
+
+
+def ns_problem(case, d, pre="", **kw):
+    p = Params(gamma=float(d[pre + "gamma"]), viscous=1, reynolds=float(d[pre + "Re"]), mach_ref=0.3,
+               sutherland_s=110.4 / float(d[pre + "ref_temperature"]), **kw)
+    qref = np.tile(d[pre + "qinf"], (len(case.bc_type), 1))
+    return O.Problem(case.mesh, case.bc_type, qref, p, twall=case.twall)
+
+
+@pytest.mark.parametrize("tag,twall", [("iso", 1.0), ("adi", -1.0)])
+def test_viscous_phases_match_reference(tag, twall):
+    """Laminar Navier-Stokes on the plate channel, isothermal and adiabatic
+    noSlip floor: strong no-slip states, gradient, inviscid + viscous
+    residual with the wall-row edits, time step, FD + analytic viscous
+    Jacobian with the hard-set wall rows, and SGS on the reference's matrix
+    (ucs/residual.tcc:389-562, ucs/compressible.tcc:1544-1893,
+    ucs/bc.tcc:747-1053)."""
+    d = load("plate_phases")
+    case = plate(twall, nx=6, ny=2, nz=6, nlayers=3)
+    m = case.mesh
+    nn = m.nnode
+    if tag == "iso":
+        np.testing.assert_array_equal(m.edges, d["mesh_edges"])
+        assert rel(m.vol, d["mesh_vol"]) <= 1e-14
+    g = lambda k: d[f"{tag}_{k}"]
+    pb = ns_problem(case, d, pre=f"{tag}_", sorder=2, limiter=0, cfl=10.0, cfl_start=10.0, nsgs=4)
+    assert rel(pb.update_bcs(g("q_in")), g("q_bc")) <= 1e-13
+    q = g("q_bc")
+    assert rel(pb.gradient(q), g("grad")) <= 1e-12
+    b, norms = pb.residual(q, g("grad"), np.ones((nn, 5)), it=int(g("iter")))
+    assert np.linalg.norm(b - g("residual")) <= 1e-12 * np.linalg.norm(g("residual"))
+    dt, _ = pb.timestep(q, float(g("cfl")))
+    assert rel(dt, g("dt")) <= 1e-13
+    ia, ja, _, M, _ = pb.jacobian(q, g("dt"))
+    where = {(i, int(ja[k])): k for i in range(nn) for k in range(ia[i], ia[i + 1])}
+    ia_r, ja_r, M_r = g("ia"), g("ja"), g("jacobian")
+    for i in range(nn):
+        scale = np.abs(M_r[ia_r[i]:ia_r[i + 1]]).max()
+        for k in range(ia_r[i], ia_r[i + 1]):
+            assert np.abs(M[where[(i, int(ja_r[k]))]] - M_r[k]).max() <= 1e-6 * scale, (i, ja_r[k])
+    x = O.sgs_on_matrix(ia_r, ja_r, ia_r[:-1].copy(), M_r, g("residual"), int(g("nsgs")))
+    assert rel(x, g("dq")) <= 1e-12
+
+
+def test_viscous_steady_solve_matches_reference():
+    """The oracle's steady loop against the reference's own run on the
+    isothermal plate channel (laminar NS) to an 8-order drop."""
+    d = load("plate_solve")
+    case = plate(1.0)
+    pb = ns_problem(case, d, sorder=2, limiter=0, cfl=50.0, cfl_start=1.0, cfl_ramp=30, nsgs=8)
+    s = O.Solver(pb, np.tile(d["qinf"], (case.mesh.nnode + case.mesh.nbedge, 1)))
+    href = d["history"]
+    hist = np.array([s.step()[0] for _ in range(len(href))])
+    assert np.all(np.abs(hist - href) <= 1e-5 * href)
+    assert hist[-1] <= 1e-8 * hist[0]
+    qn, qr = s.q[:case.mesh.nnode, :5], d["q_final"][:case.mesh.nnode, :5]
+    for k in range(5):
+        assert np.abs(qn[:, k] - qr[:, k]).max() <= 1e-9 * np.abs(qr[:, k]).max(), k
+    s.close()
