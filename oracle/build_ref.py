@@ -1,6 +1,8 @@
 """Build recipe for oracle/_ref/ref_harness -- the reference solver itself,
 compiled from its own sources where they lie under /root/reference, driven by
-oracle/ref_harness.cpp.  TEST INFRASTRUCTURE ONLY: used in the build container
+oracle/ref_harness.cpp, which also compiles the reference-side binding
+(proteuscfd_amd/examples/ucs_binding.hpp) against the reference's headers.
+TEST INFRASTRUCTURE ONLY: used in the build container
 to write golden vectors (tests/golden/make_ref_vectors.py); it never travels
 to or runs on the GPU box, and the product never links it.
 
@@ -50,12 +52,14 @@ def harness_path() -> str:
 
 def _flags():
     return ["-std=c++11", "-O2", "-include", "limits", "-w", "-isystem", os.path.join(CONDA, "include"),
+            "-I", os.path.join(os.path.dirname(HERE), "include"),
             "-I", os.path.join(REF, "ucs"), "-I", os.path.join(REF, "structuralDynamics"),
             "-I", os.path.join(REF, "TPLs", "tinyxml", "tinyxml-2.6.2")]
 
 
-def _compile(src, obj):
-    if os.path.exists(obj) and os.path.getmtime(obj) >= os.path.getmtime(src):
+def _compile(src, obj, deps=()):
+    newest = max([os.path.getmtime(src)] + [os.path.getmtime(d) for d in deps])
+    if os.path.exists(obj) and os.path.getmtime(obj) >= newest:
         return
     r = subprocess.run(["g++", *_flags(), "-c", src, "-o", obj + ".tmp"], capture_output=True, text=True)
     if r.returncode != 0:
@@ -81,8 +85,11 @@ def build(jobs: int = 8) -> str | None:
     jobs_list += [(os.path.join(REF, "structuralDynamics", s), os.path.join(objdir, "sd_" + s[:-4] + ".o"))
                   for s in SD_SRC]
     jobs_list.append((os.path.join(HERE, "ref_harness.cpp"), os.path.join(objdir, "ref_harness.o")))
+    binding = [os.path.join(os.path.dirname(HERE), "proteuscfd_amd", "examples", "ucs_binding.hpp"),
+               os.path.join(os.path.dirname(HERE), "include", "pcfd.h")]
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        list(ex.map(lambda a: _compile(*a), jobs_list))
+        list(ex.map(lambda a: _compile(*a, deps=binding if a[0].endswith("ref_harness.cpp") else ()),
+                    jobs_list))
     out = harness_path()
     cmd = ["g++", *[o for _, o in jobs_list], os.path.join(libdir, "libmpi.so.12"),
            os.path.join(libdir, "libhdf5.so.103"), "-lz", "-Wl,-rpath," + libdir, "-o", out + ".tmp"]

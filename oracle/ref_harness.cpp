@@ -11,6 +11,8 @@
 //        (ucs/solutionSpace.tcc:639-804) and writes every intermediate array.
 //   mode "pointwise": reads <dir>/pw_in.bin records and evaluates the
 //        eqnset's boundary states and ApplyDQ on each (see below).
+//   mode "describe": the C-ABI descriptors the reference-side binding
+//        (proteuscfd_amd/examples/ucs_binding.hpp) builds from this space.
 //   mode "solve" <maxiter> <drop>: the reference's own steady loop
 //        (ucs/solve.tcc:24-66: PreIterate, then PreTimeAdvance /
 //        NewtonIterate / PostTimeAdvance) from freestream until residGlobal
@@ -48,6 +50,8 @@
 #include "crs.h"
 #include "h5layer.h"
 #include <mpi.h>
+
+#include "../proteuscfd_amd/examples/ucs_binding.hpp"
 
 namespace {
 
@@ -215,6 +219,30 @@ int mode_pointwise(SolutionSpace<Real>* s) {
   return 0;
 }
 
+int mode_describe(SolutionSpace<Real>* s) {
+  pcfd_binding::MeshArrays a;
+  if (!pcfd_binding::DescribeMesh(s, a)) return 2;
+  const pcfd_mesh_desc& d = a.desc;
+  dump("desc_edges", d.edges, 2 * (size_t)d.nedge);
+  dump("desc_edge_a", d.edge_a, 4 * (size_t)d.nedge);
+  dump("desc_bedges", d.bedges, 2 * (size_t)d.nbnode);
+  dump("desc_bedge_a", d.bedge_a, 4 * (size_t)d.nbnode);
+  dump("desc_bfactag", d.bfactag, (size_t)d.nbnode);
+  dump("desc_bc_type", d.bc_type, (size_t)d.nfactag);
+  dump("desc_qref", d.qref, 5 * (size_t)d.nfactag);
+  dump("desc_twall", d.twall, (size_t)d.nfactag);
+  const pcfd_params p = pcfd_binding::DescribeParams(s, PCFD_CHOSEN_FP32);
+  const double v[] = {p.gamma, (double)p.sorder, (double)p.nfirst, p.chi, (double)p.limiter, p.venkat_K,
+                      (double)p.limiter_refresh, (double)p.limiter_freeze, p.cfl, p.cfl_start, (double)p.cfl_ramp,
+                      (double)p.nsgs, (double)p.symmetry2d, (double)p.jac_freq, (double)p.precision,
+                      (double)p.gradient_terms, (double)p.viscous, p.re_tilde, p.prandtl, p.prandtl_t,
+                      p.sutherland_s, (double)p.gradient_type, (double)p.global_dt, p.fixed_dt,
+                      (double)p.enable_vnn, p.vnn, (double)p.dynamic_cfl, (double)p.jac_field_type,
+                      (double)p.flux_type, (double)p.jac_boundary_type};
+  dump("desc_params", v, sizeof(v) / sizeof(v[0]));
+  return 0;
+}
+
 int mode_solve(SolutionSpace<Real>* s, Int maxiter, Real drop) {
   std::vector<Real> hist;
   s->PreIterate();
@@ -238,7 +266,7 @@ int mode_solve(SolutionSpace<Real>* s, Int maxiter, Real drop) {
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::cerr << "usage: ref_harness <dir/> <casename> phases|pointwise|solve [args]" << std::endl;
+    std::cerr << "usage: ref_harness <dir/> <casename> phases|pointwise|describe|solve [args]" << std::endl;
     return 1;
   }
   MPI_Init(&argc, &argv);
@@ -272,6 +300,7 @@ int main(int argc, char** argv) {
   int rc = 1;
   if (mode == "phases" && argc >= 6) rc = mode_phases(s, std::atoi(argv[4]), std::atoi(argv[5]));
   else if (mode == "pointwise") rc = mode_pointwise(s);
+  else if (mode == "describe") rc = mode_describe(s);
   else if (mode == "solve" && argc >= 6) rc = mode_solve(s, std::atoi(argv[4]), std::atof(argv[5]));
   else std::cerr << "unknown mode or missing arguments" << std::endl;
   g_meta.close();

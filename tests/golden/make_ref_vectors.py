@@ -333,9 +333,33 @@ def make_plate_solve():
             "ref_temperature": np.float64(run.meta["ref_temperature"])}
 
 
+def make_binding():
+    """The descriptors the reference-side binding (proteuscfd_amd/examples/
+    ucs_binding.hpp, compiled against the reference's headers) builds from
+    the reference's own SolutionSpace: a bump (farfield + slip wall, limiter
+    3) and the isothermal plate (laminar NS, noSlip at 300 K)."""
+    out = {}
+    specs = [("bump", cases.bump_case(n=4, rcm=False), euler_keys(spatialOrder=2, limiter=3, numberSGS=8, CFL=50.0,
+                                                                   startingCFL=1.0, rampCFL=50, velocity=0.5),
+              bump_bc(), {}),
+             ("plate", plate(1.0, nx=6, ny=2, nz=6, nlayers=3),
+              ns_keys(spatialOrder=2, limiter=0, numberSGS=4, CFL=10.0, startingCFL=1.0, rampCFL=7), *plate_bc(1.0))]
+    for tag, case, keys, bc, extra in specs:
+        run = Run(case, keys, bc, extra=extra)
+        run("describe")
+        for k in ["edges", "bedges", "bfactag", "bc_type"]:
+            out[f"{tag}_{k}"] = run.arr("desc_" + k, np.int32)
+        for k in ["edge_a", "bedge_a", "qref", "twall", "params"]:
+            out[f"{tag}_{k}"] = run.arr("desc_" + k)
+        for k in ["gamma", "Re", "ref_temperature"]:
+            out[f"{tag}_{k}"] = np.float64(run.meta[k])
+        out[f"{tag}_qinf"] = run.arr("qinf")
+    return out
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
           "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
-          "plate_solve": make_plate_solve}
+          "plate_solve": make_plate_solve, "binding": make_binding}
 
 
 def main(names):

@@ -262,3 +262,39 @@ def test_viscous_steady_solve_matches_reference():
     for k in range(5):
         assert np.abs(qn[:, k] - qr[:, k]).max() <= 1e-9 * np.abs(qr[:, k]).max(), k
     s.close()
+
+
+@pytest.mark.parametrize("tag", ["bump", "plate"])
+def test_reference_side_binding_descriptors(tag):
+    """The binding a ucs.x maintainer adds (proteuscfd_amd/examples/
+    ucs_binding.hpp), compiled against the reference's headers and run on the
+    reference's own SolutionSpace, fills pcfd_mesh_desc and pcfd_params with
+    what this repository's cases hand the library: the same edges,
+    half-edges, metrics, BC table, wall temperatures and parameters."""
+    from proteuscfd_amd import gpu
+    d = load("binding")
+    g = lambda k: d[f"{tag}_{k}"]
+    if tag == "bump":
+        case = bump(4)
+        p = Params(gamma=float(g("gamma")), sorder=2, limiter=3, venkat_K=1.0, cfl=50.0, cfl_start=1.0,
+                   cfl_ramp=50, nsgs=8, precision=1)
+    else:
+        case = plate(1.0, nx=6, ny=2, nz=6, nlayers=3)
+        p = Params(gamma=float(g("gamma")), sorder=2, limiter=0, cfl=10.0, cfl_start=1.0, cfl_ramp=7, nsgs=4,
+                   viscous=1, reynolds=float(g("Re")), mach_ref=0.3,
+                   sutherland_s=110.4 / float(g("ref_temperature")), precision=1)
+    m = case.mesh
+    np.testing.assert_array_equal(g("edges").reshape(-1, 2), m.edges)
+    np.testing.assert_array_equal(g("bedges").reshape(-1, 2), m.bedges)
+    np.testing.assert_array_equal(g("bfactag"), m.bfactag)
+    assert np.abs(g("edge_a").reshape(-1, 4) - m.edge_a).max() <= 1e-14 * max(1.0, np.abs(m.edge_a).max())
+    assert np.abs(g("bedge_a").reshape(-1, 4) - m.bedge_a).max() <= 1e-14
+    ntag = len(g("bc_type"))
+    np.testing.assert_array_equal(g("bc_type"), case.bc_type[:ntag])
+    np.testing.assert_array_equal(g("qref").reshape(-1, 5)[1:], np.tile(g("qinf")[:5], (ntag - 1, 1)))
+    tw = case.twall[:ntag] if case.twall is not None else np.full(ntag, -1.0)
+    assert np.allclose(g("twall")[1:], tw[1:], rtol=1e-15, atol=0)
+    cp = gpu.to_cparams(p)
+    names = [f for f, _ in gpu.CParams._fields_]
+    for name, v in zip(names, g("params")):
+        assert getattr(cp, name) == pytest.approx(v, rel=1e-14, abs=0), name
