@@ -177,6 +177,19 @@ int mode_phases(SolutionSpace<Real>* s, Int iter, Int nsgs) {
   Real dd = s->crs->SGS(nsgs, NULL, NULL, NULL);
   meta("sgs_delta", dd);
   dump("dq", s->crs->x, (size_t)nnode * neqn);
+  // NaN/Inf rows of the correction zeroed and counted, then ApplyDQ
+  // (ucs/solutionSpace.tcc:772-804)
+  Int zerocount = 0;
+  for (Int j = 0; j < nnode; ++j) {
+    bool zero = false;
+    for (Int k = 0; k < neqn; ++k)
+      if (std::isnan(s->crs->x[j * neqn + k]) || std::isinf(s->crs->x[j * neqn + k])) zero = true;
+    if (zero) {
+      ++zerocount;
+      for (Int k = 0; k < neqn; ++k) s->crs->x[j * neqn + k] = 0.0;
+    }
+  }
+  meta("zerocount", zerocount);
   for (Int j = 0; j < nnode; ++j) eq->ApplyDQ(&s->crs->x[j * neqn], &s->q[j * nvars], &m->xyz[j * 3]);
   dump_q(s, "q_new");
   return 0;

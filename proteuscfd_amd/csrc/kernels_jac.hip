@@ -243,11 +243,12 @@ __global__ void __launch_bounds__(kBlock) k_jac_boundary(JacArgs A, const int* _
   for (int rr = 0; rr < 5; ++rr) Db[(long long)t * 25 + rr * 5 + c] = Dc[rr];
 }
 
-// the edge flux of the generic form, evaluated from node i's side
+// the edge flux of the generic form, evaluated from node i's side (x is
+// node i's state); a is the canonical n0 -> n1 normal, `left` whether i is n0
 __device__ __forceinline__ void jac_flux(const JacArgs& A, const double* x, const double* y, const double* ai,
-                                         double* F) {
+                                         const double* a, bool left, double* F) {
   if (A.flux_alg) alg_flux(x, y, ai, A.gamma, F);
-  else roe_flux_fast(x, y, ai, A.gamma, A.sym2d, F);
+  else if (roe_flux_fast(x, y, ai, A.gamma, A.sym2d, F) && !left) roe_flux_canonical(y, x, a, A.gamma, A.sym2d, F);
 }
 
 // GENERIC = false: Roe flux, one-sided differences (the reference defaults),
@@ -347,7 +348,10 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
           RoeSide Sc = S0, Sjc;
           roe_side_th(Sc, aic);
           roe_side(qc, aic, gamma, Sjc);
-          roe_flux_s(qi, Sc, qc, Sjc, aic, gamma, A.sym2d, fSg);
+          if (roe_flux_s(qi, Sc, qc, Sjc, aic, gamma, A.sym2d, fSg) && i > jc) {
+            const double ac[4] = {-aic[0], -aic[1], -aic[2], aic[3]};
+            roe_flux_canonical(qc, qi, ac, gamma, A.sym2d, fSg);
+          }
         }
       }
       roe_side_th(S0, ai);
@@ -356,18 +360,21 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
       roe_side(qj, ai, gamma, Sj);
       double fS[5], fP[5];
       if constexpr (VISC) {
-        roe_flux_s(qi, S0, qj, Sj, ai, gamma, A.sym2d, fS);
+        if (roe_flux_s(qi, S0, qj, Sj, ai, gamma, A.sym2d, fS) && !left)
+          roe_flux_canonical(qj, qi, a, gamma, A.sym2d, fS);
       } else {
 #pragma unroll
         for (int k = 0; k < 5; ++k) fS[k] = __shfl(fSg[k], g0 + s % 5);
       }
       __builtin_amdgcn_sched_barrier(0);
-      roe_flux_s(qiP, SP, qj, Sj, ai, gamma, A.sym2d, fP);
+      if (roe_flux_s(qiP, SP, qj, Sj, ai, gamma, A.sym2d, fP) && !left)
+        roe_flux_canonical(qj, qiP, a, gamma, A.sym2d, fP);
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fP[rr] - fS[rr]) * kInvH;
       __builtin_amdgcn_sched_barrier(0);
       roe_side(qjP, ai, gamma, SjP);
-      roe_flux_s(qi, S0, qjP, SjP, ai, gamma, A.sym2d, fP);
+      if (roe_flux_s(qi, S0, qjP, SjP, ai, gamma, A.sym2d, fP) && !left)
+        roe_flux_canonical(qjP, qi, a, gamma, A.sym2d, fP);
 #pragma unroll
       for (int rr = 0; rr < 5; ++rr) offc[rr] = (fP[rr] - fS[rr]) * kInvH;
       __builtin_amdgcn_sched_barrier(0);
@@ -376,20 +383,20 @@ __global__ void __launch_bounds__(kBlock) k_jacobian(JacArgs A, const int* __res
       if (A.centred) {
         double qjM[5];
         perturb(qj, c, -h, qjM);
-        jac_flux(A, qiP, qj, ai, fA);
-        jac_flux(A, qiM, qj, ai, fB);
+        jac_flux(A, qiP, qj, ai, a, left, fA);
+        jac_flux(A, qiM, qj, ai, a, left, fB);
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fA[rr] - fB[rr]) * kInv2H;
-        jac_flux(A, qi, qjP, ai, fA);
-        jac_flux(A, qi, qjM, ai, fB);
+        jac_flux(A, qi, qjP, ai, a, left, fA);
+        jac_flux(A, qi, qjM, ai, a, left, fB);
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) offc[rr] = (fA[rr] - fB[rr]) * kInv2H;
       } else {
-        jac_flux(A, qi, qj, ai, fB);
-        jac_flux(A, qiP, qj, ai, fA);
+        jac_flux(A, qi, qj, ai, a, left, fB);
+        jac_flux(A, qiP, qj, ai, a, left, fA);
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) dcol[rr] = (fA[rr] - fB[rr]) * kInvH;
-        jac_flux(A, qi, qjP, ai, fA);
+        jac_flux(A, qi, qjP, ai, a, left, fA);
 #pragma unroll
         for (int rr = 0; rr < 5; ++rr) offc[rr] = (fA[rr] - fB[rr]) * kInvH;
       }

@@ -97,7 +97,10 @@ PCFD_DEV double hh_fix(double e, double eL, double eR) {
   return (fabs(e) < eps) ? 0.5 * (e * e / eps + eps) : fabs(e);
 }
 
-PCFD_DEV void roe_flux_s(const double* QL, const RoeSide& L, const double* QR, const RoeSide& R, const double* a,
+// Returns true for an unphysical state: a NaN component was zeroed, or a
+// side's sound speed is NaN (negative pressure), where the entropy fix's
+// max() keeps or drops the NaN by argument position.
+PCFD_DEV bool roe_flux_s(const double* QL, const RoeSide& L, const double* QR, const RoeSide& R, const double* a,
                          double gamma, int sym2d, double* F) {
   const double gm1 = gamma - 1.0;
   const double nx = a[0], ny = a[1], nz = a[2];
@@ -140,17 +143,36 @@ PCFD_DEV void roe_flux_s(const double* QL, const RoeSide& L, const double* QR, c
   if (sym2d == 1) F[1] = 0.0;
   else if (sym2d == 2) F[2] = 0.0;
   else if (sym2d == 3) F[3] = 0.0;
+  bool nan = __builtin_isnan(L.c) || __builtin_isnan(R.c);
 #pragma unroll
   for (int i = 0; i < 5; ++i)
-    if (__builtin_isnan(F[i])) F[i] = 0.0;
+    if (__builtin_isnan(F[i])) {
+      F[i] = 0.0;
+      nan = true;
+    }
+  return nan;
 }
 
-PCFD_DEV void roe_flux_fast(const double* QL, const double* QR, const double* a, double gamma, int sym2d,
+PCFD_DEV bool roe_flux_fast(const double* QL, const double* QR, const double* a, double gamma, int sym2d,
                             double* F) {
   RoeSide L, R;
   roe_side(QL, a, gamma, L);
   roe_side(QR, a, gamma, R);
-  roe_flux_s(QL, L, QR, R, a, gamma, sym2d, F);
+  return roe_flux_s(QL, L, QR, R, a, gamma, sym2d, F);
+}
+
+// F(L, R, a) = -F(R, L, -a) holds for the Roe flux of physical states, but
+// not for an unphysical one: the entropy fix's max() keeps or drops a NaN
+// sound speed by argument position, and NaN components are zeroed
+// afterwards.  A flux evaluated from the right node's side on such a state is
+// therefore re-evaluated in the reference's (left, right) order:
+// G = -F(q_left, q_right, a_canonical).
+PCFD_DEV void roe_flux_canonical(const double* q_left, const double* q_right, const double* a_canon, double gamma,
+                                 int sym2d, double* G) {
+  double F[5];
+  roe_flux_fast(q_left, q_right, a_canon, gamma, sym2d, F);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) G[i] = -F[i];
 }
 
 // EqnSet::NumericalFlux with fluxType algebraicFlux (ucs/eqnset.tcc:56-63,

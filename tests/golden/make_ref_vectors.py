@@ -357,9 +357,41 @@ def make_binding():
     return out
 
 
+def make_bump5_failure():
+    """Failure paths on a 5^3 bump at first order: (a) one node with negative
+    pressure and one with negative density -- the Roe flux turns NaN there
+    and NumericalFlux zeroes the NaN components (ucs/eqnset.tcc:74-88),
+    ApplyDQ clips the density (ucs/compressible.tcc:928-993); (b) one node
+    whose state is NaN -- the NaN reaches the correction through the
+    Jacobian's diagonal and NewtonIterate zeroes and counts the NaN rows
+    (ucs/solutionSpace.tcc:772-800)."""
+    case = cases.bump_case(n=5, rcm=False)
+    out = {}
+    for tag in ("neg", "nan"):
+        run = Run(case, euler_keys(spatialOrder=1, limiter=0, numberSGS=4, CFL=5.0, startingCFL=5.0, velocity=0.5),
+                  bump_bc())
+        gamma = ref_gamma(run)
+        q = cases.aux(cases.perturbed_state(case, amp=0.05, seed=3).copy(), gamma)
+        if tag == "neg":
+            r, v = q[62, 0], q[62, 1:4] / q[62, 0]
+            q[62, 4] = 0.5 * r * (v @ v) - 0.3 / (gamma - 1.0)   # pressure -0.3
+            q[100, 0] = -0.2 * q[100, 0]                          # density < 0
+        else:
+            q[62, :5] = np.nan
+        q = cases.aux(q, gamma)
+        r = phases(run, q, 1, 4)
+        keep = ["q_in", "q_bc", "residual", "residual_norms", "dt", "dq", "q_new", "gamma", "qinf", "cfl", "nsgs"]
+        if tag == "neg":
+            keep += ["ia", "ja", "jacobian"]
+        for k in keep:
+            out[f"{tag}_{k}"] = r[k]
+        out[f"{tag}_zerocount"] = np.int32(run.meta["zerocount"])
+    return out
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
           "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
-          "plate_solve": make_plate_solve, "binding": make_binding}
+          "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure}
 
 
 def main(names):

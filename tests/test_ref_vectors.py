@@ -298,3 +298,56 @@ def test_reference_side_binding_descriptors(tag):
     names = [f for f, _ in gpu.CParams._fields_]
     for name, v in zip(names, g("params")):
         assert getattr(cp, name) == pytest.approx(v, rel=1e-14, abs=0), name
+
+
+def apply_update(q, x, gamma):
+    """NewtonIterate's update: NaN/Inf rows of the correction zeroed and
+    counted, then ApplyDQ per node (ucs/solutionSpace.tcc:772-804)."""
+    q = q.copy()
+    x = x.copy()
+    bad = ~np.isfinite(x).all(axis=1)
+    x[bad] = 0.0
+    for i in range(len(x)):
+        row = np.ascontiguousarray(q[i])
+        dq = np.ascontiguousarray(x[i])
+        O.lib().o_apply_dq(dq.ctypes.data_as(O.dp), row.ctypes.data_as(O.dp), gamma)
+        q[i] = row
+    return q, int(bad.sum())
+
+
+def test_failure_paths_match_reference():
+    """A node with negative pressure and one with negative density: the Roe
+    flux is NaN there and NumericalFlux zeroes the NaN components
+    (ucs/eqnset.tcc:74-88); the Jacobian and SGS stay finite; ApplyDQ clips
+    the density to 1e-10 (ucs/compressible.tcc:928-993).  A node whose state
+    is NaN: the residual stays finite, the NaN reaches every row of the
+    correction through SGS, and every row is zeroed and counted
+    (ucs/solutionSpace.tcc:772-800)."""
+    d = load("bump5_failure")
+    case = bump(5)
+    nn = case.mesh.nnode
+    for tag in ("neg", "nan"):
+        g = lambda k: d[f"{tag}_{k}"]
+        pb = problem(case, {"gamma": g("gamma"), "qinf": g("qinf")}, sorder=1, limiter=0, cfl=5.0, cfl_start=5.0,
+                     nsgs=4)
+        qbc = pb.update_bcs(g("q_in"))
+        np.testing.assert_array_equal(np.isnan(qbc), np.isnan(g("q_bc")))
+        assert rel(np.nan_to_num(qbc), np.nan_to_num(g("q_bc"))) <= 1e-13
+        q = g("q_bc")
+        b, _ = pb.residual(q, it=1)
+        assert np.isfinite(b).all() and np.isfinite(g("residual")).all()
+        assert np.linalg.norm(b - g("residual")) <= 1e-12 * np.linalg.norm(g("residual"))
+        if tag == "neg":
+            ia, ja, _, M, _ = pb.jacobian(q, g("dt"))
+            assert np.isfinite(M).all()
+            x = O.sgs_on_matrix(g("ia"), g("ja"), g("ia")[:-1].copy(), g("jacobian"), g("residual"), int(g("nsgs")))
+            assert rel(x, g("dq")) <= 1e-12
+            qn, nz = apply_update(q[:nn], x, float(g("gamma")))
+            assert nz == int(g("zerocount")) == 0
+            assert rel(qn, g("q_new")[:nn]) <= 1e-15
+            assert qn[62, 0] == 1e-10 and qn[100, 0] == 1e-10
+        else:
+            x = pb.sgs(q, g("dt"), g("residual"), int(g("nsgs")))
+            qn, nz = apply_update(q[:nn], x, float(g("gamma")))
+            assert nz == int(g("zerocount")) == nn
+            np.testing.assert_array_equal(np.isnan(qn), np.isnan(g("q_new")[:nn]))
