@@ -387,3 +387,257 @@ class Chem:
         T = lib().o_fr_t_from_p(C.byref(self.c), _d(_f64(rhoi)), float(p), float(Tinit), float(ref_pressure),
                                 C.byref(it))
         return T, it.value
+
+
+# --- finite-rate (FR) equation set (oracle_fr.c) ---------------------------
+
+class OFr(C.Structure):
+    _fields_ = [("chem", C.POINTER(OChem)), ("ref_pressure", C.c_double), ("ref_velocity", C.c_double),
+                ("Pref", C.c_double), ("beta", C.c_double), ("sym2d", C.c_int), ("rxn_on", C.c_int),
+                ("local_dt", C.c_int), ("dt_param", C.c_double)]
+
+
+class OFrCrs(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("nblocks", C.c_int), ("neqn", C.c_int), ("ia", ip), ("ja", ip), ("iau", ip),
+                ("e_lr", ip), ("e_rl", ip), ("pv", ip), ("M", dp)]
+
+
+class OFrSolver(C.Structure):
+    _fields_ = [("m", C.POINTER(OMesh)), ("f", OFr), ("q", dp), ("qref", dp), ("b", dp), ("x", dp), ("dt", dp),
+                ("A", OFrCrs), ("iter", C.c_int), ("nsgs", C.c_int), ("jac_freq", C.c_int), ("cfl_ramp", C.c_int),
+                ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_end", C.c_double), ("order", ip)]
+
+
+_FR_BOUND = False
+
+
+def _fr_lib():
+    global _FR_BOUND
+    L = lib()
+    if not _FR_BOUND:
+        F, M, R = C.POINTER(OFr), C.POINTER(OMesh), C.POINTER(OFrCrs)
+        L.o_fr_aux_q.argtypes = [F, dp]
+        L.o_fr_total_enthalpy.argtypes = [F, dp]
+        L.o_fr_total_enthalpy.restype = C.c_double
+        L.o_fr_total_energy.argtypes = [F, dp]
+        L.o_fr_total_energy.restype = C.c_double
+        L.o_fr_hllc.argtypes = [F, dp, dp, dp, C.c_double, dp]
+        L.o_fr_hllc.restype = C.c_int
+        L.o_fr_max_eig.argtypes = [F, dp, dp, C.c_double]
+        L.o_fr_max_eig.restype = C.c_double
+        L.o_fr_eigensystem.argtypes = [F, dp, dp, C.c_double, dp, dp, dp]
+        L.o_fr_bc_state.argtypes = [F, C.c_int, dp, dp, dp, dp, C.c_double]
+        L.o_fr_apply_dq.argtypes = [F, dp, dp]
+        L.o_fr_temporal.argtypes = [F, dp, C.c_double, C.c_double, C.c_double, dp]
+        L.o_fr_qinf.argtypes = [F, dp, C.c_double, C.c_double, dp, dp]
+        L.o_fr_update_bcs.argtypes = [F, M, dp, dp]
+        L.o_fr_residual.argtypes = [F, M, dp, dp, dp]
+        L.o_fr_residual.restype = C.c_double
+        L.o_fr_timestep.argtypes = [F, M, dp, C.c_double, dp]
+        L.o_fr_timestep.restype = C.c_double
+        L.o_fr_crs_init.argtypes = [M, C.c_int, R]
+        L.o_fr_crs_free.argtypes = [R]
+        L.o_fr_jacobian.argtypes = [F, M, dp, dp, dp, R]
+        L.o_fr_sgs.argtypes = [R, C.c_int, dp, dp, ip]
+        L.o_fr_sgs.restype = C.c_double
+        L.o_fr_solver_init.argtypes = [C.POINTER(OFrSolver), M, F, dp, dp, C.c_double, C.c_double, C.c_int,
+                                       C.c_int, C.c_int]
+        L.o_fr_step.argtypes = [C.POINTER(OFrSolver), dp, ip]
+        L.o_fr_step.restype = C.c_double
+        L.o_fr_solver_free.argtypes = [C.POINTER(OFrSolver)]
+        _FR_BOUND = True
+    return L
+
+
+class FR:
+    """Oracle view of the finite-rate equation set on one chemistry model.
+
+    ref: dict with ref_pressure, ref_velocity, ref_density, ref_time,
+    ref_temperature (Param::PostCompute); Pref the freestream pressure
+    (non-dimensional); beta the preconditioning parameter."""
+
+    def __init__(self, model, ref, Pref=0.0, beta=1.0, sym2d=0, rxn_on=True, local_dt=True, dt_param=-1.0):
+        self.model = model.with_reference(ref["ref_density"], ref["ref_time"], ref["ref_temperature"])
+        self.chem = Chem(self.model)
+        self.ns = self.model.ns
+        self.neqn, self.nvars = self.ns + 4, 3 * self.ns + 6
+        self.f = OFr(C.pointer(self.chem.c), float(ref["ref_pressure"]), float(ref["ref_velocity"]), float(Pref),
+                     float(beta), int(sym2d), int(rxn_on), int(local_dt), float(dt_param))
+        _fr_lib()
+
+    @property
+    def beta(self):
+        return self.f.beta
+
+    def aux(self, Q):
+        """Rows of nvars (or neqn) values -> rows of nvars with aux recomputed."""
+        Q = np.atleast_2d(_f64(Q))
+        out = np.zeros((Q.shape[0], self.nvars))
+        out[:, :self.neqn] = Q[:, :self.neqn]
+        for r in out:
+            _fr_lib().o_fr_aux_q(C.byref(self.f), _d(r))
+        return out
+
+    def qinf(self, Y, p_dim, T_dim, vel):
+        q = np.zeros(self.nvars)
+        _fr_lib().o_fr_qinf(C.byref(self.f), _d(_f64(Y)), float(p_dim), float(T_dim), _d(_f64(vel)), _d(q))
+        return q
+
+    def hllc(self, QL, QR, avec, beta=None):
+        F = np.zeros(self.neqn)
+        z = _fr_lib().o_fr_hllc(C.byref(self.f), _d(_f64(QL)), _d(_f64(QR)), _d(_f64(avec)),
+                                self.beta if beta is None else float(beta), _d(F))
+        return F, z
+
+    def max_eig(self, Q, avec, beta=None):
+        return _fr_lib().o_fr_max_eig(C.byref(self.f), _d(_f64(Q)), _d(_f64(avec)),
+                                      self.beta if beta is None else float(beta))
+
+    def eigensystem(self, Q, avec, beta=None):
+        n = self.neqn
+        eig, T, Ti = np.zeros(n), np.zeros(n * n), np.zeros(n * n)
+        _fr_lib().o_fr_eigensystem(C.byref(self.f), _d(_f64(Q)), _d(_f64(avec)),
+                                   self.beta if beta is None else float(beta), _d(eig), _d(T), _d(Ti))
+        return eig, T.reshape(n, n), Ti.reshape(n, n)
+
+    def total_enthalpy(self, Q):
+        return _fr_lib().o_fr_total_enthalpy(C.byref(self.f), _d(_f64(Q)))
+
+    def total_energy(self, Q):
+        return _fr_lib().o_fr_total_energy(C.byref(self.f), _d(_f64(Q)))
+
+    def bc_state(self, bctype, QL, QR, Qref, avec, beta=None):
+        QL, QR = _f64(QL).copy(), _f64(QR).copy()
+        _fr_lib().o_fr_bc_state(C.byref(self.f), int(bctype), _d(QL), _d(QR), _d(_f64(Qref)), _d(_f64(avec)),
+                                self.beta if beta is None else float(beta))
+        return QL, QR
+
+    def apply_dq(self, dQ, Q):
+        Q = _f64(Q).copy()
+        _fr_lib().o_fr_apply_dq(C.byref(self.f), _d(_f64(dQ)), _d(Q))
+        return Q
+
+    def temporal(self, Q, vol, dtau, beta=None):
+        A = np.zeros(self.neqn * self.neqn)
+        _fr_lib().o_fr_temporal(C.byref(self.f), _d(_f64(Q)), float(vol), float(dtau),
+                                self.beta if beta is None else float(beta), _d(A))
+        return A.reshape(self.neqn, self.neqn)
+
+
+class FRProblem:
+    """Mesh + BCs + FR equation set; qref holds nvars values per face tag."""
+
+    def __init__(self, mesh, bc_type, qref, fr: FR):
+        nt = len(bc_type)
+        self.fr = fr
+        self.keep = dict(edges=_i32(mesh.edges), edge_a=_f64(mesh.edge_a), bedges=_i32(mesh.bedges),
+                         bedge_a=_f64(mesh.bedge_a), bfactag=_i32(mesh.bfactag), xyz=_f64(mesh.xyz),
+                         vol=_f64(mesh.vol), bc_type=_i32(bc_type), qref10=np.zeros((nt, 10)),
+                         twall=np.full(nt, -1.0), bnormal=np.full(mesh.nbedge, -1, dtype=np.int32),
+                         qref=_f64(np.asarray(qref).reshape(nt, fr.nvars)))
+        k = self.keep
+        self.nnode, self.nbnode, self.nedge = mesh.nnode, mesh.nbedge, mesh.nedge
+        self.m = OMesh(mesh.nnode, mesh.nbedge, mesh.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
+                       _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), nt,
+                       _i(k["bc_type"]), _d(k["qref10"]), _d(k["twall"]), _i(k["bnormal"]))
+
+    def update_bcs(self, q):
+        q = _f64(q).copy()
+        _fr_lib().o_fr_update_bcs(C.byref(self.fr.f), C.byref(self.m), _d(self.keep["qref"]), _d(q))
+        return q
+
+    def residual(self, q):
+        b = np.zeros((self.nnode, self.fr.neqn))
+        norms = np.zeros(1 + self.fr.neqn)
+        _fr_lib().o_fr_residual(C.byref(self.fr.f), C.byref(self.m), _d(_f64(q)), _d(b), _d(norms))
+        return b, norms
+
+    def timestep(self, q, cfl):
+        dt = np.zeros(self.nnode)
+        dtmin = _fr_lib().o_fr_timestep(C.byref(self.fr.f), C.byref(self.m), _d(_f64(q)), float(cfl), _d(dt))
+        return dt, dtmin
+
+    def jacobian(self, q, dt):
+        """(ia, ja, iau, M[nblocks, neqn, neqn], q after the boundary Jacobian)."""
+        n = self.fr.neqn
+        A = OFrCrs()
+        _fr_lib().o_fr_crs_init(C.byref(self.m), n, C.byref(A))
+        q = _f64(q).copy()
+        _fr_lib().o_fr_jacobian(C.byref(self.fr.f), C.byref(self.m), _d(self.keep["qref"]), _d(q),
+                                _d(_f64(dt)), C.byref(A))
+        nb = A.nblocks
+        ia = np.ctypeslib.as_array(A.ia, (self.nnode + 1,)).copy()
+        ja = np.ctypeslib.as_array(A.ja, (nb,)).copy()
+        iau = np.ctypeslib.as_array(A.iau, (self.nnode,)).copy()
+        Mm = np.ctypeslib.as_array(A.M, (nb * n * n,)).copy().reshape(nb, n, n)
+        _fr_lib().o_fr_crs_free(C.byref(A))
+        return ia, ja, iau, Mm, q
+
+
+def fr_sgs_on_matrix(ia, ja, iau, M, b, nsgs, order=None):
+    """PrepareSGS + SGS (x0 = 0) on a given neqn x neqn block-CRS matrix;
+    returns (x, |xNorm - xNormOld|)."""
+    ia, ja, iau = _i32(ia), _i32(ja), _i32(iau)
+    M = _f64(M)
+    n = M.shape[-1]
+    Mc = M.copy().reshape(-1)
+    nn = len(ia) - 1
+    pv = np.zeros(n * nn, dtype=np.int32)
+    A = OFrCrs(nn, len(ja), n, _i(ia), _i(ja), _i(iau), None, None, _i(pv), _d(Mc))
+    x = np.zeros((nn, n))
+    o = None if order is None else _i32(order)
+    d = _fr_lib().o_fr_sgs(C.byref(A), nsgs, _d(_f64(b)), _d(x), None if o is None else _i(o))
+    return x, d
+
+
+class FRSolver:
+    """The oracle's first-order FR Newton loop (PreTimeAdvance + NewtonIterate)."""
+
+    def __init__(self, pb: FRProblem, q0, cfl, cfl_start, cfl_ramp, nsgs, jac_freq=1, order=None, ja=None):
+        """ja: optional block-column order per row (same pattern, diagonal
+        first) -- the reference's CRS keeps each row's neighbours in psp
+        order, and SGS sums a row's products in that order."""
+        self.pb = pb
+        self.s = OFrSolver()
+        self._q0 = _f64(q0).copy()
+        self._order = None if order is None else _i32(order)
+        _fr_lib().o_fr_solver_init(C.byref(self.s), C.byref(pb.m), C.byref(pb.fr.f), _d(pb.keep["qref"]),
+                                   _d(self._q0), float(cfl), float(cfl_start), int(cfl_ramp), int(nsgs),
+                                   int(jac_freq))
+        if self._order is not None:
+            self.s.order = _i(self._order)
+        if ja is not None:
+            ja = _i32(ja)
+            cur = np.ctypeslib.as_array(self.s.A.ja, (self.s.A.nblocks,))
+            ia = np.ctypeslib.as_array(self.s.A.ia, (pb.nnode + 1,))
+            for i in range(pb.nnode):
+                a, b = ia[i], ia[i + 1]
+                if ja[a] != i or sorted(ja[a:b]) != sorted(cur[a:b]):
+                    raise ValueError(f"ja row {i} is not a reordering of the mesh's row")
+            cur[:] = ja
+
+    def step(self):
+        norms = np.zeros(1 + self.pb.fr.neqn)
+        nz = C.c_int(0)
+        r = _fr_lib().o_fr_step(C.byref(self.s), _d(norms), C.byref(nz))
+        return r, norms, nz.value
+
+    @property
+    def q(self):
+        n, nv = self.pb.nnode + self.pb.nbnode, self.pb.fr.nvars
+        return np.ctypeslib.as_array(self.s.q, (n * nv,)).copy().reshape(n, nv)
+
+    @property
+    def iter(self):
+        return self.s.iter
+
+    def close(self):
+        if self.s is not None:
+            _fr_lib().o_fr_solver_free(C.byref(self.s))
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -151,6 +151,64 @@ double o_chem_cp(const o_chem* c, int s, double T);
 void o_fr_aux(const o_chem* c, const double* Q, double ref_pressure, double ref_velocity, double* aux);
 double o_fr_t_from_p(const o_chem* c, const double* rhoi, double p, double Tinit, double ref_pressure, int* iters);
 
+/* Finite-rate (FR) equation set, first-order implicit path (oracle_fr.c).
+ * State per node: rho_1..rho_ns, u, v, w, T (neqn = ns + 4), then aux
+ * P, rho, cv_1..cv_ns, conc_1..conc_ns (nvars = 3 ns + 6). */
+typedef struct {
+  const o_chem* chem;  /* with ref_density, ref_time, ref_temperature set */
+  double ref_pressure, ref_velocity;
+  double Pref;         /* gauge reference: the freestream pressure (UpdateQinf) */
+  double beta;         /* preconditioning beta (1 at and above Mach 1) */
+  int sym2d;           /* symmetry2D */
+  int rxn_on;          /* reactionsOn */
+  int local_dt;        /* localTimeStepping */
+  double dt_param;     /* timeStep parameter (default -1) */
+} o_fr;
+
+typedef struct {
+  int nnode, nblocks, neqn;
+  int *ia, *ja, *iau, *e_lr, *e_rl, *pv;
+  double* M; /* [nblocks*neqn*neqn] */
+} o_fr_crs;
+
+typedef struct {
+  const o_mesh* m;
+  o_fr f;
+  double* q;    /* (nnode+nbnode)*nvars */
+  double* qref; /* nfactag*nvars */
+  double *b, *x, *dt;
+  o_fr_crs A;
+  int iter, nsgs, jac_freq, cfl_ramp;
+  double cfl, cfl_start, cfl_end;
+  const int* order;
+} o_fr_solver;
+
+int o_fr_neqn(const o_fr* f);
+int o_fr_nvars(const o_fr* f);
+void o_fr_aux_q(const o_fr* f, double* Q);
+double o_fr_total_enthalpy(const o_fr* f, const double* Q);
+double o_fr_total_energy(const o_fr* f, const double* Q);
+int o_fr_hllc(const o_fr* f, const double* QL, const double* QR, const double* avec, double beta, double* flux);
+double o_fr_max_eig(const o_fr* f, const double* Q, const double* avec, double beta);
+void o_fr_eigensystem(const o_fr* f, const double* Q, const double* avec, double beta, double* eig, double* T,
+                      double* Tinv);
+void o_fr_bc_state(const o_fr* f, int bctype, double* QL, double* QR, const double* Qref, const double* avec,
+                   double beta);
+void o_fr_apply_dq(const o_fr* f, const double* dQ, double* Q);
+void o_fr_temporal(const o_fr* f, const double* Q, double vol, double dtau, double beta, double* A);
+void o_fr_qinf(o_fr* f, const double* Y, double pDim, double TDim, const double* vel, double* Qinf);
+void o_fr_update_bcs(const o_fr* f, const o_mesh* m, const double* qref, double* q);
+double o_fr_residual(const o_fr* f, const o_mesh* m, const double* q, double* b, double* norms);
+double o_fr_timestep(const o_fr* f, const o_mesh* m, const double* q, double cfl, double* dt);
+int o_fr_crs_init(const o_mesh* m, int neqn, o_fr_crs* A);
+void o_fr_crs_free(o_fr_crs* A);
+void o_fr_jacobian(const o_fr* f, const o_mesh* m, const double* qref, double* q, const double* dt, o_fr_crs* A);
+double o_fr_sgs(o_fr_crs* A, int nsgs, const double* b, double* x, const int* order);
+int o_fr_solver_init(o_fr_solver* s, const o_mesh* m, const o_fr* f, const double* qref, const double* q0,
+                     double cfl, double cfl_start, int cfl_ramp, int nsgs, int jac_freq);
+double o_fr_step(o_fr_solver* s, double* norms, int* nzeroed);
+void o_fr_solver_free(o_fr_solver* s);
+
 /* Full solver state for the steady Newton / explicit loop. */
 typedef struct {
   const o_mesh* m;

@@ -18,6 +18,10 @@
 //        NewtonIterate / PostTimeAdvance) from freestream until residGlobal
 //        falls below drop * its first value; writes the history and the
 //        final state.
+//   mode "trace" <n>: the same loop for n iterations, writing the last
+//        iteration's Jacobian, time step, residual, correction and state.
+//   mode "chemdb": writes the species database file <dir>/<casename> from
+//        <dir>/chemdb.txt (see mode_chemdb), for the finite-rate eqnset.
 // Arrays are written raw (little-endian float64 or int32) to
 // <dir>/ref_<name>.bin; scalars go to <dir>/ref_meta.txt.
 #include <cmath>
@@ -275,17 +279,78 @@ int mode_solve(SolutionSpace<Real>* s, Int maxiter, Real drop) {
   return 0;
 }
 
+// trace <n>: the steady loop for n iterations; the last one's Jacobian and
+// time step (after PreTimeAdvance) and residual, correction and state (after
+// NewtonIterate) are written.
+int mode_trace(SolutionSpace<Real>* s, Int n) {
+  const Int nnode = s->m->GetNumNodes(), neqn = s->eqnset->neqn;
+  s->PreIterate();
+  for (Int it = 1; it <= n; ++it) {
+    dump_q(s, "q_pre");
+    s->PreTimeAdvance();
+    if (it == n) {
+      CRSMatrix<Real>* A = s->crs->A;
+      dump("ia", A->ia, (size_t)nnode + 1);
+      dump("ja", A->ja, (size_t)A->nblocks);
+      dump("jacobian", A->M, (size_t)A->nblocks * neqn * neqn);
+      dump("dt", s->GetFieldData("timestep", FIELDS::STATE_NONE), (size_t)nnode);
+      dump_q(s, "q_jac");
+    }
+    s->NewtonIterate();
+    if (it == n) {
+      dump("residual", s->crs->b, (size_t)nnode * neqn);
+      dump("dq", s->crs->x, (size_t)nnode * neqn);
+      dump_q(s, "q_new");
+    }
+    s->PostTimeAdvance();
+  }
+  return 0;
+}
+
+// chemdb: <dir>/chemdb.txt lines "symbol MW coeff15 c1[7] c2[7]" (MW in g/mol,
+// c1 the T <= 1000 K NASA-7 set, c2 the T > 1000 K set) -> the species
+// database file <dir>/<casename> in the layout Species::GetDBInfo reads
+// (ucs/species.tcc:140-320).  The transport curve fits it also reads are
+// written as one row each, valid from 0 to 1e6 K with zero coefficients (the
+// inviscid path never uses their values; the model's set-up prints them).
+int mode_chemdb(const std::string& out) {
+  std::ifstream in(g_dir + "chemdb.txt");
+  if (!in) {
+    std::cerr << "cannot read chemdb.txt" << std::endl;
+    return 3;
+  }
+  hid_t fid = HDF_CreateFile(g_dir + out, 1);
+  if (fid < 0) return 3;
+  std::string sym;
+  while (in >> sym) {
+    Real mw, c15, c1[7], c2[7], fit[6] = {0.0, 1.0e6, 0.0, 0.0, 0.0, 0.0};
+    in >> mw >> c15;
+    for (int k = 0; k < 7; ++k) in >> c1[k];
+    for (int k = 0; k < 7; ++k) in >> c2[k];
+    const std::string dir = "/species/" + sym;
+    HDF_WriteScalar(fid, dir, "MW", &mw);
+    HDF_WriteScalar(fid, dir, "NASA7_burcat_coeff15", &c15);
+    HDF_WriteArray(fid, dir, "NASA7_burcat1", c1, 7);
+    HDF_WriteArray(fid, dir, "NASA7_burcat2", c2, 7);
+    HDF_WriteArray(fid, dir, "k", fit, 1, 6);
+    HDF_WriteArray(fid, dir, "mu", fit, 1, 6);
+  }
+  HDF_CloseFile(fid);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
   if (argc < 4) {
-    std::cerr << "usage: ref_harness <dir/> <casename> phases|pointwise|describe|solve [args]" << std::endl;
+    std::cerr << "usage: ref_harness <dir/> <casename> phases|pointwise|describe|solve|chemdb [args]" << std::endl;
     return 1;
   }
-  MPI_Init(&argc, &argv);
   HDF_TurnOffErrorHandling();
   g_dir = argv[1];
   const std::string casename = argv[2], mode = argv[3];
+  if (mode == "chemdb") return mode_chemdb(casename);
+  MPI_Init(&argc, &argv);
   PObj<Real> pobj;
   std::vector<Param<Real>*> paramList;
   SolutionOrdering<Real> operations;
@@ -302,6 +367,11 @@ int main(int argc, char** argv) {
   meta("ref_velocity", s->param->ref_velocity);
   meta("ref_density", s->param->ref_density);
   meta("ref_pressure", s->param->ref_pressure);
+  meta("ref_time", s->param->ref_time);
+  meta("ref_length", s->param->ref_length);
+  meta("beta", s->GetFieldData("beta", FIELDS::STATE_NONE)[0]);
+  if (s->param->eqnset_id == CompressibleEulerFR || s->param->eqnset_id == CompressibleNSFR)
+    meta("Pref", static_cast<CompressibleFREqnSet<Real>*>(s->eqnset)->Pref);
   meta("neqn", s->eqnset->neqn);
   meta("nvars", s->eqnset->neqn + s->eqnset->nauxvars);
   meta("nterms", s->grad->GetNterms());
@@ -315,6 +385,7 @@ int main(int argc, char** argv) {
   else if (mode == "pointwise") rc = mode_pointwise(s);
   else if (mode == "describe") rc = mode_describe(s);
   else if (mode == "solve" && argc >= 6) rc = mode_solve(s, std::atoi(argv[4]), std::atof(argv[5]));
+  else if (mode == "trace" && argc >= 5) rc = mode_trace(s, std::atoi(argv[4]));
   else std::cerr << "unknown mode or missing arguments" << std::endl;
   g_meta.close();
   MPI_Finalize();

@@ -79,6 +79,15 @@ def load_air5() -> ChemModel:
         nloc=np.array([len(x["species"]) for x in rx], dtype=np.int32), loc=loc, nup=nup, nupp=nupp, eff=eff)
 
 
+def mass_fractions(model: ChemModel, **Y) -> np.ndarray:
+    """Mass-fraction vector in the model's species numbering, e.g.
+    mass_fractions(m, N2=0.767, O2=0.233); species not named get 0."""
+    out = np.zeros(model.ns)
+    for sym, y in Y.items():
+        out[list(model.symbols).index(sym)] = y
+    return out
+
+
 def freestream_scales(model: ChemModel, T_inf: float, rho_inf: float, Y) -> tuple[float, float, float]:
     """Reference density, time and temperature of the Mach-based
     non-dimensionalisation (Param::PostCompute, ucs/param.tcc:354-400) for a

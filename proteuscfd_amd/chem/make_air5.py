@@ -1,6 +1,10 @@
-"""Writes air5.json: the 5-species air model (species [N, N2, O2, NO, O],
-6 reactions) that ucs.x builds from chemModels/5speciesAir.rxn and the
-thermo database its chemdata/chemDB.py derives from BURCAT_FIXED.THR.
+"""Writes air5.json: the 5-species air model (6 reactions) that ucs.x builds
+from chemModels/5speciesAir.rxn and the thermo database its
+chemdata/chemDB.py derives from BURCAT_FIXED.THR.  The species are numbered
+as ChemModel::BuildGlobalSpeciesList (ucs/chem.tcc:163-204) numbers them:
+in order of first appearance over the reactions' local species lists, which
+gives [O2, O, N, N2, NO] (the file's speciesInModel line is not used for
+this).  Every per-species array of the FR state follows that numbering.
 
 The Burcat records are located the way the database builder does
 (chemdata/chemDB.py:514-613): skip the 88-line header, take the text
@@ -21,7 +25,6 @@ import os
 import re
 import sys
 
-SPECIES = ["N", "N2", "O2", "NO", "O"]
 
 
 def read_coeff(line, pos, nchar=15):
@@ -80,6 +83,19 @@ REACTIONS = [
     (2, 0, 6.75e7, 3.75e4, 0.0,
      [("N2", 1, 0, 1.0), ("O", 1, 0, 1.0), ("NO", 0, 1, 1.0), ("N", 0, 1, 1.0)]),
 ]
+
+
+def global_species(reactions):
+    """ChemModel::BuildGlobalSpeciesList: first appearance over the local lists."""
+    out = []
+    for *_, loc in reactions:
+        for x in loc:
+            if x[0] not in out:
+                out.append(x[0])
+    return out
+
+
+SPECIES = global_species(REACTIONS)
 
 
 def main(ref):

@@ -90,20 +90,22 @@ class Run:
         return {"mesh_" + k: v for k, v in out.items()}
 
 
-def phases(run, q, it, nsgs, nvars=10):
+def phases(run, q, it, nsgs, nvars=10, neqn=5, grad=True):
     """The harness's phase-by-phase implicit step on state q (all rows)."""
     m = run.case.mesh
     run.write_q(q)
     run("phases", it, nsgs)
     nn, nt = m.nnode, m.nnode + m.nbedge
-    out = {"q_in": q, "q_bc": run.arr("q_bc").reshape(nt, nvars), "grad": run.arr("grad").reshape(nn, -1),
+    out = {"q_in": q, "q_bc": run.arr("q_bc").reshape(nt, nvars),
            "residual": run.arr("residual").reshape(nn, -1), "residual_norms": run.arr("residual_norms"),
            "dt": run.arr("dt"), "ia": run.arr("ia", np.int32), "ja": run.arr("ja", np.int32),
-           "jacobian": run.arr("jacobian").reshape(-1, 5, 5), "dq": run.arr("dq").reshape(nn, -1),
+           "jacobian": run.arr("jacobian").reshape(-1, neqn, neqn), "dq": run.arr("dq").reshape(nn, -1),
            "q_new": run.arr("q_new").reshape(nt, nvars)}
+    if grad:
+        out["grad"] = run.arr("grad").reshape(nn, -1)
     if os.path.exists(run.dir + "ref_limiter.bin"):
         out["limiter"] = run.arr("limiter").reshape(nn, -1)
-    for k in ["gamma", "cfl", "dtmin", "sgs_delta", "Re", "ref_temperature", "ref_velocity"]:
+    for k in ["gamma", "cfl", "dtmin", "sgs_delta", "Re", "ref_temperature", "ref_velocity", "zerocount"]:
         out[k] = np.float64(run.meta[k])
     out["iter"] = np.int32(it)
     out["nsgs"] = np.int32(nsgs)
@@ -389,9 +391,158 @@ def make_bump5_failure():
     return out
 
 
+# ---- finite-rate (FR) equation set: 5-species air -----------------------------
+
+def fr_keys(**kw):
+    """compressibleEulerFR on the 5-species air model (chemModels/5speciesAir.rxn)
+    with the species database written from proteuscfd_amd/chem/air5.json."""
+    k = {"equationSet": "compressibleEulerFR", "timeOrder": 0, "reorderMesh": 0, "flowDirection": "[1.0, 0.0, 0.0]",
+         "chemicalDatabase": "air5.db", "reactionsOn": 1,
+         # species numbered [O2, O, N, N2, NO] (ucs/chem.tcc:163-204)
+         "massFractions": "[0.233, 0.0, 0.0, 0.767, 0.0]",
+         "spatialOrder": 1}
+    k.update(kw)
+    return k
+
+
+class FRRun(Run):
+    """A Run whose directory also holds the reaction file (the reference's
+    5speciesAir.rxn, copied in as input data) and the species database the
+    harness writes from air5.json (Species::GetDBInfo's layout)."""
+
+    def __init__(self, case, keys, bc, nsteps=1, name="c"):
+        super().__init__(case, keys, bc, nsteps=nsteps, name=name)
+        import json
+        import shutil
+        shutil.copy(os.path.join(build_ref.REF, "chemModels", "5speciesAir.rxn"), self.dir + name + ".rxn")
+        with open(os.path.join(ROOT, "proteuscfd_amd", "chem", "air5.json")) as f:
+            air = json.load(f)
+        with open(self.dir + "chemdb.txt", "w") as f:
+            for sp in air["species"]:
+                vals = [sp["MW"], sp["coeff15"], *sp["lo"], *sp["hi"]]
+                f.write(sp["symbol"] + " " + " ".join(f"{v:.17g}" for v in vals) + "\n")
+        r = subprocess.run([build_ref.harness_path(), self.dir, "air5.db", "chemdb"], capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout[-3000:] + r.stderr[-3000:])
+            raise RuntimeError("ref_harness chemdb failed")
+
+    def ref_scales(self):
+        """Reference scales and the freestream of this space (an empty pointwise call)."""
+        np.zeros((0, 32)).tofile(self.dir + "pw_in.bin")
+        self("pointwise")
+        return dict(self.meta), self.arr("qinf")
+
+
+def fr_state(run, qinf, seed, T_range, vel_amp, mach_dir):
+    """Per node: temperature log-uniform in T_range (K), species densities of
+    partly dissociated air scaled by a random factor, velocity = freestream
+    times (1 +- vel_amp) plus a random cross-flow; each phantom row a state
+    near its boundary node's.  Only the first ns+4 values of each row are
+    inputs."""
+    meta, _ = run.ref_scales()
+    rng = np.random.default_rng(seed)
+    m = run.case.mesh
+    nt = m.nnode + m.nbedge
+    ns = 5
+    q = np.zeros((nt, 3 * ns + 6))
+    Y = np.array([0.18, 0.05, 0.02, 0.70, 0.05])  # [O2, O, N, N2, NO]
+    for i in range(nt):
+        y = Y * np.exp(rng.normal(0.0, 0.3, ns))
+        y /= y.sum()
+        rho = qinf[:ns].sum() * np.exp(rng.normal(0.0, 0.3))
+        q[i, :ns] = rho * y
+        q[i, ns:ns + 3] = qinf[ns:ns + 3] * (1.0 + rng.uniform(-vel_amp, vel_amp)) + rng.normal(0.0, 0.2, 3) * mach_dir
+        q[i, ns + 3] = np.exp(rng.uniform(np.log(T_range[0]), np.log(T_range[1]))) / meta["ref_temperature"]
+    # phantoms start near their boundary node, as after an earlier boundary
+    # update (the characteristic wall/farfield sub-iterations start from them)
+    L = m.bedges[:, 0]
+    q[m.nnode:, :ns + 4] = q[L, :ns + 4] * np.exp(rng.normal(0.0, 0.02, (len(L), ns + 4)))
+    return q
+
+
+def fr_out(run, r, meta):
+    r = dict(r)
+    if "jacobian" in r:
+        # every third block row of the Jacobian (the SGS correction dq, stored
+        # whole, depends on all of them)
+        rows = np.arange(0, len(r["ia"]) - 1, 3, dtype=np.int32)
+        ia = r["ia"]
+        sel = np.concatenate([np.arange(ia[i], ia[i + 1]) for i in rows])
+        r["jac_rows"], r["jac_blocks"] = rows, r.pop("jacobian")[sel]
+    for k in ["ref_pressure", "ref_velocity", "ref_density", "ref_time", "ref_temperature", "Pref", "beta"]:
+        r[k] = np.float64(meta[k])
+    r.update(run.mesh_arrays())
+    return r
+
+
+def make_fr_wedge_phases():
+    """Finite-rate reacting Euler (5-species air), first order, on a small
+    15-degree wedge channel at Mach 8 (freestream 300 K, 1e-3 kg/m^3): sonic
+    inflow (hard-set), sonic outflow, inviscid walls (the characteristic wall
+    state with its 10 sub-iterations), symmetry planes; states from 300 K to
+    5500 K so the reactions run; HLLC residual with the source term, time
+    step, FD Jacobian with boundary re-evaluation, source Jacobian and the
+    non-conservative temporal terms, 4 SGS and ApplyDQ."""
+    case = cases.wedge_case(nx=8, ny=4, nz=2, rcm=False)
+    bc = {1: 7, 2: 8, 3: 3, 4: 3, 5: 9, 6: 9}
+    p_inf = 1.0e-3 * (0.767 * 8.31447215 / 0.0280134 + 0.233 * 8.31447215 / 0.0319988) * 300.0
+    keys = fr_keys(velocity=8.0, initialTemperature=300.0, initialPressure=f"{p_inf:.17g}",
+                   refPressure=f"{p_inf:.17g}", numberSGS=4, CFL=5.0, startingCFL=5.0)
+    run = FRRun(case, keys, bc)
+    meta, qinf = run.ref_scales()
+    q = fr_state(run, qinf, seed=31, T_range=(300.0, 5500.0), vel_amp=0.3, mach_dir=np.ones(3))
+    r = phases(run, q, 3, 4, nvars=21, neqn=9, grad=False)
+    return fr_out(run, r, meta)
+
+
+def make_fr_bump_phases():
+    """Finite-rate Euler at Mach 0.5 on a small bump with farfield boundaries
+    (the characteristic farfield state, 10 sub-iterations with the
+    Newton temperature solve) and an inviscid wall, 4^3 cells: beta = M^2 = 0.25, so
+    the preconditioned HLLC speeds, eigensystem and temporal terms all see
+    beta != 1."""
+    case = cases.bump_case(n=4, rcm=False)
+    keys = fr_keys(velocity=0.5, initialTemperature=600.0, initialPressure=101325, refPressure=101325,
+                   numberSGS=4, CFL=3.0, startingCFL=3.0)
+    run = FRRun(case, keys, bump_bc())
+    meta, qinf = run.ref_scales()
+    q = fr_state(run, qinf, seed=32, T_range=(400.0, 3000.0), vel_amp=0.3, mach_dir=np.ones(3))
+    r = phases(run, q, 2, 4, nvars=21, neqn=9, grad=False)
+    return fr_out(run, r, meta)
+
+
+def make_fr_bump_solve():
+    """The reference's own steady loop for finite-rate Euler on the small bump
+    (Mach 0.5, beta 0.25, farfield + inviscid wall), first order, natural-
+    order SGS, CFL 1 -> 20 over 30 iterations, from a uniform 3000 K partly
+    dissociated freestream (mass fractions O2 0.10, O 0.10, N 0.05, N2 0.70,
+    NO 0.05) whose reactions run throughout, to an 8-order drop of
+    residGlobal: the history and the converged field.  (From a cold
+    freestream the reference's first-order loop on the Mach 8 wedge channel
+    does not converge: its wall nodes lose their temperature within a few
+    iterations, in the reference and the oracle alike.)"""
+    case = cases.bump_case(n=4, rcm=False)
+    keys = fr_keys(velocity=0.5, initialTemperature=3000.0, initialPressure=101325, refPressure=101325,
+                   massFractions="[0.10, 0.10, 0.05, 0.70, 0.05]", numberSGS=6, CFL=20.0, startingCFL=1.0,
+                   rampCFL=30)
+    run = FRRun(case, keys, bump_bc(), nsteps=800)
+    meta, qinf = run.ref_scales()
+    run("trace", 1)
+    ia, ja = run.arr("ia", np.int32), run.arr("ja", np.int32)
+    run("solve", 800, 1e-8)
+    m = case.mesh
+    # ia/ja: the reference's block-row layout (diagonal, then the node's
+    # neighbours in its psp order), the order its SGS sums a row in
+    out = {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 21),
+           "qinf": qinf, "ia": ia, "ja": ja}
+    return fr_out(run, out, meta)
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
           "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
-          "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure}
+          "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure,
+          "fr_wedge_phases": make_fr_wedge_phases, "fr_bump_phases": make_fr_bump_phases,
+          "fr_bump_solve": make_fr_bump_solve}
 
 
 def main(names):

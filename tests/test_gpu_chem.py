@@ -37,7 +37,7 @@ def test_wdot_matches_oracle():
 
 def test_source_and_jacobian_match_oracle():
     from proteuscfd_amd import gpu
-    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, [0, 0.767, 0.233, 0, 0])
+    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, chem.mass_fractions(MODEL, N2=0.767, O2=0.233))
     m = MODEL.with_reference(rho_ref, t_ref, T_ref)
     ctx = gpu.ChemContext(m)
     rhoi, T = states(500, 2)
@@ -63,7 +63,7 @@ def test_source_and_jacobian_match_oracle():
 
 def test_fr_aux_and_t_from_p_match_oracle():
     from proteuscfd_amd import gpu
-    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, [0, 0.767, 0.233, 0, 0])
+    rho_ref, t_ref, T_ref = chem.freestream_scales(MODEL, 300.0, 1e-3, chem.mass_fractions(MODEL, N2=0.767, O2=0.233))
     m = MODEL.with_reference(rho_ref, t_ref, T_ref)
     ctx = gpu.ChemContext(m)
     rhoi, T = states(2000, 7)
