@@ -126,6 +126,10 @@ typedef struct {
   double T_min, T_max, rho_min, rho_max;  /* training box; outside it the exact rates are used */
   double tol, floor;             /* element-balance bound: fall back when
                                     imbalance > tol * (sum|w_i/M_i| + rho*floor) */
+  double atoms[2][5];            /* atoms of the two conserved elements (N, O
+                                    for air) in each species, in the model's
+                                    species numbering; imbalance =
+                                    sum_e |sum_i atoms[e][i] w_i / M_i| */
 } pcfd_surrogate_desc;
 
 /* Production rates by the surrogate for n cells (device arrays rhoi[n*5]

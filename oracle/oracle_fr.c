@@ -610,10 +610,13 @@ double o_fr_residual(const o_fr* f, const o_mesh* m, const double* q, double* b,
     o_fr_hllc(f, &q[(size_t)nv * L], &q[(size_t)nv * R], &m->bedge_a[4 * e], f->beta, F);
     for (int k = 0; k < neqn; k++) b[(size_t)neqn * L + k] += -F[k];
   }
-  if (f->rxn_on) {
+  {
+    /* SourceTerm is added for every node; it is zero without reactions */
     double S[O_CHEM_MAXS + 4];
     for (int i = 0; i < nn; i++) {
-      o_chem_source(f->chem, &q[(size_t)nv * i], m->vol[i], S);
+      if (f->rxn_on) o_chem_source(f->chem, &q[(size_t)nv * i], m->vol[i], S);
+      else
+        for (int k = 0; k < neqn; k++) S[k] = 0.0;
       for (int k = 0; k < neqn; k++) b[(size_t)neqn * i + k] += S[k];
     }
   }

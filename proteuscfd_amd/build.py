@@ -20,7 +20,7 @@ INCLUDE = os.path.join(ROOT, "include")
 
 MESH_SRC = ["mesh.cpp", "mesh_io.cpp"]
 HIP_SRC = ["ctx.hip", "kernels_flux.hip", "kernels_visc.hip", "kernels_jac.hip", "kernels_sgs.hip",
-           "kernels_misc.hip", "solver.hip", "kernels_chem.hip", "kernels_mlp.hip"]
+           "kernels_misc.hip", "solver.hip", "kernels_chem.hip", "kernels_mlp.hip", "fr.hip"]
 
 # gfx950 only.  Packed-FP32 instructions are disabled for all device code.
 HIPCC_FLAGS = [

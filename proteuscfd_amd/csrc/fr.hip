@@ -1,0 +1,1438 @@
+// Finite-rate (FR) reacting Euler, first-order implicit path, on one GPU:
+// the C ABI of include/pcfd_fr.h, its device layout and its kernels.
+//
+// Layout in HBM (ns = NS species, NE = NS + 4):
+//   q      [ntot][NE]   native state per node, interior nodes then one
+//                       phantom per half-edge (AoS: an edge kernel's two
+//                       endpoint loads are two runs of NE*8 bytes)
+//   eflux  [nedge][NE]  HLLC flux of every edge, n0 -> n1 orientation
+//   elam   [nedge]      max eigenvalue x |a| of every edge (time step)
+//   b      [nnode][NE]  residual;  dt [nnode]
+//   node lists (CSR in edge order): nptr [nnode+1], nslot [2 nedge] =
+//                       2 e + side (side 0: the node is n0 of edge e)
+//   block matrix (SELL, rows grouped by colour, 64-row chunks): off-diagonal
+//                       block of row r slot s at ((sbase[chunk]+s)*NE2 + k)*64
+//                       + lane, k = 0..NE2-1 (row-major entry); diagonal at
+//                       (chunk*NE2 + k)*64 + lane; a row's slots follow its
+//                       CSR (edge) order.  eoff[2e], eoff[2e+1]: where edge
+//                       e's blocks (n0,n1) and (n1,n0) start.
+//
+// Every flux and residual sum runs in the reference's order (edges in edge
+// order, then half-edges, then the source), as node gathers with no atomics,
+// and FP contraction is off in this unit: the FP64 results are the
+// reference's arithmetic.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/pcfd_fr.h"
+#include "../../include/pcfd_mesh.h"
+#include "chem_model.h"
+#include "fr_physics.h"
+
+namespace pcfd {
+const ChemDev* chem_device_model(const pcfd_chem* c);
+int chem_nspecies(const pcfd_chem* c);
+}  // namespace pcfd
+
+using namespace pcfd;
+using fr::Model;
+using fr::State;
+
+namespace {
+
+constexpr int kFB = 128;  // threads per block for the FR kernels
+constexpr int kLane = 64;
+
+thread_local std::string g_fr_create_err;
+
+void hcheck(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+inline int nblocks(long long n, int bs = kFB) { return (int)((n + bs - 1) / bs); }
+
+template <int NS>
+__device__ __forceinline__ void load_state(const double* __restrict__ q, long long i, State<NS>& s,
+                                           const Model& m) {
+  constexpr int NE = NS + 4;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) s.q[k] = q[i * NE + k];
+  fr::aux(m, s);
+}
+
+template <int NS>
+__device__ __forceinline__ void store_state(double* __restrict__ q, long long i, const State<NS>& s) {
+  constexpr int NE = NS + 4;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) q[i * NE + k] = s.q[k];
+}
+
+// ---- boundary states (UpdateBCs, ucs/bc.tcc:1399-1457) ---------------------
+// One thread per boundary node: its half-edges in half-edge order (a
+// Dirichlet / sonic-inflow half-edge resets the node's own state, which the
+// node's later half-edges then see, as in the reference's half-edge loop).
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_bcs(const Model* __restrict__ mp, int nbnd, const int* __restrict__ bnd,
+                                               const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                               const int* __restrict__ he_tag, const double* __restrict__ he_a,
+                                               const int* __restrict__ bc_type, const double* __restrict__ qref,
+                                               int nnode, double beta, double* __restrict__ q) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbnd) return;
+  const Model& m = *mp;
+  const int i = bnd[t];
+  State<NS> L;
+  load_state<NS>(q, i, L, m);
+  bool lchanged = false;
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    const int h = he_id[p], tag = he_tag[h];
+    const int ty = bc_type[tag];
+    State<NS> R, Qr;
+    load_state<NS>(q, (long long)nnode + h, R, m);
+    load_state<NS>(qref, tag, Qr, m);
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = he_a[4LL * h + k];
+    if (fr::bc_state<NS>(m, ty, L, R, Qr, a, beta)) {
+      store_state<NS>(q, (long long)nnode + h, R);
+      if (ty == fr::kSonicInflow || ty == fr::kDirichlet) lchanged = true;
+    }
+  }
+  if (lchanged) store_state<NS>(q, i, L);
+}
+
+// ---- edge passes -----------------------------------------------------------
+// HLLC flux of every edge (Kernel_Inviscid_Flux at first order,
+// ucs/residual.tcc:191-296), n0 -> n1.
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_edge_flux(const Model* __restrict__ mp, int nedge,
+                                                     const int* __restrict__ edges, const double* __restrict__ ea,
+                                                     const double* __restrict__ q, double beta,
+                                                     double* __restrict__ eflux) {
+  constexpr int NE = NS + 4;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedge) return;
+  const Model& m = *mp;
+  State<NS> L, R;
+  load_state<NS>(q, edges[2 * e], L, m);
+  load_state<NS>(q, edges[2 * e + 1], R, m);
+  double a[4], F[NE];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a[k] = ea[4LL * e + k];
+  fr::hllc<NS>(m, L, R, a, beta, F);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) eflux[(long long)e * NE + k] = F[k];
+}
+
+// max eigenvalue of the edge-averaged state times |a| (Kernel_Timestep,
+// ucs/timestep.tcc:72-108)
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_edge_lam(const Model* __restrict__ mp, int nedge,
+                                                    const int* __restrict__ edges, const double* __restrict__ ea,
+                                                    const double* __restrict__ q, double beta,
+                                                    double* __restrict__ elam) {
+  constexpr int NE = NS + 4;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedge) return;
+  const Model& m = *mp;
+  const long long L = edges[2 * e], R = edges[2 * e + 1];
+  State<NS> s;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) s.q[k] = 0.5 * (q[L * NE + k] + q[R * NE + k]);
+  fr::aux(m, s);
+  double a[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a[k] = ea[4LL * e + k];
+  elam[e] = fr::max_eig<NS>(m, s, a, beta) * a[3];
+}
+
+// Per-block partial sums of b^2 (total and per equation) with a fixed tree
+template <int NE>
+__device__ __forceinline__ void norm_partials(const double* bi, bool valid, double* red) {
+  __shared__ double sh[NE + 1][kFB];
+  double tot = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const double v = valid ? bi[k] * bi[k] : 0.0;
+    sh[1 + k][threadIdx.x] = v;
+    tot += v;
+  }
+  sh[0][threadIdx.x] = tot;
+  __syncthreads();
+  for (int s = kFB / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int k = 0; k <= NE; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k <= NE; ++k) red[(long long)k * gridDim.x + blockIdx.x] = sh[k][0];
+  }
+}
+
+// Residual of every node: edge fluxes in edge order (b[n0] -= F, b[n1] += F),
+// then the node's boundary fluxes (Bkernel_Inviscid_Flux, ucs/residual.tcc:
+// 298-387: b -= F of the node against its phantom), then the source term
+// (SpatialResidual, :100-114)
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_residual(const Model* __restrict__ mp, const ChemDev* __restrict__ cm,
+                                                    int nnode, const int* __restrict__ nptr,
+                                                    const int* __restrict__ nslot, const double* __restrict__ eflux,
+                                                    const int* __restrict__ he_ptr, const int* __restrict__ he_id,
+                                                    const double* __restrict__ he_a, const double* __restrict__ q,
+                                                    const double* __restrict__ vol, int rxn, double beta,
+                                                    double* __restrict__ b, double* __restrict__ red) {
+  constexpr int NE = NS + 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < nnode;
+  double acc[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) acc[k] = 0.0;
+  if (valid) {
+    const Model& m = *mp;
+    for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
+      const int s = nslot[p];
+      const double* F = eflux + (long long)(s >> 1) * NE;
+      if (s & 1) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) acc[k] += F[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) acc[k] += -F[k];
+      }
+    }
+    if (he_ptr[i + 1] > he_ptr[i]) {
+      State<NS> L;
+      load_state<NS>(q, i, L, m);
+      for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+        const int h = he_id[p];
+        State<NS> R;
+        load_state<NS>(q, (long long)nnode + h, R, m);
+        double a[4], F[NE];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] = he_a[4LL * h + k];
+        fr::hllc<NS>(m, L, R, a, beta, F);
+#pragma unroll
+        for (int k = 0; k < NE; ++k) acc[k] += -F[k];
+      }
+    }
+    // the source vector is added whole (zero without reactions, and zero in
+    // the momentum and energy entries), as SpatialResidual does
+    double S[NS];
+    if (rxn) {
+      double Q[NE];
+#pragma unroll
+      for (int k = 0; k < NE; ++k) Q[k] = q[(long long)i * NE + k];
+      source_of(*cm, Q, vol[i], S);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NS; ++k) S[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] += S[k];
+#pragma unroll
+    for (int k = NS; k < NE; ++k) acc[k] += 0.0;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) b[(long long)i * NE + k] = acc[k];
+  }
+  norm_partials<NE>(acc, valid, red);
+}
+
+// Time step of every node (ComputeTimesteps with local time stepping,
+// ucs/timestep.tcc:6-40): edge eigenvalues in edge order, then the node's
+// half-edges against their phantoms; dt = CFL vol / sum
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_timestep(const Model* __restrict__ mp, int nnode,
+                                                    const int* __restrict__ nptr, const int* __restrict__ nslot,
+                                                    const double* __restrict__ elam, const int* __restrict__ he_ptr,
+                                                    const int* __restrict__ he_id, const double* __restrict__ he_a,
+                                                    const double* __restrict__ q, const double* __restrict__ vol,
+                                                    double cfl, double beta, double* __restrict__ dt) {
+  constexpr int NE = NS + 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const Model& m = *mp;
+  double s = 0.0;
+  for (int p = nptr[i]; p < nptr[i + 1]; ++p) s += elam[nslot[p] >> 1];
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    const int h = he_id[p];
+    const long long R = (long long)nnode + h;
+    State<NS> t;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) t.q[k] = 0.5 * (q[(long long)i * NE + k] + q[R * NE + k]);
+    fr::aux(m, t);
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = he_a[4LL * h + k];
+    s += fr::max_eig<NS>(m, t, a, beta) * a[3];
+  }
+  dt[i] = cfl * (vol[i] / s);
+}
+
+// ---- Jacobian --------------------------------------------------------------
+// Kernel_NumJac (ucs/jacobian.tcc:253-304): one-sided differences (h = 1e-8)
+// of the edge's HLLC flux in each native variable of either side, aux
+// recomputed for every perturbed state.  Block (n0,n1) = dF/dq_n1, block
+// (n1,n0) = -dF/dq_n0; both are written into their SELL slots.
+template <int NS, typename TM>
+__global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ mp, int nedge,
+                                                    const int* __restrict__ edges, const double* __restrict__ ea,
+                                                    const double* __restrict__ q, double beta,
+                                                    const long long* __restrict__ eoff, TM* __restrict__ M) {
+  constexpr int NE = NS + 4;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedge) return;
+  const Model& m = *mp;
+  State<NS> L, R;
+  load_state<NS>(q, edges[2 * e], L, m);
+  load_state<NS>(q, edges[2 * e + 1], R, m);
+  double a[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) a[k] = ea[4LL * e + k];
+  double fS[NE];
+  fr::hllc<NS>(m, L, R, a, beta, fS);
+  const double h = 1.0e-8;
+  TM* oLR = M + eoff[2LL * e];
+  TM* oRL = M + eoff[2LL * e + 1];
+  for (int i = 0; i < NE; ++i) {
+    State<NS> P = L;
+    P.q[i] += h;
+    fr::aux(m, P);
+    double fL[NE], fR[NE];
+    fr::hllc<NS>(m, P, R, a, beta, fL);
+    P = R;
+    P.q[i] += h;
+    fr::aux(m, P);
+    fr::hllc<NS>(m, L, P, a, beta, fR);
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      // added to a zeroed block, as the reference's scatter does
+      oRL[(long long)(j * NE + i) * kLane] = (TM)(0.0 + (fS[j] - fL[j]) / h);
+      oLR[(long long)(j * NE + i) * kLane] = (TM)(0.0 + (fR[j] - fS[j]) / h);
+    }
+  }
+}
+
+// Bkernel_NumJac (ucs/jacobian.tcc:458-535) with jacobianBoundaryEval 0:
+// per boundary node, its half-edges in order; the BC state is set on the
+// stored states first (q updated, as the reference's in-place call does),
+// then each native variable of the node perturbed, the BC re-evaluated from
+// the perturbed state and the boundary flux differenced.  The node's
+// boundary blocks are summed in half-edge order into jdb.
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_jac_bnd(const Model* __restrict__ mp, int nbnd,
+                                                   const int* __restrict__ bnd, const int* __restrict__ he_ptr,
+                                                   const int* __restrict__ he_id, const int* __restrict__ he_tag,
+                                                   const double* __restrict__ he_a, const int* __restrict__ bc_type,
+                                                   const double* __restrict__ qref, int nnode, double beta,
+                                                   double* __restrict__ q, double* __restrict__ jdb) {
+  constexpr int NE = NS + 4, NE2 = NE * NE;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbnd) return;
+  const Model& m = *mp;
+  const int i = bnd[t];
+  double* J = jdb + (long long)t * NE2;
+  for (int k = 0; k < NE2; ++k) J[k] = 0.0;
+  State<NS> L;
+  load_state<NS>(q, i, L, m);
+  bool lchanged = false;
+  const double hh = 1.0e-8;
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    const int h = he_id[p], tag = he_tag[h];
+    const int ty = bc_type[tag];
+    State<NS> R, Qr;
+    load_state<NS>(q, (long long)nnode + h, R, m);
+    load_state<NS>(qref, tag, Qr, m);
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = he_a[4LL * h + k];
+    if (fr::bc_state<NS>(m, ty, L, R, Qr, a, beta)) {
+      store_state<NS>(q, (long long)nnode + h, R);
+      if (ty == fr::kSonicInflow || ty == fr::kDirichlet) lchanged = true;
+    }
+    double fS[NE];
+    fr::hllc<NS>(m, L, R, a, beta, fS);
+    for (int c = 0; c < NE; ++c) {
+      State<NS> PL = L, PR = R;
+      PL.q[c] += hh;
+      fr::aux(m, PL);
+      fr::bc_state<NS>(m, ty, PL, PR, Qr, a, beta);
+      double fL[NE];
+      fr::hllc<NS>(m, PL, PR, a, beta, fL);
+#pragma unroll
+      for (int j = 0; j < NE; ++j) J[j * NE + c] += (fL[j] - fS[j]) / hh;
+    }
+  }
+  if (lchanged) store_state<NS>(q, i, L);
+}
+
+// Diagonal block of every row (Kernel_Diag_NumJac, ucs/jacobian.tcc:436-456:
+// minus the transposed off-diagonal blocks in edge order, after the boundary
+// part), minus the source Jacobian (:198-209, EqnSet::SourceTermJacobian,
+// ucs/eqnset.tcc:162-187), plus the temporal terms (:213-250).  One thread
+// per SELL row; the block is built in LDS.
+template <int NS, typename TM>
+__global__ void __launch_bounds__(64) k_fr_jac_diag(const Model* __restrict__ mp, const ChemDev* __restrict__ cm,
+                                                   int nrow_pad, const int* __restrict__ srow,
+                                                   const int* __restrict__ nptr, const int* __restrict__ nslot,
+                                                   const long long* __restrict__ eoff, const TM* __restrict__ M,
+                                                   const int* __restrict__ bidx, const double* __restrict__ jdb,
+                                                   const double* __restrict__ q, const double* __restrict__ vol,
+                                                   const double* __restrict__ dt, int rxn, double beta,
+                                                   TM* __restrict__ D) {
+  constexpr int NE = NS + 4, NE2 = NE * NE;
+  __shared__ double sh[NE2][64];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x;
+  if (r >= nrow_pad) return;
+  const int i = srow[r];
+  TM* Dr = D + (long long)(r >> 6) * NE2 * kLane + (r & 63);
+  if (i < 0) {
+    // padding rows: identity, so the factorisation of the chunk stays finite
+    for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)((k % (NE + 1)) == 0 ? 1.0 : 0.0);
+    return;
+  }
+  const Model& m = *mp;
+  double(*A)[64] = sh;
+  const int bi = bidx[i];
+  for (int k = 0; k < NE2; ++k) A[k][lane] = bi >= 0 ? jdb[(long long)bi * NE2 + k] : 0.0;
+  for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
+    const int s = nslot[p];
+    // the block of the other row in this node's column
+    const TM* o = M + eoff[(long long)(s >> 1) * 2 + ((s & 1) ? 0 : 1)];
+    for (int k = 0; k < NE2; ++k) A[k][lane] += -(double)o[(long long)k * kLane];
+  }
+  State<NS> S;
+  load_state<NS>(q, i, S, m);
+  const double v = vol[i];
+  if (rxn) {
+    double s0[NS], s1[NS], Q[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) Q[k] = S.q[k];
+    source_of(*cm, Q, v, s0);
+    const double h = 1.0e-8;
+    for (int c = 0; c < NE; ++c) {
+      if (c >= NS && c < NS + 3) continue;  // the source does not read u, v, w: (S - S) / h = 0
+      const double keep = Q[c];
+      Q[c] += h;
+      source_of(*cm, Q, v, s1);
+      Q[c] = keep;
+      for (int j = 0; j < NS; ++j) A[j * NE + c][lane] -= (s1[j] - s0[j]) / h;
+    }
+  }
+  double T[NE2];
+  for (int k = 0; k < NE2; ++k) T[k] = A[k][lane];
+  fr::temporal<NS>(m, S, v, dt[i], beta, T);
+  for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)T[k];
+}
+
+// PrepareSGS (ucs/crsmatrix.tcc:839-876): LU with logical pivoting of every
+// diagonal block, in place; the row order is packed 4 bits per entry.
+template <int NS, typename TM>
+__global__ void __launch_bounds__(64) k_fr_prepare(int nrow_pad, TM* __restrict__ D, long long* __restrict__ piv) {
+  constexpr int NE = NS + 4, NE2 = NE * NE;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrow_pad) return;
+  TM* Dr = D + (long long)(r >> 6) * NE2 * kLane + (r & 63);
+  double A[NE2];
+  for (int k = 0; k < NE2; ++k) A[k] = (double)Dr[(long long)k * kLane];
+  int p[NE];
+  if constexpr (sizeof(TM) == 8) {
+    fr::lu<NE>(A, p);
+  } else {
+    // FP32 form: factor in FP32 arithmetic
+    float Af[NE2];
+    for (int k = 0; k < NE2; ++k) Af[k] = (float)A[k];
+    int row = 0;
+    for (int k = 0; k < NE; ++k) p[k] = k;
+    for (int c = 0; c < NE; ++c) {
+      float large = 0.0f;
+      for (int j = c; j < NE; ++j)
+        if (fabsf(Af[p[j] * NE + c]) > fabsf(large)) {
+          large = Af[p[j] * NE + c];
+          row = j;
+        }
+      const int tt = p[c];
+      p[c] = p[row];
+      p[row] = tt;
+      large = 1.0f / large;
+      for (int j = c + 1; j < NE; ++j) Af[p[j] * NE + c] *= large;
+      for (int j = c + 1; j < NE; ++j)
+        for (int k = c + 1; k < NE; ++k) Af[p[j] * NE + k] -= Af[p[j] * NE + c] * Af[p[c] * NE + k];
+    }
+    for (int k = 0; k < NE2; ++k) A[k] = Af[k];
+  }
+  long long pk = 0;
+  for (int k = 0; k < NE; ++k) pk |= (long long)p[k] << (4 * k);
+  piv[r] = pk;
+  for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)A[k];
+}
+
+// One colour of a symmetric Gauss-Seidel sweep (CRS::SGS, ucs/crs.tcc:61-173):
+// rows of a colour share no block, so they update together; per row the
+// neighbours' products are subtracted in slot (edge) order, then LuSolve.
+template <int NS, typename TM>
+__global__ void __launch_bounds__(kFB) k_fr_sgs_color(int row0, int nrows, const int* __restrict__ srow,
+                                                     const int* __restrict__ swidth, const int* __restrict__ sbase,
+                                                     const int* __restrict__ scol, const TM* __restrict__ Moff,
+                                                     const TM* __restrict__ D, const long long* __restrict__ piv,
+                                                     const double* __restrict__ b, TM* __restrict__ x) {
+  constexpr int NE = NS + 4, NE2 = NE * NE;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrows) return;
+  const int r = row0 + t;
+  const int i = srow[r];
+  if (i < 0) return;
+  const int chunk = r >> 6, lane = r & 63;
+  TM rhs[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) rhs[k] = (TM)b[(long long)i * NE + k];
+  const int w = swidth[chunk];
+  const long long cb = (long long)sbase[chunk] * kLane + lane;
+  const TM* mb = Moff + (long long)sbase[chunk] * NE2 * kLane + lane;
+  for (int s = 0; s < w; ++s) {
+    const int j = scol[cb + (long long)s * kLane];
+    if (j < 0) break;
+    TM xj[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) xj[k] = x[(long long)j * NE + k];
+    const TM* blk = mb + (long long)s * NE2 * kLane;
+#pragma unroll
+    for (int rr = 0; rr < NE; ++rr) {
+      TM v = blk[(long long)(rr * NE) * kLane] * xj[0];
+#pragma unroll
+      for (int c = 1; c < NE; ++c) v += blk[(long long)(rr * NE + c) * kLane] * xj[c];
+      rhs[rr] -= v;
+    }
+  }
+  const TM* dg = D + (long long)chunk * NE2 * kLane + lane;
+  TM A[NE2];
+#pragma unroll
+  for (int k = 0; k < NE2; ++k) A[k] = dg[(long long)k * kLane];
+  const long long pk = piv[r];
+  int p[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) p[k] = (int)((pk >> (4 * k)) & 15);
+  fr::lu_solve<NE, TM>(A, rhs, p);
+#pragma unroll
+  for (int k = 0; k < NE; ++k) x[(long long)i * NE + k] = rhs[k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFB) k_fr_zero(long long n, T* __restrict__ x) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = (T)0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kFB) k_fr_sumsq(long long n, const T* __restrict__ x, double* __restrict__ red) {
+  __shared__ double sh[kFB];
+  double s = 0.0;
+  for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+    const double v = (double)x[k];
+    s += v * v;
+  }
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kFB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red[blockIdx.x] = sh[0];
+}
+
+// NaN/Inf rows of the correction zeroed and counted (NewtonIterate,
+// ucs/solutionSpace.tcc:772-800), then ApplyDQ on the node
+template <int NS, typename TX>
+__global__ void __launch_bounds__(kFB) k_fr_apply(const Model* __restrict__ mp, int nnode, const TX* __restrict__ x,
+                                                 double* __restrict__ q, int* __restrict__ nzero) {
+  constexpr int NE = NS + 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const Model& m = *mp;
+  double dq[NE];
+  bool bad = false;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    dq[k] = (double)x[(long long)i * NE + k];
+    if (isnan(dq[k]) || isinf(dq[k])) bad = true;
+  }
+  if (bad) {
+#pragma unroll
+    for (int k = 0; k < NE; ++k) dq[k] = 0.0;
+    atomicAdd(nzero, 1);
+  }
+  State<NS> s;
+  load_state<NS>(q, i, s, m);
+  fr::apply_dq<NS>(m, dq, s);
+  store_state<NS>(q, i, s);
+}
+
+// ComputeAuxiliaryVariables in full for a download: P, rho, cv_i, conc_i
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_full_aux(const Model* __restrict__ mp, long long n,
+                                                    const double* __restrict__ q, double* __restrict__ aux) {
+  constexpr int NE = NS + 4, NA = 2 + 2 * NS;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Model& m = *mp;
+  State<NS> s;
+  load_state<NS>(q, i, s, m);
+  double* o = aux + i * NA;
+  o[0] = s.P;
+  o[1] = s.rho;
+  const double TDim = s.q[NS + 3] * m.ref_temperature;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    o[2 + k] = (fr::cp_species(m, k, TDim) - m.R[k]) / m.s_ref;
+    o[2 + NS + k] = s.q[k] / m.mw[k] / 1000.0;
+  }
+}
+
+}  // namespace
+
+// ---- context -----------------------------------------------------------------
+
+struct pcfd_fr {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+  std::string err;
+  const pcfd_chem* chem = nullptr;
+  int ns = 0, ne = 0, nnode = 0, nbnode = 0, nedge = 0, ntot = 0, ntag = 0, nbnd = 0;
+  int ncolor = 0, nchunk = 0, nrow_pad = 0;
+  long long nslot = 0;
+  std::vector<int> color_chunk, h_nptr, h_nslot, h_edges, h_srow, h_spos, h_sbase;
+  std::vector<long long> h_eoff;
+  Model model{};
+  Model* dmodel = nullptr;
+  pcfd_fr_params prm{};
+  bool have_params = false;
+  int iter = 0;
+  double cfl = 0.0;
+  bool lu_done = false;
+  double phase_ms[PCFD_FR_NPHASE] = {0};
+  long long bytes = 0;
+  // device arrays
+  double *q = nullptr, *qref = nullptr, *vol = nullptr, *edge_a = nullptr, *he_a = nullptr, *eflux = nullptr,
+         *elam = nullptr, *b = nullptr, *dt = nullptr, *jdb = nullptr, *red = nullptr, *aux = nullptr;
+  int *edges = nullptr, *nptr = nullptr, *nslot_d = nullptr, *he_ptr = nullptr, *he_id = nullptr, *he_tag = nullptr,
+      *bc_type = nullptr, *bnd = nullptr, *bidx = nullptr, *srow = nullptr, *swidth = nullptr, *sbase = nullptr,
+      *scol = nullptr, *icount = nullptr;
+  long long *eoff = nullptr, *piv = nullptr;
+  double *Md = nullptr, *Dd = nullptr, *xd = nullptr;
+  float *Mf = nullptr, *Df = nullptr, *xf = nullptr;
+};
+
+namespace {
+
+template <typename T>
+void dalloc(pcfd_fr& f, T** p, size_t n) {
+  hcheck(hipMalloc((void**)p, sizeof(T) * std::max<size_t>(n, 1)), "hipMalloc");
+  f.bytes += (long long)(sizeof(T) * std::max<size_t>(n, 1));
+}
+
+template <typename T>
+void dput(pcfd_fr& f, T** p, const std::vector<T>& h) {
+  dalloc(f, p, h.size());
+  if (!h.empty()) hcheck(hipMemcpy(*p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_all(pcfd_fr& f) {
+  dfree(f.q); dfree(f.qref); dfree(f.vol); dfree(f.edge_a); dfree(f.he_a); dfree(f.eflux); dfree(f.elam);
+  dfree(f.b); dfree(f.dt); dfree(f.jdb); dfree(f.red); dfree(f.aux); dfree(f.edges); dfree(f.nptr);
+  dfree(f.nslot_d); dfree(f.he_ptr); dfree(f.he_id); dfree(f.he_tag); dfree(f.bc_type); dfree(f.bnd);
+  dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount); dfree(f.eoff);
+  dfree(f.piv); dfree(f.Md); dfree(f.Dd); dfree(f.xd); dfree(f.Mf); dfree(f.Df); dfree(f.xf); dfree(f.dmodel);
+  if (f.e0) (void)hipEventDestroy(f.e0);
+  if (f.e1) (void)hipEventDestroy(f.e1);
+  if (f.e2) (void)hipEventDestroy(f.e2);
+  if (f.e3) (void)hipEventDestroy(f.e3);
+  if (f.stream) (void)hipStreamDestroy(f.stream);
+  f.e0 = f.e1 = f.e2 = f.e3 = nullptr;
+  f.stream = nullptr;
+}
+
+template <typename F>
+int guarded(pcfd_fr* f, F&& fn) {
+  if (!f) return PCFD_ERR_ARG;
+  try {
+    hcheck(hipSetDevice(f->device), "hipSetDevice");
+    return fn(*f);
+  } catch (const std::exception& e) {
+    f->err = e.what();
+    return PCFD_ERR_HIP;
+  }
+}
+
+int need_params(pcfd_fr& f) {
+  if (!f.have_params) {
+    f.err = "parameters not set (pcfd_fr_set_params)";
+    return PCFD_ERR_STATE;
+  }
+  return PCFD_OK;
+}
+
+bool fp32(const pcfd_fr& f) { return f.prm.precision == PCFD_CHOSEN_FP32; }
+
+// phase timers: [k] += device time between e0/e1 (or e2/e3 for a sub-timer)
+void tbeg(pcfd_fr& f) { hcheck(hipEventRecord(f.e0, f.stream), "hipEventRecord"); }
+void tend(pcfd_fr& f, int k) {
+  hcheck(hipEventRecord(f.e1, f.stream), "hipEventRecord");
+  hcheck(hipEventSynchronize(f.e1), "hipEventSynchronize");
+  float ms = 0.f;
+  hcheck(hipEventElapsedTime(&ms, f.e0, f.e1), "hipEventElapsedTime");
+  f.phase_ms[k] += ms;
+}
+void sbeg(pcfd_fr& f) { hcheck(hipEventRecord(f.e2, f.stream), "hipEventRecord"); }
+void send(pcfd_fr& f, int k) {
+  hcheck(hipEventRecord(f.e3, f.stream), "hipEventRecord");
+  hcheck(hipEventSynchronize(f.e3), "hipEventSynchronize");
+  float ms = 0.f;
+  hcheck(hipEventElapsedTime(&ms, f.e2, f.e3), "hipEventElapsedTime");
+  f.phase_ms[k] += ms;
+}
+
+constexpr int NS5 = 5;
+
+void run_bcs(pcfd_fr& f) {
+  if (f.nbnd == 0) return;
+  hipLaunchKernelGGL(k_fr_bcs<NS5>, dim3(nblocks(f.nbnd)), dim3(kFB), 0, f.stream, f.dmodel, f.nbnd, f.bnd, f.he_ptr,
+                     f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref, f.nnode, f.prm.beta, f.q);
+  hcheck(hipGetLastError(), "k_fr_bcs");
+}
+
+// residual into f.b; norms[1+neqn] (residGlobal, per-equation) when non-null
+void run_residual(pcfd_fr& f, double* norms) {
+  if (f.nedge > 0) {
+    sbeg(f);
+    hipLaunchKernelGGL(k_fr_edge_flux<NS5>, dim3(nblocks(f.nedge)), dim3(kFB), 0, f.stream, f.dmodel, f.nedge, f.edges,
+                       f.edge_a, f.q, f.prm.beta, f.eflux);
+    hcheck(hipGetLastError(), "k_fr_edge_flux");
+    send(f, 7);
+  }
+  const int nb = nblocks(f.nnode);
+  hipLaunchKernelGGL(k_fr_residual<NS5>, dim3(nb), dim3(kFB), 0, f.stream, f.dmodel, chem_device_model(f.chem),
+                     f.nnode, f.nptr, f.nslot_d, f.eflux, f.he_ptr, f.he_id, f.he_a, f.q, f.vol, f.prm.reactions,
+                     f.prm.beta, f.b, f.red);
+  hcheck(hipGetLastError(), "k_fr_residual");
+  if (norms) {
+    const int ne = f.ne;
+    std::vector<double> h((size_t)(ne + 1) * nb);
+    hcheck(hipMemcpyAsync(h.data(), f.red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, f.stream), "copy");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    for (int k = 0; k <= ne; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < nb; ++j) s += h[(size_t)k * nb + j];
+      norms[k] = k == 0 ? std::sqrt(s) / ((double)f.nnode * ne) : std::sqrt(s) / (double)f.nnode;
+    }
+  }
+}
+
+double run_timestep(pcfd_fr& f, double cfl, bool want_min) {
+  if (f.nedge > 0) {
+    hipLaunchKernelGGL(k_fr_edge_lam<NS5>, dim3(nblocks(f.nedge)), dim3(kFB), 0, f.stream, f.dmodel, f.nedge, f.edges,
+                       f.edge_a, f.q, f.prm.beta, f.elam);
+    hcheck(hipGetLastError(), "k_fr_edge_lam");
+  }
+  hipLaunchKernelGGL(k_fr_timestep<NS5>, dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream, f.dmodel, f.nnode, f.nptr,
+                     f.nslot_d, f.elam, f.he_ptr, f.he_id, f.he_a, f.q, f.vol, cfl, f.prm.beta, f.dt);
+  hcheck(hipGetLastError(), "k_fr_timestep");
+  if (!want_min) return 0.0;
+  std::vector<double> h(f.nnode);
+  hcheck(hipMemcpyAsync(h.data(), f.dt, sizeof(double) * f.nnode, hipMemcpyDeviceToHost, f.stream), "copy");
+  hcheck(hipStreamSynchronize(f.stream), "sync");
+  double m = h[0];
+  for (int i = 1; i < f.nnode; ++i) m = std::min(m, h[i]);
+  return m;
+}
+
+template <typename TM>
+void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
+  if (f.nedge > 0) {
+    sbeg(f);
+    hipLaunchKernelGGL((k_fr_jac_edge<NS5, TM>), dim3(nblocks(f.nedge)), dim3(kFB), 0, f.stream, f.dmodel, f.nedge,
+                       f.edges, f.edge_a, f.q, f.prm.beta, f.eoff, M);
+    hcheck(hipGetLastError(), "k_fr_jac_edge");
+    send(f, 6);
+  }
+  if (f.nbnd > 0) {
+    hipLaunchKernelGGL(k_fr_jac_bnd<NS5>, dim3(nblocks(f.nbnd)), dim3(kFB), 0, f.stream, f.dmodel, f.nbnd, f.bnd,
+                       f.he_ptr, f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref, f.nnode, f.prm.beta, f.q, f.jdb);
+    hcheck(hipGetLastError(), "k_fr_jac_bnd");
+  }
+  hipLaunchKernelGGL((k_fr_jac_diag<NS5, TM>), dim3(nblocks(f.nrow_pad, 64)), dim3(64), 0, f.stream, f.dmodel,
+                     chem_device_model(f.chem), f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.eoff, M, f.bidx, f.jdb, f.q,
+                     f.vol, f.dt, f.prm.reactions, f.prm.beta, D);
+  hcheck(hipGetLastError(), "k_fr_jac_diag");
+  f.lu_done = false;
+}
+
+void run_jacobian(pcfd_fr& f) {
+  if (fp32(f)) run_jacobian_t<float>(f, f.Mf, f.Df);
+  else run_jacobian_t<double>(f, f.Md, f.Dd);
+}
+
+void run_prepare(pcfd_fr& f) {
+  if (fp32(f))
+    hipLaunchKernelGGL((k_fr_prepare<NS5, float>), dim3(nblocks(f.nrow_pad, 64)), dim3(64), 0, f.stream, f.nrow_pad,
+                       f.Df, f.piv);
+  else
+    hipLaunchKernelGGL((k_fr_prepare<NS5, double>), dim3(nblocks(f.nrow_pad, 64)), dim3(64), 0, f.stream, f.nrow_pad,
+                       f.Dd, f.piv);
+  hcheck(hipGetLastError(), "k_fr_prepare");
+  f.lu_done = true;
+}
+
+template <typename TM>
+double xnorm(pcfd_fr& f, const TM* x) {
+  const int nb = 256;
+  const long long n = (long long)f.nnode * f.ne;
+  hipLaunchKernelGGL(k_fr_sumsq<TM>, dim3(nb), dim3(kFB), 0, f.stream, n, x, f.red);
+  hcheck(hipGetLastError(), "k_fr_sumsq");
+  double h[nb];
+  hcheck(hipMemcpyAsync(h, f.red, sizeof(h), hipMemcpyDeviceToHost, f.stream), "copy");
+  hcheck(hipStreamSynchronize(f.stream), "sync");
+  double s = 0.0;
+  for (int k = 0; k < nb; ++k) s += h[k];
+  return std::sqrt(s) / (double)n;
+}
+
+template <typename TM>
+void run_sgs_t(pcfd_fr& f, int nsgs, const TM* M, const TM* D, TM* x, double* delta) {
+  const long long n = (long long)f.nnode * f.ne;
+  hipLaunchKernelGGL(k_fr_zero<TM>, dim3(nblocks(n)), dim3(kFB), 0, f.stream, n, x);
+  hcheck(hipGetLastError(), "k_fr_zero");
+  double xold = 0.0, xn = 0.0;
+  for (int it = 0; it < nsgs; ++it) {
+    for (int dir = 0; dir < 2; ++dir)
+      for (int cc = 0; cc < f.ncolor; ++cc) {
+        const int col = dir == 0 ? cc : f.ncolor - 1 - cc;
+        const int row0 = f.color_chunk[col] * kLane;
+        const int nrows = (f.color_chunk[col + 1] - f.color_chunk[col]) * kLane;
+        if (nrows == 0) continue;
+        hipLaunchKernelGGL((k_fr_sgs_color<NS5, TM>), dim3(nblocks(nrows)), dim3(kFB), 0, f.stream, row0, nrows,
+                           f.srow, f.swidth, f.sbase, f.scol, M, D, f.piv, f.b, x);
+        hcheck(hipGetLastError(), "k_fr_sgs_color");
+      }
+    if (delta && it >= nsgs - 2) {
+      xold = xn;
+      xn = xnorm(f, x);
+    }
+  }
+  if (delta) *delta = std::fabs(xold - xn);
+}
+
+void run_sgs(pcfd_fr& f, int nsgs, double* delta) {
+  if (fp32(f)) run_sgs_t<float>(f, nsgs, f.Mf, f.Df, f.xf, delta);
+  else run_sgs_t<double>(f, nsgs, f.Md, f.Dd, f.xd, delta);
+}
+
+int run_apply(pcfd_fr& f) {
+  hcheck(hipMemsetAsync(f.icount, 0, sizeof(int), f.stream), "hipMemsetAsync");
+  if (fp32(f))
+    hipLaunchKernelGGL((k_fr_apply<NS5, float>), dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream, f.dmodel, f.nnode,
+                       f.xf, f.q, f.icount);
+  else
+    hipLaunchKernelGGL((k_fr_apply<NS5, double>), dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream, f.dmodel, f.nnode,
+                       f.xd, f.q, f.icount);
+  hcheck(hipGetLastError(), "k_fr_apply");
+  int nz = 0;
+  hcheck(hipMemcpyAsync(&nz, f.icount, sizeof(int), hipMemcpyDeviceToHost, f.stream), "copy");
+  hcheck(hipStreamSynchronize(f.stream), "sync");
+  return nz;
+}
+
+// Param::UpdateCFL without dynamic CFL (ucs/param.tcc:695-726)
+double cfl_of(const pcfd_fr_params& p, int iter) {
+  if (p.cfl_ramp > 0 && iter < p.cfl_ramp)
+    return (p.cfl - p.cfl_start) / (double)p.cfl_ramp * (double)iter + p.cfl_start;
+  return p.cfl;
+}
+
+// PreTimeAdvance + NewtonIterate + PostTimeAdvance (ucs/solutionSpace.tcc:572-916)
+double step(pcfd_fr& f, double* norms, int* nzeroed) {
+  const int jf = f.prm.jac_freq > 0 ? f.prm.jac_freq : 1;
+  if (f.prm.nsgs > 0 && (f.iter % jf == 0 || f.iter == 1)) {
+    tbeg(f);
+    run_jacobian(f);
+    tend(f, 0);
+  }
+  f.cfl = cfl_of(f.prm, f.iter);
+  tbeg(f);
+  run_timestep(f, f.cfl, false);
+  tend(f, 1);
+  tbeg(f);
+  run_bcs(f);
+  tend(f, 2);
+  std::vector<double> nrm(f.ne + 1);
+  tbeg(f);
+  run_residual(f, nrm.data());
+  tend(f, 3);
+  int nz = 0;
+  if (f.prm.nsgs > 0) {
+    tbeg(f);
+    // PrepareSGS factors the stored diagonal blocks every iteration, as the
+    // reference's NewtonIterate does
+    run_prepare(f);
+    run_sgs(f, f.prm.nsgs, nullptr);
+    tend(f, 4);
+    tbeg(f);
+    nz = run_apply(f);
+    tend(f, 5);
+  } else {
+    throw std::runtime_error("pcfd_fr_step: the explicit (nsgs = 0) update is not provided for FR");
+  }
+  if (norms) std::copy(nrm.begin(), nrm.end(), norms);
+  if (nzeroed) *nzeroed = nz;
+  f.iter++;
+  return nrm[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int pcfd_fr_create(const pcfd_mesh_desc* m, pcfd_chem* chem, const double* qref, pcfd_fr** out) {
+  if (!m || !chem || !qref || !out) {
+    g_fr_create_err = "pcfd_fr_create: null argument";
+    return PCFD_ERR_ARG;
+  }
+  *out = nullptr;
+  const int ns = chem_nspecies(chem);
+  if (ns != NS5) {
+    g_fr_create_err = "pcfd_fr_create: " + std::to_string(ns) + " species; this build supports 5";
+    return PCFD_ERR_UNSUPPORTED;
+  }
+  if (m->nnode <= 0 || m->nbnode < 0 || m->nedge < 0 || m->nfactag <= 0 || !m->vol || !m->bc_type ||
+      (m->nedge > 0 && (!m->edges || !m->edge_a)) || (m->nbnode > 0 && (!m->bedges || !m->bedge_a || !m->bfactag))) {
+    g_fr_create_err = "pcfd_fr_create: invalid mesh description";
+    return PCFD_ERR_ARG;
+  }
+  const int nn = m->nnode, nb = m->nbnode, ne = m->nedge, nt = m->nfactag, NE = ns + 4, NE2 = NE * NE;
+  for (long long e = 0; e < ne; ++e) {
+    const int a = m->edges[2 * e], c = m->edges[2 * e + 1];
+    if (a < 0 || c < 0 || a >= nn || c >= nn || a == c) {
+      g_fr_create_err = "pcfd_fr_create: edge " + std::to_string(e) + " has invalid nodes";
+      return PCFD_ERR_ARG;
+    }
+  }
+  for (int h = 0; h < nb; ++h) {
+    const int a = m->bedges[2 * h], ph = m->bedges[2 * h + 1], tag = m->bfactag[h];
+    if (a < 0 || a >= nn || ph != nn + h || tag < 0 || tag >= nt) {
+      g_fr_create_err = "pcfd_fr_create: half-edge " + std::to_string(h) + " invalid";
+      return PCFD_ERR_ARG;
+    }
+  }
+  for (int t = 0; t < nt; ++t) {
+    const int ty = m->bc_type[t];
+    if (!(ty == 0 || ty == 1 || ty == 2 || ty == 3 || ty == 6 || ty == 7 || ty == 8 || ty == 9)) {
+      bool used = false;
+      for (int h = 0; h < nb && !used; ++h) used = m->bfactag[h] == t;
+      if (used) {
+        g_fr_create_err = "pcfd_fr_create: face tag " + std::to_string(t) + " has a boundary type (" +
+                          std::to_string(ty) + ") the FR path does not provide";
+        return PCFD_ERR_UNSUPPORTED;
+      }
+    }
+  }
+  pcfd_fr* f = new pcfd_fr();
+  try {
+    hcheck(hipGetDevice(&f->device), "hipGetDevice");
+    hcheck(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hcheck(hipEventCreate(&f->e0), "hipEventCreate");
+    hcheck(hipEventCreate(&f->e1), "hipEventCreate");
+    hcheck(hipEventCreate(&f->e2), "hipEventCreate");
+    hcheck(hipEventCreate(&f->e3), "hipEventCreate");
+    f->chem = chem;
+    f->ns = ns;
+    f->ne = NE;
+    f->nnode = nn;
+    f->nbnode = nb;
+    f->nedge = ne;
+    f->ntot = nn + nb;
+    f->ntag = nt;
+    // node lists in edge order (the order the reference's edge loop adds to a node)
+    std::vector<int> nptr(nn + 1, 0);
+    for (int e = 0; e < ne; ++e) {
+      nptr[m->edges[2 * e] + 1]++;
+      nptr[m->edges[2 * e + 1] + 1]++;
+    }
+    for (int i = 0; i < nn; ++i) nptr[i + 1] += nptr[i];
+    std::vector<int> nslot(2 * (size_t)ne), nbr(2 * (size_t)ne);
+    {
+      std::vector<int> fill(nptr.begin(), nptr.end() - 1);
+      for (int e = 0; e < ne; ++e) {
+        const int a = m->edges[2 * e], c = m->edges[2 * e + 1];
+        nbr[fill[a]] = c;
+        nslot[fill[a]++] = 2 * e;
+        nbr[fill[c]] = a;
+        nslot[fill[c]++] = 2 * e + 1;
+      }
+      std::vector<int> stamp(nn, -1);
+      for (int i = 0; i < nn; ++i)
+        for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
+          if (stamp[nbr[p]] == i) throw std::runtime_error("pcfd_fr_create: duplicate edge between nodes");
+          stamp[nbr[p]] = i;
+        }
+    }
+    // half-edges per node, in half-edge order
+    std::vector<int> he_ptr(nn + 1, 0), he_id(nb);
+    for (int h = 0; h < nb; ++h) he_ptr[m->bedges[2 * h] + 1]++;
+    for (int i = 0; i < nn; ++i) he_ptr[i + 1] += he_ptr[i];
+    {
+      std::vector<int> fill(he_ptr.begin(), he_ptr.end() - 1);
+      for (int h = 0; h < nb; ++h) he_id[fill[m->bedges[2 * h]]++] = h;
+    }
+    std::vector<int> bnd, bidx(nn, -1);
+    for (int i = 0; i < nn; ++i)
+      if (he_ptr[i + 1] > he_ptr[i]) {
+        bidx[i] = (int)bnd.size();
+        bnd.push_back(i);
+      }
+    f->nbnd = (int)bnd.size();
+    // colouring and the SELL layout of the block matrix
+    std::vector<int> colour(nn);
+    const int ncol = pcfd_color_nodes(nn, ne, m->edges, colour.data());
+    if (ncol < 0) throw std::runtime_error(pcfd_mesh_error());
+    f->ncolor = ncol;
+    std::vector<std::vector<int>> members(ncol);
+    for (int i = 0; i < nn; ++i) members[colour[i]].push_back(i);
+    auto deg = [&](int i) { return nptr[i + 1] - nptr[i]; };
+    std::vector<int> srow;
+    f->color_chunk.assign(ncol + 1, 0);
+    for (int k = 0; k < ncol; ++k) {
+      auto& v = members[k];
+      std::stable_sort(v.begin(), v.end(), [&](int a, int c) { return deg(a) > deg(c); });
+      srow.insert(srow.end(), v.begin(), v.end());
+      while (srow.size() % kLane) srow.push_back(-1);
+      f->color_chunk[k + 1] = (int)(srow.size() / kLane);
+    }
+    f->nrow_pad = (int)srow.size();
+    f->nchunk = f->nrow_pad / kLane;
+    std::vector<int> swidth(f->nchunk, 0), sbase(f->nchunk, 0), spos(nn, -1);
+    long long slots = 0;
+    for (int k = 0; k < f->nchunk; ++k) {
+      int w = 0;
+      for (int l = 0; l < kLane; ++l) {
+        const int i = srow[(size_t)k * kLane + l];
+        if (i >= 0) {
+          w = std::max(w, deg(i));
+          spos[i] = k * kLane + l;
+        }
+      }
+      swidth[k] = w;
+      sbase[k] = (int)slots;
+      slots += w;
+    }
+    f->nslot = slots;
+    std::vector<int> scol((size_t)slots * kLane, -1);
+    std::vector<long long> eoff(2 * (size_t)ne);
+    for (int i = 0; i < nn; ++i) {
+      const int r = spos[i], k = r / kLane, l = r % kLane;
+      for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
+        const long long s = (long long)sbase[k] + (p - nptr[i]);
+        scol[(size_t)s * kLane + l] = nbr[p];
+        // row i's block for this neighbour: edge e's (n0,n1) block when i is n0
+        eoff[2 * (size_t)(nslot[p] >> 1) + (nslot[p] & 1)] = s * NE2 * kLane + l;
+      }
+    }
+    f->h_nptr = nptr;
+    f->h_nslot = nslot;
+    f->h_edges.assign(m->edges, m->edges + 2 * (size_t)ne);
+    f->h_srow = srow;
+    f->h_spos = spos;
+    f->h_sbase = sbase;
+    f->h_eoff = eoff;
+    // model constants (scales set with the parameters)
+    Model md{};
+    md.ns = ns;
+    // species data from the chemistry model
+    ChemDev cd{};
+    hcheck(hipMemcpy(&cd, chem_device_model(chem), sizeof(ChemDev), hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int s = 0; s < ns; ++s) {
+      md.mw[s] = cd.mw[s];
+      md.R[s] = fr::kUnivR / cd.mw[s];
+      for (int k = 0; k < 7; ++k) {
+        md.lo[s][k] = cd.lo[s][k];
+        md.hi[s][k] = cd.hi[s][k];
+      }
+    }
+    md.ref_density = cd.ref_density;
+    md.ref_temperature = cd.ref_temperature;
+    f->model = md;
+    // device arrays
+    std::vector<double> he_a(4 * (size_t)nb), vol(m->vol, m->vol + nn), ea(m->edge_a, m->edge_a + 4 * (size_t)ne);
+    for (int h = 0; h < nb; ++h)
+      for (int k = 0; k < 4; ++k) he_a[4 * (size_t)h + k] = m->bedge_a[4 * (size_t)h + k];
+    std::vector<int> edges(m->edges, m->edges + 2 * (size_t)ne), he_tag(m->bfactag, m->bfactag + nb),
+        bct(m->bc_type, m->bc_type + nt);
+    std::vector<double> qr(qref, qref + (size_t)nt * NE);
+    hcheck(hipMalloc((void**)&f->dmodel, sizeof(Model)), "hipMalloc");
+    dput(*f, &f->edges, edges);
+    dput(*f, &f->edge_a, ea);
+    dput(*f, &f->vol, vol);
+    dput(*f, &f->he_a, he_a);
+    dput(*f, &f->he_ptr, he_ptr);
+    dput(*f, &f->he_id, he_id);
+    dput(*f, &f->he_tag, he_tag);
+    dput(*f, &f->bc_type, bct);
+    dput(*f, &f->qref, qr);
+    dput(*f, &f->bnd, bnd);
+    dput(*f, &f->bidx, bidx);
+    dput(*f, &f->nptr, nptr);
+    dput(*f, &f->nslot_d, nslot);
+    dput(*f, &f->srow, srow);
+    dput(*f, &f->swidth, swidth);
+    dput(*f, &f->sbase, sbase);
+    dput(*f, &f->scol, scol);
+    dput(*f, &f->eoff, eoff);
+    dalloc(*f, &f->q, (size_t)f->ntot * NE);
+    dalloc(*f, &f->eflux, (size_t)ne * NE);
+    dalloc(*f, &f->elam, (size_t)ne);
+    dalloc(*f, &f->b, (size_t)nn * NE);
+    dalloc(*f, &f->dt, (size_t)nn);
+    dalloc(*f, &f->jdb, (size_t)std::max(f->nbnd, 1) * NE2);
+    dalloc(*f, &f->red, (size_t)(NE + 1) * std::max(nblocks(nn), 256));
+    dalloc(*f, &f->icount, 4);
+    dalloc(*f, &f->piv, (size_t)f->nrow_pad);
+    hcheck(hipMemset(f->q, 0, sizeof(double) * (size_t)f->ntot * NE), "hipMemset");
+  } catch (const std::exception& e) {
+    g_fr_create_err = e.what();
+    free_all(*f);
+    delete f;
+    return PCFD_ERR_HIP;
+  }
+  *out = f;
+  return PCFD_OK;
+}
+
+int pcfd_fr_destroy(pcfd_fr* f) {
+  if (!f) return PCFD_ERR_ARG;
+  free_all(*f);
+  delete f;
+  return PCFD_OK;
+}
+
+const char* pcfd_fr_last_error(const pcfd_fr* f) { return f ? f->err.c_str() : g_fr_create_err.c_str(); }
+
+int pcfd_fr_set_params(pcfd_fr* fp, const pcfd_fr_params* p) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!p || !(p->ref_pressure > 0.0) || !(p->ref_velocity > 0.0) || !(p->beta > 0.0) ||
+        (p->precision != PCFD_PLAIN_FP64 && p->precision != PCFD_CHOSEN_FP32) || p->nsgs < 0 ||
+        !(p->cfl > 0.0) || (p->cfl_ramp > 0 && !(p->cfl_start > 0.0))) {
+      f.err = "pcfd_fr_set_params: invalid parameters";
+      return PCFD_ERR_ARG;
+    }
+    const bool was32 = f.have_params && fp32(f);
+    f.prm = *p;
+    f.have_params = true;
+    Model& md = f.model;
+    md.ref_pressure = p->ref_pressure;
+    md.ref_velocity = p->ref_velocity;
+    md.vref2 = p->ref_velocity * p->ref_velocity;
+    md.s_ref = p->ref_velocity * p->ref_velocity / md.ref_temperature;
+    md.Pref = p->Pref;
+    md.sym2d = p->symmetry2d;
+    md.local_dt = p->local_dt;
+    md.dt_param = p->dt_param;
+    hcheck(hipMemcpy(f.dmodel, &md, sizeof(Model), hipMemcpyHostToDevice), "hipMemcpy");
+    const size_t NE2 = (size_t)f.ne * f.ne;
+    const size_t noff = (size_t)f.nslot * kLane * NE2, ndg = (size_t)f.nchunk * kLane * NE2;
+    if (fp32(f)) {
+      if (!f.Mf) {
+        dalloc(f, &f.Mf, noff);
+        dalloc(f, &f.Df, ndg);
+        dalloc(f, &f.xf, (size_t)f.nnode * f.ne);
+      }
+    } else if (!f.Md) {
+      dalloc(f, &f.Md, noff);
+      dalloc(f, &f.Dd, ndg);
+      dalloc(f, &f.xd, (size_t)f.nnode * f.ne);
+    }
+    if (was32 != fp32(f)) f.lu_done = false;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_info(const pcfd_fr* f, int* ncolor, long long* nblk, long long* device_bytes) {
+  if (!f) return PCFD_ERR_ARG;
+  if (ncolor) *ncolor = f->ncolor;
+  if (nblk) *nblk = (long long)f->nnode + 2LL * f->nedge;
+  if (device_bytes) *device_bytes = f->bytes;
+  return PCFD_OK;
+}
+
+int pcfd_fr_upload_q(pcfd_fr* fp, const double* q, int stride) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!q || stride < f.ne) {
+      f.err = "pcfd_fr_upload_q: stride must be at least ns + 4";
+      return PCFD_ERR_ARG;
+    }
+    std::vector<double> h((size_t)f.ntot * f.ne);
+    for (size_t i = 0; i < (size_t)f.ntot; ++i)
+      for (int k = 0; k < f.ne; ++k) h[i * f.ne + k] = q[i * stride + k];
+    hcheck(hipMemcpyAsync(f.q, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, f.stream), "copy");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_download_q(pcfd_fr* fp, double* q, int stride) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!q || stride < f.ne) {
+      f.err = "pcfd_fr_download_q: stride must be at least ns + 4";
+      return PCFD_ERR_ARG;
+    }
+    std::vector<double> h((size_t)f.ntot * f.ne);
+    hcheck(hipMemcpyAsync(h.data(), f.q, sizeof(double) * h.size(), hipMemcpyDeviceToHost, f.stream), "copy");
+    const int na = 2 + 2 * f.ns;
+    std::vector<double> ha;
+    const bool full = stride >= f.ne + na && f.have_params;
+    if (full) {
+      if (!f.aux) dalloc(f, &f.aux, (size_t)f.ntot * na);
+      hipLaunchKernelGGL(k_fr_full_aux<NS5>, dim3(nblocks(f.ntot)), dim3(kFB), 0, f.stream, f.dmodel,
+                         (long long)f.ntot, f.q, f.aux);
+      hcheck(hipGetLastError(), "k_fr_full_aux");
+      ha.resize((size_t)f.ntot * na);
+      hcheck(hipMemcpyAsync(ha.data(), f.aux, sizeof(double) * ha.size(), hipMemcpyDeviceToHost, f.stream), "copy");
+    }
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    for (size_t i = 0; i < (size_t)f.ntot; ++i) {
+      for (int k = 0; k < f.ne; ++k) q[i * stride + k] = h[i * f.ne + k];
+      if (full)
+        for (int k = 0; k < na; ++k) q[i * stride + f.ne + k] = ha[i * na + k];
+    }
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_update_bcs(pcfd_fr* fp) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    run_bcs(f);
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_residual(pcfd_fr* fp, double* b_out, double* norms) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    std::vector<double> nrm(f.ne + 1);
+    run_residual(f, nrm.data());
+    if (norms) std::copy(nrm.begin(), nrm.end(), norms);
+    if (b_out)
+      hcheck(hipMemcpyAsync(b_out, f.b, sizeof(double) * (size_t)f.nnode * f.ne, hipMemcpyDeviceToHost, f.stream),
+             "copy");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_timestep(pcfd_fr* fp, double cfl, double* dt_out, double* dtmin) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    const double m = run_timestep(f, cfl, true);
+    if (dtmin) *dtmin = m;
+    if (dt_out)
+      hcheck(hipMemcpyAsync(dt_out, f.dt, sizeof(double) * f.nnode, hipMemcpyDeviceToHost, f.stream), "copy");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_set_timestep(pcfd_fr* fp, const double* dt) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!dt) return PCFD_ERR_ARG;
+    hcheck(hipMemcpyAsync(f.dt, dt, sizeof(double) * f.nnode, hipMemcpyHostToDevice, f.stream), "copy");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_jacobian(pcfd_fr* fp) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    run_jacobian(f);
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_download_jacobian(pcfd_fr* fp, const int* ia, const int* ja, double* M) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    if (!ia || !ja || !M) return PCFD_ERR_ARG;
+    if (f.lu_done) {
+      f.err = "pcfd_fr_download_jacobian: the diagonal blocks are factored (call pcfd_fr_jacobian again)";
+      return PCFD_ERR_STATE;
+    }
+    const size_t NE2 = (size_t)f.ne * f.ne;
+    const size_t noff = (size_t)f.nslot * kLane * NE2, ndg = (size_t)f.nchunk * kLane * NE2;
+    std::vector<double> off(noff), dg(ndg);
+    if (fp32(f)) {
+      std::vector<float> a(noff), c(ndg);
+      hcheck(hipMemcpy(a.data(), f.Mf, sizeof(float) * noff, hipMemcpyDeviceToHost), "copy");
+      hcheck(hipMemcpy(c.data(), f.Df, sizeof(float) * ndg, hipMemcpyDeviceToHost), "copy");
+      std::copy(a.begin(), a.end(), off.begin());
+      std::copy(c.begin(), c.end(), dg.begin());
+    } else {
+      hcheck(hipMemcpy(off.data(), f.Md, sizeof(double) * noff, hipMemcpyDeviceToHost), "copy");
+      hcheck(hipMemcpy(dg.data(), f.Dd, sizeof(double) * ndg, hipMemcpyDeviceToHost), "copy");
+    }
+    for (int i = 0; i < f.nnode; ++i) {
+      if (ia[i + 1] - ia[i] != f.h_nptr[i + 1] - f.h_nptr[i] + 1) {
+        f.err = "pcfd_fr_download_jacobian: row " + std::to_string(i) + " has a different block count";
+        return PCFD_ERR_ARG;
+      }
+      const int r = f.h_spos[i];
+      for (int k = ia[i]; k < ia[i + 1]; ++k) {
+        const int j = ja[k];
+        double* out = M + (size_t)k * NE2;
+        if (j == i) {
+          const size_t base = (size_t)(r / kLane) * NE2 * kLane + r % kLane;
+          for (size_t c = 0; c < NE2; ++c) out[c] = dg[base + c * kLane];
+          continue;
+        }
+        int p = f.h_nptr[i];
+        const int pe = f.h_nptr[i + 1];
+        for (; p < pe; ++p) {
+          const int e = f.h_nslot[p] >> 1, side = f.h_nslot[p] & 1;
+          if (f.h_edges[2 * (size_t)e + (1 - side)] == j) break;
+        }
+        if (p == pe) {
+          f.err = "pcfd_fr_download_jacobian: (" + std::to_string(i) + "," + std::to_string(j) + ") is not a block";
+          return PCFD_ERR_ARG;
+        }
+        const long long base = f.h_eoff[2 * (size_t)(f.h_nslot[p] >> 1) + (f.h_nslot[p] & 1)];
+        for (size_t c = 0; c < NE2; ++c) out[c] = off[(size_t)base + c * kLane];
+      }
+    }
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_pattern(pcfd_fr* fp, int* ia, int* ja) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!ia || !ja) return PCFD_ERR_ARG;
+    int k = 0;
+    for (int i = 0; i < f.nnode; ++i) {
+      ia[i] = k;
+      ja[k++] = i;
+      for (int p = f.h_nptr[i]; p < f.h_nptr[i + 1]; ++p) {
+        const int e = f.h_nslot[p] >> 1, side = f.h_nslot[p] & 1;
+        ja[k++] = f.h_edges[2 * (size_t)e + (1 - side)];
+      }
+    }
+    ia[f.nnode] = k;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_sgs(pcfd_fr* fp, int nsgs, double* x_out, double* delta) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    if (nsgs < 0) return PCFD_ERR_ARG;
+    if (!f.lu_done) run_prepare(f);
+    run_sgs(f, nsgs, delta);
+    if (x_out) {
+      const size_t n = (size_t)f.nnode * f.ne;
+      if (fp32(f)) {
+        std::vector<float> h(n);
+        hcheck(hipMemcpyAsync(h.data(), f.xf, sizeof(float) * n, hipMemcpyDeviceToHost, f.stream), "copy");
+        hcheck(hipStreamSynchronize(f.stream), "sync");
+        std::copy(h.begin(), h.end(), x_out);
+      } else {
+        hcheck(hipMemcpyAsync(x_out, f.xd, sizeof(double) * n, hipMemcpyDeviceToHost, f.stream), "copy");
+      }
+    }
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_sgs_order(pcfd_fr* fp, int* order) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (!order) return PCFD_ERR_ARG;
+    int k = 0;
+    for (int r : f.h_srow)
+      if (r >= 0) order[k++] = r;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_apply_dq(pcfd_fr* fp, int* nzeroed) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    const int nz = run_apply(f);
+    if (nzeroed) *nzeroed = nz;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_init_iterations(pcfd_fr* fp) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    // PreIterate (ucs/solutionSpace.tcc:509-569): the counter starts at 0,
+    // the first time step at iteration 0's CFL
+    f.iter = 0;
+    f.cfl = cfl_of(f.prm, 0);
+    run_timestep(f, f.cfl, false);
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_step(pcfd_fr* fp, double* norms, int* nzeroed) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    step(f, norms, nzeroed);
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_solve(pcfd_fr* fp, int maxiter, double drop, double* history, int* iters) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (int rc = need_params(f)) return rc;
+    if (maxiter < 0) return PCFD_ERR_ARG;
+    double r0 = 0.0;
+    int it = 0;
+    for (; it < maxiter;) {
+      const double r = step(f, nullptr, nullptr);
+      if (history) history[it] = r;
+      if (it == 0) r0 = r;
+      ++it;
+      if (!(r > drop * r0)) break;
+    }
+    if (iters) *iters = it;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_phase_times(pcfd_fr* f, double* ms, int reset) {
+  if (!f) return PCFD_ERR_ARG;
+  if (ms)
+    for (int k = 0; k < PCFD_FR_NPHASE; ++k) ms[k] = f->phase_ms[k];
+  if (reset)
+    for (int k = 0; k < PCFD_FR_NPHASE; ++k) f->phase_ms[k] = 0.0;
+  return PCFD_OK;
+}
+
+}  // extern "C"

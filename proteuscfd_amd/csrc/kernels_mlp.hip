@@ -769,9 +769,9 @@ int pcfd_mlp_train(pcfd_mlp* c, int n, const float* x, const float* t, int steps
 // lnrho_half, f2..f6 = (Y_i - y_mid)/y_scale.  Targets: asinh(wdot_i /
 // (rho s0)) / sigma_i, inverted here.  A cell falls back to the exact rates
 // when it lies outside the training box or when the predicted rates break
-// element conservation -- the exact rates conserve N and O atoms exactly
-// (every reaction does), so the molar imbalance
-//   |w_N/M_N + 2 w_N2/M_N2 + w_NO/M_NO| + |w_O/M_O + 2 w_O2/M_O2 + w_NO/M_NO|
+// element conservation -- the exact rates conserve every element's atoms
+// exactly (every reaction does), so the molar imbalance of the two elements,
+//   sum_e |sum_i atoms[e][i] w_i / M_i|   (for air: N and O atoms),
 // over (sum_i |w_i/M_i| + rho * floor) estimates the surrogate's error.
 namespace pcfd {
 
@@ -781,6 +781,7 @@ struct SurrogateDev {
   double sigma[5];
   double T_min, T_max, rho_min, rho_max, tol, floor;
   double imw[5];  // 1 / molecular weight
+  double atoms[2][5];
 };
 
 __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, SurrogateDev sd, int n,
@@ -834,8 +835,14 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
         m[k] = wv5[k] * sd.imw[k];
         mag += fabs(m[k]);
       }
-      // element balance, species order N, N2, O2, NO, O
-      const double eN = fabs(m[0] + 2.0 * m[1] + m[3]), eO = fabs(m[4] + 2.0 * m[2] + m[3]);
+      double eN = 0.0, eO = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        eN += sd.atoms[0][k] * m[k];
+        eO += sd.atoms[1][k] * m[k];
+      }
+      eN = fabs(eN);
+      eO = fabs(eO);
       const bool out = !(Tc >= sd.T_min && Tc <= sd.T_max && rho >= sd.rho_min && rho <= sd.rho_max);
       const bool bad = (eN + eO) > sd.tol * (mag + rho * sd.floor);
 #pragma unroll
@@ -986,6 +993,8 @@ int surrogate_launch(pcfd_chem* chem, pcfd_mlp& m, const pcfd_surrogate_desc* d,
   for (int k = 0; k < 5; ++k) {
     sd.sigma[k] = d->sigma[k];
     sd.imw[k] = 1.0 / chem_mw(chem)[k];
+    sd.atoms[0][k] = d->atoms[0][k];
+    sd.atoms[1][k] = d->atoms[1][k];
   }
   sd.T_min = d->T_min;
   sd.T_max = d->T_max;

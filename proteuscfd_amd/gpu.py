@@ -483,7 +483,8 @@ class SurrogateDesc(C.Structure):
     _fields_ = [("lnT_mid", C.c_double), ("lnT_half", C.c_double), ("lnrho_mid", C.c_double),
                 ("lnrho_half", C.c_double), ("y_mid", C.c_double), ("y_scale", C.c_double), ("y_eps", C.c_double),
                 ("rate_scale", C.c_double), ("sigma", C.c_double * 5), ("T_min", C.c_double), ("T_max", C.c_double),
-                ("rho_min", C.c_double), ("rho_max", C.c_double), ("tol", C.c_double), ("floor", C.c_double)]
+                ("rho_min", C.c_double), ("rho_max", C.c_double), ("tol", C.c_double), ("floor", C.c_double),
+                ("atoms", (C.c_double * 5) * 2)]
 
 
 def _mlp_lib():
@@ -579,3 +580,197 @@ class MLP:
     @property
     def last_ms(self):
         return _mlp_lib().pcfd_mlp_last_kernel_ms(self.h)
+
+
+# ---------------------------------------------------------------------------
+# finite-rate reacting Euler (include/pcfd_fr.h, pcfd_fr_*)
+
+FR_SYMBOLS = ["pcfd_fr_create", "pcfd_fr_destroy", "pcfd_fr_last_error", "pcfd_fr_set_params", "pcfd_fr_info",
+              "pcfd_fr_upload_q", "pcfd_fr_download_q", "pcfd_fr_update_bcs", "pcfd_fr_residual", "pcfd_fr_timestep",
+              "pcfd_fr_set_timestep", "pcfd_fr_jacobian", "pcfd_fr_download_jacobian", "pcfd_fr_pattern",
+              "pcfd_fr_sgs", "pcfd_fr_sgs_order", "pcfd_fr_apply_dq", "pcfd_fr_init_iterations", "pcfd_fr_step",
+              "pcfd_fr_solve", "pcfd_fr_phase_times"]
+FR_PHASES = ["jacobian", "timestep", "bcs", "residual", "sgs", "update", "k_jac_edge", "k_edge_flux"]
+
+
+class FrParams(C.Structure):
+    _fields_ = [("ref_pressure", C.c_double), ("ref_velocity", C.c_double), ("Pref", C.c_double),
+                ("beta", C.c_double), ("symmetry2d", C.c_int), ("reactions", C.c_int), ("local_dt", C.c_int),
+                ("dt_param", C.c_double), ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_ramp", C.c_int),
+                ("nsgs", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int)]
+
+
+def _fr_lib():
+    L = lib()
+    if not getattr(L, "_fr_ready", False):
+        vp = C.c_void_p
+        L.pcfd_fr_create.argtypes = [C.POINTER(MeshDesc), vp, dp, C.POINTER(vp)]
+        L.pcfd_fr_destroy.argtypes = [vp]
+        L.pcfd_fr_last_error.argtypes = [vp]
+        L.pcfd_fr_last_error.restype = C.c_char_p
+        L.pcfd_fr_set_params.argtypes = [vp, C.POINTER(FrParams)]
+        L.pcfd_fr_info.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        L.pcfd_fr_upload_q.argtypes = [vp, dp, C.c_int]
+        L.pcfd_fr_download_q.argtypes = [vp, dp, C.c_int]
+        L.pcfd_fr_update_bcs.argtypes = [vp]
+        L.pcfd_fr_residual.argtypes = [vp, dp, dp]
+        L.pcfd_fr_timestep.argtypes = [vp, C.c_double, dp, dp]
+        L.pcfd_fr_set_timestep.argtypes = [vp, dp]
+        L.pcfd_fr_jacobian.argtypes = [vp]
+        L.pcfd_fr_download_jacobian.argtypes = [vp, ip, ip, dp]
+        L.pcfd_fr_pattern.argtypes = [vp, ip, ip]
+        L.pcfd_fr_sgs.argtypes = [vp, C.c_int, dp, dp]
+        L.pcfd_fr_sgs_order.argtypes = [vp, ip]
+        L.pcfd_fr_apply_dq.argtypes = [vp, ip]
+        L.pcfd_fr_init_iterations.argtypes = [vp]
+        L.pcfd_fr_step.argtypes = [vp, dp, ip]
+        L.pcfd_fr_solve.argtypes = [vp, C.c_int, C.c_double, dp, ip]
+        L.pcfd_fr_phase_times.argtypes = [vp, dp, C.c_int]
+        L._fr_ready = True
+    return L
+
+
+class FrContext:
+    """One finite-rate (FR) context on the current HIP device.
+
+    mesh: a proteuscfd_amd.mesh.Mesh-like object (edges, edge_a, bedges,
+    bedge_a, bfactag, xyz, vol); bc_type [ntag]; qref [ntag, ns+4] native
+    reference states; chem: a ChemContext (kept alive by this object)."""
+
+    def __init__(self, mesh, bc_type, qref, chem: ChemContext):
+        L = _fr_lib()
+        self.chem = chem
+        self.ns = chem.model.ns
+        self.neqn, self.nvars = self.ns + 4, 3 * self.ns + 6
+        self.nnode, self.nbnode, self.nedge = int(mesh.nnode), int(mesh.nbedge), int(mesh.nedge)
+        bc_type = _i32(bc_type)
+        qref = _f64(np.asarray(qref)[:, :self.neqn])
+        if qref.shape != (len(bc_type), self.neqn):
+            raise ValueError(f"qref has shape {qref.shape}; needs ({len(bc_type)}, {self.neqn})")
+        self._keep = dict(edges=_i32(mesh.edges), edge_a=_f64(mesh.edge_a), bedges=_i32(mesh.bedges),
+                          bedge_a=_f64(mesh.bedge_a), bfactag=_i32(mesh.bfactag), xyz=_f64(mesh.xyz),
+                          vol=_f64(mesh.vol), bc_type=bc_type, qref5=np.zeros((len(bc_type), 5)),
+                          twall=np.full(len(bc_type), -1.0), qref=qref)
+        k = self._keep
+        desc = MeshDesc(self.nnode, self.nbnode, self.nedge, _i(k["edges"]), _d(k["edge_a"]), _i(k["bedges"]),
+                        _d(k["bedge_a"]), _i(k["bfactag"]), _d(k["xyz"]), _d(k["vol"]), len(bc_type),
+                        _i(k["bc_type"]), _d(k["qref5"]), _d(k["twall"]))
+        h = C.c_void_p()
+        rc = L.pcfd_fr_create(C.byref(desc), chem.h, _d(k["qref"]), C.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"pcfd_fr_create failed ({rc}): {L.pcfd_fr_last_error(None).decode()}")
+        self.h = h
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} failed ({rc}): {_fr_lib().pcfd_fr_last_error(self.h).decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            _fr_lib().pcfd_fr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, ref_pressure, ref_velocity, Pref, beta, cfl=1.0, cfl_start=1.0, cfl_ramp=0, nsgs=4,
+                   jac_freq=1, precision=PLAIN_FP64, reactions=True, local_dt=True, dt_param=-1.0, symmetry2d=0):
+        p = FrParams(float(ref_pressure), float(ref_velocity), float(Pref), float(beta), int(symmetry2d),
+                     int(reactions), int(local_dt), float(dt_param), float(cfl), float(cfl_start), int(cfl_ramp),
+                     int(nsgs), int(jac_freq), int(precision))
+        self._chk(_fr_lib().pcfd_fr_set_params(self.h, C.byref(p)), "pcfd_fr_set_params")
+
+    def info(self):
+        nc, nb, by = C.c_int(), C.c_longlong(), C.c_longlong()
+        self._chk(_fr_lib().pcfd_fr_info(self.h, C.byref(nc), C.byref(nb), C.byref(by)), "pcfd_fr_info")
+        return dict(ncolor=nc.value, nblocks=nb.value, device_bytes=by.value)
+
+    def upload_q(self, q):
+        q = _f64(q)
+        if q.ndim != 2 or q.shape[0] != self.nnode + self.nbnode or q.shape[1] < self.neqn:
+            raise ValueError(f"q has shape {q.shape}; needs ({self.nnode + self.nbnode}, >= {self.neqn})")
+        self._chk(_fr_lib().pcfd_fr_upload_q(self.h, _d(q), q.shape[1]), "pcfd_fr_upload_q")
+
+    def download_q(self, full=True):
+        q = np.zeros((self.nnode + self.nbnode, self.nvars if full else self.neqn))
+        self._chk(_fr_lib().pcfd_fr_download_q(self.h, _d(q), q.shape[1]), "pcfd_fr_download_q")
+        return q
+
+    def update_bcs(self):
+        self._chk(_fr_lib().pcfd_fr_update_bcs(self.h), "pcfd_fr_update_bcs")
+
+    def residual(self, download=True):
+        b = np.zeros((self.nnode, self.neqn)) if download else None
+        norms = np.zeros(1 + self.neqn)
+        self._chk(_fr_lib().pcfd_fr_residual(self.h, _d(b) if download else None, _d(norms)), "pcfd_fr_residual")
+        return b, norms
+
+    def timestep(self, cfl, download=True):
+        dt = np.zeros(self.nnode) if download else None
+        dtmin = C.c_double()
+        self._chk(_fr_lib().pcfd_fr_timestep(self.h, float(cfl), _d(dt) if download else None, C.byref(dtmin)),
+                  "pcfd_fr_timestep")
+        return dt, dtmin.value
+
+    def set_timestep(self, dt):
+        dt = _f64(dt)
+        if dt.shape != (self.nnode,):
+            raise ValueError(f"dt has shape {dt.shape}; needs ({self.nnode},)")
+        self._chk(_fr_lib().pcfd_fr_set_timestep(self.h, _d(dt)), "pcfd_fr_set_timestep")
+
+    def jacobian(self):
+        self._chk(_fr_lib().pcfd_fr_jacobian(self.h), "pcfd_fr_jacobian")
+
+    def pattern(self):
+        ia = np.zeros(self.nnode + 1, dtype=np.int32)
+        ja = np.zeros(self.nnode + 2 * self.nedge, dtype=np.int32)
+        self._chk(_fr_lib().pcfd_fr_pattern(self.h, _i(ia), _i(ja)), "pcfd_fr_pattern")
+        return ia, ja
+
+    def download_jacobian(self, ia, ja):
+        ia, ja = _i32(ia), _i32(ja)
+        if ia.shape != (self.nnode + 1,) or ja.shape != (self.nnode + 2 * self.nedge,) or ia[-1] != len(ja):
+            raise ValueError("ia/ja do not describe this mesh's block pattern")
+        M = np.zeros((len(ja), self.neqn, self.neqn))
+        self._chk(_fr_lib().pcfd_fr_download_jacobian(self.h, _i(ia), _i(ja), _d(M)), "pcfd_fr_download_jacobian")
+        return M
+
+    def sgs(self, nsgs, download=True):
+        x = np.zeros((self.nnode, self.neqn)) if download else None
+        delta = C.c_double()
+        self._chk(_fr_lib().pcfd_fr_sgs(self.h, int(nsgs), _d(x) if download else None, C.byref(delta)),
+                  "pcfd_fr_sgs")
+        return x, delta.value
+
+    def sgs_order(self):
+        o = np.zeros(self.nnode, dtype=np.int32)
+        self._chk(_fr_lib().pcfd_fr_sgs_order(self.h, _i(o)), "pcfd_fr_sgs_order")
+        return o
+
+    def apply_dq(self):
+        n = C.c_int()
+        self._chk(_fr_lib().pcfd_fr_apply_dq(self.h, C.byref(n)), "pcfd_fr_apply_dq")
+        return n.value
+
+    def init_iterations(self):
+        self._chk(_fr_lib().pcfd_fr_init_iterations(self.h), "pcfd_fr_init_iterations")
+
+    def step(self):
+        norms = np.zeros(1 + self.neqn)
+        nz = C.c_int()
+        self._chk(_fr_lib().pcfd_fr_step(self.h, _d(norms), C.byref(nz)), "pcfd_fr_step")
+        return norms[0], norms, nz.value
+
+    def solve(self, max_iter, drop=1e-8):
+        hist = np.zeros(max_iter)
+        n = C.c_int()
+        self._chk(_fr_lib().pcfd_fr_solve(self.h, int(max_iter), float(drop), _d(hist), C.byref(n)), "pcfd_fr_solve")
+        return hist[:n.value]
+
+    def phase_times(self, reset=False):
+        ms = np.zeros(len(FR_PHASES))
+        self._chk(_fr_lib().pcfd_fr_phase_times(self.h, _d(ms), int(reset)), "pcfd_fr_phase_times")
+        return dict(zip(FR_PHASES, ms))

@@ -14,6 +14,7 @@ by more than the bound, to the exact rates.
 from __future__ import annotations
 
 import ctypes as C
+import re
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -67,9 +68,23 @@ class Transforms:
         self.sigma = np.maximum(np.arcsinh(wdot / (rho[:, None] * self.rate_scale)).std(axis=0), 1e-3)
 
 
-def desc(tr: Transforms, tol=0.05, floor=10.0):
-    from . import gpu
+def atom_counts(symbols, elements=("N", "O")):
+    """Atoms of each element in each species symbol (e.g. "N2" -> N: 2)."""
+    out = np.zeros((len(elements), len(symbols)))
+    for k, sym in enumerate(symbols):
+        for el, n in re.findall(r"([A-Z][a-z]?)(\d*)", sym):
+            if el in elements:
+                out[elements.index(el), k] += int(n) if n else 1
+    return out
+
+
+def desc(tr: Transforms, tol=0.05, floor=10.0, symbols=None):
+    from . import chem, gpu
     d = gpu.SurrogateDesc()
+    atoms = atom_counts(symbols if symbols is not None else chem.load_air5().symbols)
+    for e in range(2):
+        for k in range(5):
+            d.atoms[e][k] = float(atoms[e, k])
     d.lnT_mid, d.lnT_half, d.lnrho_mid, d.lnrho_half = tr.lnT_mid, tr.lnT_half, tr.lnrho_mid, tr.lnrho_half
     d.y_mid, d.y_scale, d.y_eps, d.rate_scale = tr.y_mid, tr.y_scale, tr.y_eps, tr.rate_scale
     for k in range(5):

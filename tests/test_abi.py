@@ -16,7 +16,7 @@ def declared(header):
     return sorted(set(re.findall(r"\b(pcfd_[a-z0-9_]+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("lib,headers", [("libpcfd.so", ["pcfd.h", "pcfd_mesh.h", "pcfd_chem.h"]),
+@pytest.mark.parametrize("lib,headers", [("libpcfd.so", ["pcfd.h", "pcfd_mesh.h", "pcfd_chem.h", "pcfd_fr.h"]),
                                          ("libpcfd_mesh.so", ["pcfd_mesh.h"]),
                                          ("libpcfd_h5.so", ["pcfd_mesh.h", "pcfd_mesh_h5.h"])])
 def test_library_exports_declared_symbols(lib, headers):
@@ -39,6 +39,7 @@ def test_python_binding_covers_the_abi():
     from proteuscfd_amd import gpu
     assert sorted(gpu.SYMBOLS) == declared("pcfd.h")
     assert sorted(gpu.CHEM_SYMBOLS) == declared("pcfd_chem.h")
+    assert sorted(gpu.FR_SYMBOLS) == declared("pcfd_fr.h")
 
 
 def test_version_string_without_gpu():
@@ -57,3 +58,12 @@ def test_create_rejects_bad_mesh_before_touching_the_device():
     h = ctypes.c_void_p()
     assert L.pcfd_create(ctypes.byref(desc), ctypes.byref(h)) == -1
     assert b"invalid" in L.pcfd_last_error(None)
+
+
+def test_fr_create_rejects_bad_arguments_on_the_host():
+    """pcfd_fr_create refuses null arguments before any HIP call."""
+    from proteuscfd_amd import gpu
+    L = gpu._fr_lib()
+    h = ctypes.c_void_p()
+    assert L.pcfd_fr_create(None, None, None, ctypes.byref(h)) == -1
+    assert b"null" in L.pcfd_fr_last_error(None)

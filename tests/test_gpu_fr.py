@@ -1,0 +1,210 @@
+"""The HIP finite-rate (FR) path (include/pcfd_fr.h) against the reference
+solver's own FR vectors (tests/golden/ref_fr_*.npz) and the oracle: one
+implicit step phase by phase, the steady reacting solve in both precision
+forms, and bitwise reruns.
+
+Where the arithmetic is the reference's (HLLC fluxes, characteristic boundary
+states, time steps, the flux Jacobian, SGS, ApplyDQ) the GPU agrees to the
+bit; the finite-rate source uses exp/pow/log, whose device and host library
+results differ in the last bit, so quantities that contain it are compared
+to stated tolerances: 1e-13 of a row's scale for the residual, and 1e-6 of a
+block row's scale for the diagonal blocks (the source Jacobian is a forward
+difference with h = 1e-8, which scales a last-bit difference of the source
+by 1e8)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from proteuscfd_amd import chem, gpu
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MODEL = chem.load_air5()
+NS = MODEL.ns
+NE = NS + 4
+SCALES = ["ref_pressure", "ref_velocity", "ref_density", "ref_time", "ref_temperature"]
+BCT = {"fr_wedge_phases": [0, 7, 8, 3, 3, 9, 9], "fr_bump_phases": [0, 6, 6, 6, 6, 3, 6],
+       "fr_bump_solve": [0, 6, 6, 6, 6, 3, 6]}
+
+
+def load(name):
+    return dict(np.load(os.path.join(GOLD, f"ref_{name}.npz")))
+
+
+class _Mesh:
+    def __init__(self, d):
+        for k, v in d.items():
+            if k.startswith("mesh_"):
+                setattr(self, k[5:], v)
+        self.nnode, self.nbedge, self.nedge = len(self.vol), len(self.bfactag), len(self.edges)
+
+
+def context(name, precision=gpu.PLAIN_FP64, **kw):
+    d = load(name)
+    model = MODEL.with_reference(float(d["ref_density"]), float(d["ref_time"]), float(d["ref_temperature"]))
+    cc = gpu.ChemContext(model)
+    bct = np.array(BCT[name], dtype=np.int32)
+    f = gpu.FrContext(_Mesh(d), bct, np.tile(d["qinf"][:NE], (len(bct), 1)), cc)
+    f.set_params(float(d["ref_pressure"]), float(d["ref_velocity"]), float(d["Pref"]), float(d["beta"]),
+                 precision=precision, **kw)
+    fr = O.FR(MODEL, {k: float(d[k]) for k in SCALES}, Pref=float(d["Pref"]), beta=float(d["beta"]))
+    return d, f, fr
+
+
+def row_rel(a, b):
+    """max |a - b| over each row's max |b| (rows of the last axis)."""
+    a, b = np.asarray(a), np.asarray(b)
+    s = np.maximum(np.abs(b).max(axis=-1, keepdims=True), 1e-300)
+    return float((np.abs(a - b) / s).max())
+
+
+@pytest.mark.parametrize("name", ["fr_wedge_phases", "fr_bump_phases"])
+def test_fr_phases_against_reference(name):
+    print(f"[fr phases] {name}", flush=True)
+    d, f, fr = context(name, cfl=float(d_cfl(name)), nsgs=int(load(name)["nsgs"]))
+    nn = f.nnode
+    f.upload_q(d["q_in"])
+    f.update_bcs()
+    q = f.download_q()
+    np.testing.assert_array_equal(q[:, :NE], d["q_bc"][:, :NE])
+    assert np.abs(q - d["q_bc"]).max() <= 1e-15 * np.abs(d["q_bc"]).max()
+    b, norms = f.residual()
+    assert row_rel(b, d["residual"]) < 1e-13
+    np.testing.assert_allclose(norms, d["residual_norms"], rtol=1e-12)
+    dt, dtmin = f.timestep(float(d["cfl"]))
+    np.testing.assert_array_equal(dt, d["dt"])
+    f.set_timestep(d["dt"])
+    f.jacobian()
+    qj = f.download_q()
+    M = f.download_jacobian(d["ia"], d["ja"])
+    sel = np.concatenate([np.arange(d["ia"][i], d["ia"][i + 1]) for i in d["jac_rows"]])
+    Mr = M[sel]
+    diag = np.array([d["ja"][k] == i for i in d["jac_rows"] for k in range(d["ia"][i], d["ia"][i + 1])])
+    # off-diagonal blocks: the flux Jacobian, the reference's arithmetic
+    np.testing.assert_array_equal(Mr[~diag], d["jac_blocks"][~diag])
+    # diagonal blocks carry the source Jacobian (device exp/pow)
+    assert row_rel(Mr[diag], d["jac_blocks"][diag]) < 1e-6
+    # SGS on the device matrix against the oracle's sweep in the device's row order
+    ia, ja = f.pattern()
+    Mg = f.download_jacobian(ia, ja)
+    x, delta = f.sgs(int(d["nsgs"]))
+    xo, _ = O.fr_sgs_on_matrix(ia, ja, ia[:-1].copy(), Mg, b, int(d["nsgs"]), order=f.sgs_order())
+    assert row_rel(x, xo) < 1e-13
+    # the boundary Jacobian left the stored states as the reference's in-place BC call does
+    np.testing.assert_array_equal(qj[:, :NE], O.FRProblem(_Mesh(d), np.array(BCT[name], dtype=np.int32),
+                                                          np.tile(d["qinf"], (len(BCT[name]), 1)),
+                                                          fr).jacobian(fr.aux(d["q_bc"]), d["dt"])[4][:, :NE])
+    nz = f.apply_dq()
+    assert nz == 0
+    qn = f.download_q()
+    want = np.array([fr.apply_dq(x[i], fr.aux(qj[i])[0]) for i in range(nn)])
+    np.testing.assert_array_equal(qn[:nn, :NE], want[:, :NE])
+
+
+def d_cfl(name):
+    return load(name)["cfl"]
+
+
+def _solve(precision, steps=None, drop=1e-8):
+    d, f, _ = context("fr_bump_solve", precision=precision, cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6)
+    q0 = np.tile(d["qinf"], (f.nnode + f.nbnode, 1))
+    f.upload_q(q0)
+    f.init_iterations()
+    if steps is not None:
+        h = np.array([f.step()[0] for _ in range(steps)])
+    else:
+        h = f.solve(400, drop)
+    return d, f, h
+
+
+def _field_error(q, qr):
+    return [float(np.abs(q[:, k] - qr[:, k]).max() / np.abs(qr[:, k]).max()) if np.abs(qr[:, k]).max() > 0 else 0.0
+            for k in range(NE)]
+
+
+@pytest.mark.parametrize("precision", [gpu.PLAIN_FP64, gpu.CHOSEN_FP32])
+def test_fr_reacting_solve_against_reference_field(precision):
+    """The steady reacting solve (3000 K partly dissociated freestream,
+    Mach 0.5, beta 0.25) to the reference's 8-order drop: the multicolour SGS
+    sweeps in another row order than the reference's natural order, so the
+    iterates differ; the field at that drop agrees with the reference's to
+    the 1e-6 per variable the north star states for the chosen form (an
+    8-order drop itself leaves ~1e-5 of the converged field unsettled, see
+    the deep solve below)."""
+    print(f"[fr solve] precision {precision}", flush=True)
+    d, f, h = _solve(precision)
+    ref = d["history"]
+    assert h[-1] <= 1e-8 * h[0]
+    assert len(h) <= 1.5 * len(ref)
+    err = _field_error(f.download_q()[:f.nnode, :NE], d["q_final"][:f.nnode, :NE])
+    print("per-variable error vs the reference", err, "iterations", len(h), "reference", len(ref), flush=True)
+    assert max(err) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [gpu.PLAIN_FP64, gpu.CHOSEN_FP32])
+def test_fr_deep_solve_matches_oracle(precision):
+    """Both solved 14 orders down (the FP64 residual at round-off), the
+    device's field (FP64, or FP32 Jacobian and SGS) and the oracle's
+    (natural-order SGS, the reference's loop bit for bit) agree to 1e-11 per
+    variable: the same discrete solution."""
+    print(f"[fr deep solve] precision {precision}", flush=True)
+    d, f, h = _solve(precision, drop=1e-14)
+    assert h[-1] <= 1e-14 * h[0], (len(h), h[-1] / h[0])
+    fr = O.FR(MODEL, {k: float(d[k]) for k in SCALES}, Pref=float(d["Pref"]), beta=float(d["beta"]))
+    pb = O.FRProblem(_Mesh(d), np.array(BCT["fr_bump_solve"], dtype=np.int32),
+                     np.tile(d["qinf"], (len(BCT["fr_bump_solve"]), 1)), fr)
+    s = O.FRSolver(pb, np.tile(d["qinf"], (f.nnode + f.nbnode, 1)), cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6,
+                   ja=d["ja"])
+    ho = [s.step()[0]]
+    while ho[-1] > 1e-14 * ho[0] and len(ho) < 200:
+        ho.append(s.step()[0])
+    err = _field_error(f.download_q()[:f.nnode, :NE], s.q[:f.nnode, :NE])
+    print("per-variable error vs the oracle", err, "iterations", len(h), "oracle", len(ho), flush=True)
+    assert max(err) < 1e-11
+
+
+@pytest.mark.parametrize("reactions", [False, True])
+def test_fr_steps_match_oracle_in_device_order(reactions):
+    """Three full steps on the device against the oracle run with the
+    device's SGS row order and block-row order.  Without reactions every
+    operation is the reference's arithmetic and the states agree to the bit.
+    With reactions the device's exp/pow differ from the host's in the last
+    bit, the source Jacobian's forward difference scales that by 1e8, and the
+    residual histories agree to 2e-6, the states to 1e-5 per variable (the
+    largest difference sits in the N density, 1e-4 of the mixture); converged,
+    the fields agree to 1e-11 (test_fr_deep_solve_matches_oracle)."""
+    d = load("fr_bump_solve")
+    _, f, _ = context("fr_bump_solve", cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6, reactions=reactions)
+    q0 = np.tile(d["qinf"], (f.nnode + f.nbnode, 1))
+    f.upload_q(q0)
+    f.init_iterations()
+    h = np.array([f.step()[0] for _ in range(3)])
+    fr = O.FR(MODEL, {k: float(d[k]) for k in SCALES}, Pref=float(d["Pref"]), beta=float(d["beta"]),
+              rxn_on=reactions)
+    pb = O.FRProblem(_Mesh(d), np.array(BCT["fr_bump_solve"], dtype=np.int32),
+                     np.tile(d["qinf"], (len(BCT["fr_bump_solve"]), 1)), fr)
+    ia, ja = f.pattern()
+    s = O.FRSolver(pb, q0, cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6, order=f.sgs_order(), ja=ja)
+    ho = np.array([s.step()[0] for _ in range(3)])
+    q, qo = f.download_q()[:, :NE], s.q[:, :NE]
+    # the norms sum in another order: residGlobal agrees to rounding
+    if not reactions:
+        np.testing.assert_array_equal(q, qo)
+        np.testing.assert_allclose(h, ho, rtol=1e-13)
+    else:
+        assert np.abs(h / ho - 1).max() < 2e-6
+        assert max(_field_error(q[:f.nnode], qo[:f.nnode])) < 1e-5
+
+
+def test_fr_rerun_is_bitwise_identical():
+    """Two runs of 12 steps from the same state give the same residual
+    history and field to the bit (no atomics in any floating-point sum)."""
+    runs = []
+    for _ in range(2):
+        d, f, h = _solve(gpu.PLAIN_FP64, steps=12)
+        runs.append((h, f.download_q()))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
