@@ -20,7 +20,8 @@
  * The chemistry model (species thermo, reactions, reference density, time
  * and temperature) comes from a pcfd_chem object (include/pcfd_chem.h).
  * Boundary types handled: sonicInflow, dirichlet, sonicOutflow, neumann,
- * farField (characteristic), impermeableWall, symmetry.
+ * farField, impermeableWall, symmetry (the characteristic states, or with
+ * noCVBC the upwind / mirrored ones, ucs/compressibleFR.tcc:962-973,1109-1128).
  */
 #ifndef PCFD_FR_H
 #define PCFD_FR_H
@@ -49,6 +50,7 @@ typedef struct {
   int nsgs;             /* numberSGS */
   int jac_freq;         /* jacobianFreq */
   int precision;        /* PCFD_PLAIN_FP64 (drop-in) or PCFD_CHOSEN_FP32 (Jacobian and SGS in FP32) */
+  int no_cvbc;          /* noCVBC: upwind farfield and mirrored-velocity walls instead of the characteristic states */
 } pcfd_fr_params;
 
 /* mesh: as for pcfd_create (its qref/twall are not used here); qref: per face

@@ -394,7 +394,7 @@ class Chem:
 class OFr(C.Structure):
     _fields_ = [("chem", C.POINTER(OChem)), ("ref_pressure", C.c_double), ("ref_velocity", C.c_double),
                 ("Pref", C.c_double), ("beta", C.c_double), ("sym2d", C.c_int), ("rxn_on", C.c_int),
-                ("local_dt", C.c_int), ("dt_param", C.c_double)]
+                ("local_dt", C.c_int), ("dt_param", C.c_double), ("no_cvbc", C.c_int)]
 
 
 class OFrCrs(C.Structure):
@@ -456,13 +456,14 @@ class FR:
     ref_temperature (Param::PostCompute); Pref the freestream pressure
     (non-dimensional); beta the preconditioning parameter."""
 
-    def __init__(self, model, ref, Pref=0.0, beta=1.0, sym2d=0, rxn_on=True, local_dt=True, dt_param=-1.0):
+    def __init__(self, model, ref, Pref=0.0, beta=1.0, sym2d=0, rxn_on=True, local_dt=True, dt_param=-1.0,
+                 no_cvbc=False):
         self.model = model.with_reference(ref["ref_density"], ref["ref_time"], ref["ref_temperature"])
         self.chem = Chem(self.model)
         self.ns = self.model.ns
         self.neqn, self.nvars = self.ns + 4, 3 * self.ns + 6
         self.f = OFr(C.pointer(self.chem.c), float(ref["ref_pressure"]), float(ref["ref_velocity"]), float(Pref),
-                     float(beta), int(sym2d), int(rxn_on), int(local_dt), float(dt_param))
+                     float(beta), int(sym2d), int(rxn_on), int(local_dt), float(dt_param), int(no_cvbc))
         _fr_lib()
 
     @property

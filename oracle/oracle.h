@@ -163,6 +163,7 @@ typedef struct {
   int rxn_on;          /* reactionsOn */
   int local_dt;        /* localTimeStepping */
   double dt_param;     /* timeStep parameter (default -1) */
+  int no_cvbc;         /* noCVBC: upwind farfield and mirror walls instead of the characteristic states */
 } o_fr;
 
 typedef struct {

@@ -365,11 +365,23 @@ void o_fr_eigensystem(const o_fr* f, const double* Q, const double* avec, double
   Tinv[neqn * tloc + tloc] = 1.0 / (2.0 * rho * cPrime);
 }
 
-/* GetFarfieldBoundaryVariables (ucs/compressibleFR.tcc:940-1040), the CVBC
- * branch, vdotn = 0 */
+/* GetFarfieldBoundaryVariables (ucs/compressibleFR.tcc:940-1040), vdotn = 0:
+ * the characteristic (CVBC) branch, or with noCVBC the upwind copy of the
+ * interior or freestream state by the sign of the first eigenvalue */
 static void fr_farfield(const o_fr* f, const double* QL, double* QR, const double* Qinf, const double* avec,
                         double beta) {
-  const int ns = f->chem->ns, neqn = ns + 4;
+  const int ns = f->chem->ns, neqn = ns + 4, nv = 3 * ns + 6;
+  if (f->no_cvbc) {
+    double qavg[3 * O_CHEM_MAXS + 6], eig[O_CHEM_MAXS + 4];
+    double T[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)], Tinv[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)];
+    for (int subit = 0; subit < FR_NSUBIT; subit++) {
+      for (int i = 0; i < neqn; i++) qavg[i] = 0.5 * (QL[i] + QR[i]);
+      o_fr_aux_q(f, qavg);
+      o_fr_eigensystem(f, qavg, avec, beta, eig, T, Tinv);
+      memcpy(QR, eig[0] >= 0.0 ? QL : Qinf, sizeof(double) * nv);
+    }
+    return;
+  }
   double qavg[3 * O_CHEM_MAXS + 6], eig[O_CHEM_MAXS + 4];
   double T[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)], Tinv[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)];
   double rhs[O_CHEM_MAXS + 4], ql[O_CHEM_MAXS + 4], qinf[O_CHEM_MAXS + 4];
@@ -395,10 +407,27 @@ static void fr_farfield(const o_fr* f, const double* QL, double* QR, const doubl
   }
 }
 
-/* GetInviscidWallBoundaryVariables (ucs/compressibleFR.tcc:1042-1154), the
- * CVBC branch, vdotn = 0 */
+/* GetInviscidWallBoundaryVariables (ucs/compressibleFR.tcc:1042-1154),
+ * vdotn = 0: the characteristic (CVBC) branch, or with noCVBC the interior
+ * state with its velocity mirrored in the wall (MirrorVector,
+ * ucs/geometry.h:469-476) */
 static void fr_wall(const o_fr* f, const double* QL, double* QR, const double* avec, double beta) {
   const int ns = f->chem->ns, neqn = ns + 4;
+  if (f->no_cvbc) {
+    double QLmod[3 * O_CHEM_MAXS + 6];
+    memcpy(QLmod, QL, sizeof(double) * (3 * ns + 6));
+    const double vdotn = 0.0;
+    QLmod[ns] += vdotn * avec[0];
+    QLmod[ns + 1] += vdotn * avec[1];
+    QLmod[ns + 2] += vdotn * avec[2];
+    o_fr_aux_q(f, QLmod);
+    for (int i = 0; i < neqn; i++) QR[i] = QLmod[i];
+    const double dot = 2.0 * (QLmod[ns] * avec[0] + QLmod[ns + 1] * avec[1] + QLmod[ns + 2] * avec[2]);
+    QR[ns] = QLmod[ns] - dot * avec[0];
+    QR[ns + 1] = QLmod[ns + 1] - dot * avec[1];
+    QR[ns + 2] = QLmod[ns + 2] - dot * avec[2];
+    return;
+  }
   double qavg[3 * O_CHEM_MAXS + 6], eig[O_CHEM_MAXS + 4];
   double T[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)], Tinv[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)];
   double rhs[O_CHEM_MAXS + 4], scr[O_CHEM_MAXS + 4], ql[O_CHEM_MAXS + 4];

@@ -1152,6 +1152,7 @@ int pcfd_fr_set_params(pcfd_fr* fp, const pcfd_fr_params* p) {
     md.sym2d = p->symmetry2d;
     md.local_dt = p->local_dt;
     md.dt_param = p->dt_param;
+    md.no_cvbc = p->no_cvbc;
     hcheck(hipMemcpy(f.dmodel, &md, sizeof(Model), hipMemcpyHostToDevice), "hipMemcpy");
     const size_t NE2 = (size_t)f.ne * f.ne;
     const size_t noff = (size_t)f.nslot * kLane * NE2, ndg = (size_t)f.nchunk * kLane * NE2;

@@ -50,6 +50,7 @@ struct Model {
   int sym2d;
   int local_dt;          // localTimeStepping
   double dt_param;       // timeStep parameter
+  int no_cvbc;           // noCVBC: upwind farfield, mirrored walls
 };
 
 template <int NS>
@@ -466,6 +467,18 @@ template <int NS>
 PCFD_HD void farfield(const Model& m, const State<NS>& L, State<NS>& R, const State<NS>& Qinf, const double* a,
                       double beta) {
   constexpr int NE = NS + 4;
+  if (m.no_cvbc) {
+    // noCVBC: the interior or the freestream state by the sign of the first
+    // eigenvalue (theta of the average state), 10 times over
+    for (int sub = 0; sub < kSubit; ++sub) {
+      State<NS> avg;
+#pragma unroll
+      for (int i = 0; i < NE; ++i) avg.q[i] = 0.5 * (L.q[i] + R.q[i]);
+      aux(m, avg);
+      R = theta_of(avg, a) >= 0.0 ? L : Qinf;
+    }
+    return;
+  }
   double eig[NE], T[NE * NE], Ti[NE * NE], rhs[NE], ql[NE], qi[NE];
   for (int sub = 0; sub < kSubit; ++sub) {
     State<NS> avg;
@@ -500,6 +513,23 @@ PCFD_HD void farfield(const Model& m, const State<NS>& L, State<NS>& R, const St
 template <int NS>
 PCFD_HD void wall(const Model& m, const State<NS>& L, State<NS>& R, const double* a, double beta) {
   constexpr int NE = NS + 4;
+  if (m.no_cvbc) {
+    // noCVBC: the interior state with its velocity mirrored in the wall
+    // (MirrorVector, ucs/geometry.h:469-476), vdotn = 0
+    State<NS> Lm = L;
+    const double vdotn = 0.0;
+    Lm.q[NS] += vdotn * a[0];
+    Lm.q[NS + 1] += vdotn * a[1];
+    Lm.q[NS + 2] += vdotn * a[2];
+    aux(m, Lm);
+#pragma unroll
+    for (int i = 0; i < NE; ++i) R.q[i] = Lm.q[i];
+    const double dot = 2.0 * (Lm.q[NS] * a[0] + Lm.q[NS + 1] * a[1] + Lm.q[NS + 2] * a[2]);
+    R.q[NS] = Lm.q[NS] - dot * a[0];
+    R.q[NS + 1] = Lm.q[NS + 1] - dot * a[1];
+    R.q[NS + 2] = Lm.q[NS + 2] - dot * a[2];
+    return;
+  }
   double eig[NE], T[NE * NE], Ti[NE * NE], rhs[NE], ql[NE];
   int p[NE];
   for (int sub = 0; sub < kSubit; ++sub) {

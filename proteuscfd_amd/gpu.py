@@ -597,7 +597,7 @@ class FrParams(C.Structure):
     _fields_ = [("ref_pressure", C.c_double), ("ref_velocity", C.c_double), ("Pref", C.c_double),
                 ("beta", C.c_double), ("symmetry2d", C.c_int), ("reactions", C.c_int), ("local_dt", C.c_int),
                 ("dt_param", C.c_double), ("cfl", C.c_double), ("cfl_start", C.c_double), ("cfl_ramp", C.c_int),
-                ("nsgs", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int)]
+                ("nsgs", C.c_int), ("jac_freq", C.c_int), ("precision", C.c_int), ("no_cvbc", C.c_int)]
 
 
 def _fr_lib():
@@ -677,10 +677,11 @@ class FrContext:
             pass
 
     def set_params(self, ref_pressure, ref_velocity, Pref, beta, cfl=1.0, cfl_start=1.0, cfl_ramp=0, nsgs=4,
-                   jac_freq=1, precision=PLAIN_FP64, reactions=True, local_dt=True, dt_param=-1.0, symmetry2d=0):
+                   jac_freq=1, precision=PLAIN_FP64, reactions=True, local_dt=True, dt_param=-1.0, symmetry2d=0,
+                   no_cvbc=False):
         p = FrParams(float(ref_pressure), float(ref_velocity), float(Pref), float(beta), int(symmetry2d),
                      int(reactions), int(local_dt), float(dt_param), float(cfl), float(cfl_start), int(cfl_ramp),
-                     int(nsgs), int(jac_freq), int(precision))
+                     int(nsgs), int(jac_freq), int(precision), int(no_cvbc))
         self._chk(_fr_lib().pcfd_fr_set_params(self.h, C.byref(p)), "pcfd_fr_set_params")
 
     def info(self):

@@ -538,11 +538,35 @@ def make_fr_bump_solve():
     return fr_out(run, out, meta)
 
 
+def make_fr_wedge_nocvbc():
+    """The Mach 8 reacting wedge (freestream 300 K, 1e-3 kg/m^3) with
+    noCVBC (upwind farfield, mirrored-velocity walls and symmetry planes),
+    first order, CFL 1 -> 5 over 50 iterations, 300 iterations of the
+    reference's loop from freestream: its residual falls 2.7 orders and then
+    stays at 2.17e-3 of its first value (the reference's own plateau; with
+    the characteristic wall states the same start diverges within a few
+    iterations).  History, the final field and the block-row layout."""
+    case = cases.wedge_case(nx=8, ny=4, nz=2, rcm=False)
+    bc = {1: 7, 2: 8, 3: 3, 4: 3, 5: 9, 6: 9}
+    p_inf = 1.0e-3 * (0.767 * 8.31447215 / 0.0280134 + 0.233 * 8.31447215 / 0.0319988) * 300.0
+    keys = fr_keys(velocity=8.0, initialTemperature=300.0, initialPressure=f"{p_inf:.17g}",
+                   refPressure=f"{p_inf:.17g}", numberSGS=6, CFL=5.0, startingCFL=1.0, rampCFL=50, noCVBC=1)
+    run = FRRun(case, keys, bc, nsteps=300)
+    meta, qinf = run.ref_scales()
+    run("trace", 1)
+    ia, ja = run.arr("ia", np.int32), run.arr("ja", np.int32)
+    run("solve", 300, 1e-12)
+    m = case.mesh
+    out = {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 21),
+           "qinf": qinf, "ia": ia, "ja": ja}
+    return fr_out(run, out, meta)
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
           "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
           "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure,
           "fr_wedge_phases": make_fr_wedge_phases, "fr_bump_phases": make_fr_bump_phases,
-          "fr_bump_solve": make_fr_bump_solve}
+          "fr_bump_solve": make_fr_bump_solve, "fr_wedge_nocvbc": make_fr_wedge_nocvbc}
 
 
 def main(names):

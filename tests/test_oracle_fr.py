@@ -19,7 +19,7 @@ NS = MODEL.ns
 SCALES = ["ref_pressure", "ref_velocity", "ref_density", "ref_time", "ref_temperature"]
 # BC ids per face tag of the fixtures' meshes (ucs/bc_defines.h)
 BCT = {"fr_wedge_phases": [0, 7, 8, 3, 3, 9, 9], "fr_bump_phases": [0, 6, 6, 6, 6, 3, 6],
-       "fr_bump_solve": [0, 6, 6, 6, 6, 3, 6]}
+       "fr_bump_solve": [0, 6, 6, 6, 6, 3, 6], "fr_wedge_nocvbc": [0, 7, 8, 3, 3, 9, 9]}
 
 
 def load(name):
@@ -39,7 +39,8 @@ class _Mesh:
 
 def setup(name):
     d = load(name)
-    fr = O.FR(MODEL, {k: float(d[k]) for k in SCALES}, Pref=float(d["Pref"]), beta=float(d["beta"]))
+    fr = O.FR(MODEL, {k: float(d[k]) for k in SCALES}, Pref=float(d["Pref"]), beta=float(d["beta"]),
+              no_cvbc="nocvbc" in name)
     bct = np.array(BCT[name], dtype=np.int32)
     pb = O.FRProblem(_Mesh(d), bct, np.tile(d["qinf"], (len(bct), 1)), fr)
     return d, fr, pb
@@ -102,6 +103,20 @@ def test_fr_steady_solve_matches_reference_bitwise():
     np.testing.assert_array_equal(s.q[:pb.nnode], d["q_final"][:pb.nnode])
     T = s.q[:pb.nnode, NS + 3] * float(d["ref_temperature"])
     assert 3000.0 < T.min() and T.max() < 4000.0  # recombination heats the gas
+
+
+def test_fr_wedge_nocvbc_matches_reference_bitwise():
+    """The Mach 8 reacting wedge with noCVBC (upwind farfield, mirrored
+    walls and symmetry planes): 300 iterations of the reference's loop
+    (the residual falls 2.7 orders, then holds at 2.17e-3 of its first
+    value), every iteration's residGlobal and the final field bit for bit."""
+    d, fr, pb = setup("fr_wedge_nocvbc")
+    q0 = np.tile(d["qinf"], (pb.nnode + pb.nbnode, 1))
+    s = O.FRSolver(pb, q0, cfl=5.0, cfl_start=1.0, cfl_ramp=50, nsgs=6, ja=d["ja"])
+    hist = np.array([s.step()[0] for _ in range(len(d["history"]))])
+    np.testing.assert_array_equal(hist, d["history"])
+    np.testing.assert_array_equal(s.q[:pb.nnode], d["q_final"][:pb.nnode])
+    assert 2.1e-3 < hist[-1] / hist[0] < 2.2e-3
 
 
 def test_fr_solve_in_sorted_row_order_close_to_reference():
