@@ -106,6 +106,19 @@ int pcfd_fr_step(pcfd_fr* f, double* norms, int* nzeroed);
  * history [maxiter] (may be NULL); *iters steps taken */
 int pcfd_fr_solve(pcfd_fr* f, int maxiter, double drop, double* history, int* iters);
 
+/* The learned part (no counterpart in the reference): with a surrogate set,
+ * the residual's finite-rate source takes its production rates from the
+ * MLP surrogate with the exact-rate fallback of
+ * pcfd_chem_surrogate_wdot_device (include/pcfd_chem.h); the source
+ * Jacobian in the preconditioner stays exact.  mlp = NULL restores the exact
+ * rates.  mlp and d are borrowed (d is copied). */
+int pcfd_fr_set_surrogate(pcfd_fr* f, pcfd_mlp* mlp, const pcfd_surrogate_desc* d);
+/* cells sent to the exact rates by the last residual; residuals evaluated
+ * and cells sent back since the surrogate was set; device ms of the
+ * surrogate passes */
+int pcfd_fr_surrogate_stats(const pcfd_fr* f, int* last_fallback, long long* calls, long long* fallback_total,
+                            double* ms_total);
+
 /* Device milliseconds per phase since the last reset: [0] Jacobian, [1] time
  * step, [2] boundary states, [3] residual, [4] SGS, [5] update, [6] the
  * Jacobian's edge kernel alone, [7] the residual's edge flux kernel alone */

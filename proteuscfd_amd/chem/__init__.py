@@ -104,3 +104,51 @@ def freestream_scales(model: ChemModel, T_inf: float, rho_inf: float, Y) -> tupl
     p = rho_inf * R * T_inf
     c = np.sqrt(gamma * R * T_inf)
     return p / (c * c), 1.0 / c, T_inf
+
+
+R_UNIV_PARAM = 8.314459848  # J/(mol K), the gas constant of the case parameters (ucs/param.tcc:13)
+
+
+def fr_case_scales(p0: float, T0: float, ref_pressure: float, MW: float = 28.966, Cp: float = 1006.43,
+                   ref_length: float = 1.0) -> dict:
+    """Reference scales of an FR case as Param::PostCompute derives them
+    (ucs/param.tcc:355-400) from initialPressure p0 (Pa), initialTemperature
+    T0 (K), refPressure and the parameter gas (MW g/mol, Cp J/(kg K)):
+    velocity by that gas's sound speed, density by p_ref / u_ref^2, time by
+    L / u_ref, and the temperature for which the gas's non-dimensional c^2
+    equals its non-dimensional T."""
+    Rs = R_UNIV_PARAM / (MW / 1000.0)
+    Cv = Cp - Rs
+    gamma = Cp / Cv
+    ref_velocity = np.sqrt(gamma * Rs * T0)
+    ref_density = ref_pressure / (ref_velocity * ref_velocity)
+    ref_time = ref_length / ref_velocity
+    rho_d = p0 / (Rs * T0)
+    P_nd = p0 / ref_pressure
+    rho_nd = rho_d / ref_density
+    c2_nd = gamma * P_nd / rho_nd
+    return dict(ref_pressure=float(ref_pressure), ref_velocity=float(ref_velocity), ref_density=float(ref_density),
+                ref_time=float(ref_time), ref_temperature=float(T0 / c2_nd))
+
+
+def fr_freestream(model: ChemModel, scales: dict, Y, p0: float, T0: float, velocity) -> tuple[np.ndarray, float]:
+    """Qinf (ns + 4 native values) and Pref = P(Qinf) as UpdateQinf sets them
+    (ucs/compressibleFR.tcc:1465-1518): the pressure split by mass fraction,
+    rho_i = Y_i sum_j (p0 Y_j / (R_j T0)) / rho_ref, velocity non-dimensional
+    (flow direction times the Mach number), T = T0 / T_ref."""
+    Y = np.asarray(Y, dtype=float)
+    ns = model.ns
+    R = UNIV_R / model.mw
+    rDim = 0.0
+    for i in range(ns):
+        rDim += p0 * Y[i] / (R[i] * T0)
+    r = rDim / scales["ref_density"]
+    q = np.zeros(ns + 4)
+    q[:ns] = Y * r
+    q[ns:ns + 3] = velocity
+    q[ns + 3] = T0 / scales["ref_temperature"]
+    TDim = q[ns + 3] * scales["ref_temperature"]
+    P = 0.0
+    for i in range(ns):
+        P += (q[i] * scales["ref_density"]) * R[i] * TDim
+    return q, P / scales["ref_pressure"]

@@ -40,6 +40,10 @@
 namespace pcfd {
 const ChemDev* chem_device_model(const pcfd_chem* c);
 int chem_nspecies(const pcfd_chem* c);
+// the production-rate surrogate with exact fallback (kernels_mlp.hip): runs on
+// the surrogate's stream and returns with it synchronised
+int surrogate_launch(pcfd_chem* chem, pcfd_mlp& m, const pcfd_surrogate_desc* d, int n, const double* rhoi,
+                     const double* T, double* wdot, int* lst, int* cnt, int* off, int* nfallback, double* ms);
 }  // namespace pcfd
 
 using namespace pcfd;
@@ -190,7 +194,8 @@ __global__ void __launch_bounds__(kFB) k_fr_residual(const Model* __restrict__ m
                                                     const int* __restrict__ nslot, const double* __restrict__ eflux,
                                                     const int* __restrict__ he_ptr, const int* __restrict__ he_id,
                                                     const double* __restrict__ he_a, const double* __restrict__ q,
-                                                    const double* __restrict__ vol, int rxn, double beta,
+                                                    const double* __restrict__ vol, int rxn,
+                                                    const double* __restrict__ wrates, double beta,
                                                     double* __restrict__ b, double* __restrict__ red) {
   constexpr int NE = NS + 4;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -229,7 +234,15 @@ __global__ void __launch_bounds__(kFB) k_fr_residual(const Model* __restrict__ m
     // the source vector is added whole (zero without reactions, and zero in
     // the momentum and energy entries), as SpatialResidual does
     double S[NS];
-    if (rxn) {
+    if (rxn && wrates) {
+      // production rates from the surrogate (kg/m^3/s), scaled as SourceTerm does
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        double w = wrates[(long long)i * NS + k];
+        w /= (cm->ref_density / cm->ref_time);
+        S[k] = vol[i] * w;
+      }
+    } else if (rxn) {
       double Q[NE];
 #pragma unroll
       for (int k = 0; k < NE; ++k) Q[k] = q[(long long)i * NE + k];
@@ -246,6 +259,21 @@ __global__ void __launch_bounds__(kFB) k_fr_residual(const Model* __restrict__ m
     for (int k = 0; k < NE; ++k) b[(long long)i * NE + k] = acc[k];
   }
   norm_partials<NE>(acc, valid, red);
+}
+
+// Dimensional species densities and temperature of every node, the
+// production-rate surrogate's inputs (SourceTerm's scaling,
+// ucs/compressibleFR.tcc:1284-1290)
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_chem_inputs(const ChemDev* __restrict__ cm, int nnode,
+                                                       const double* __restrict__ q, double* __restrict__ rhoi,
+                                                       double* __restrict__ T) {
+  constexpr int NE = NS + 4;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) rhoi[(long long)i * NS + k] = q[(long long)i * NE + k] * cm->ref_density;
+  T[i] = q[(long long)i * NE + NS + 3] * cm->ref_temperature;
 }
 
 // Time step of every node (ComputeTimesteps with local time stepping,
@@ -632,6 +660,14 @@ struct pcfd_fr {
   long long *eoff = nullptr, *piv = nullptr;
   double *Md = nullptr, *Dd = nullptr, *xd = nullptr;
   float *Mf = nullptr, *Df = nullptr, *xf = nullptr;
+  // production-rate surrogate in the residual (null: exact rates)
+  pcfd_mlp* mlp = nullptr;
+  pcfd_surrogate_desc sdesc{};
+  double *s_rhoi = nullptr, *s_T = nullptr, *s_w = nullptr;
+  int *s_lst = nullptr, *s_cnt = nullptr, *s_off = nullptr;
+  int s_last_fallback = 0;
+  long long s_calls = 0, s_fallback_total = 0;
+  double s_ms = 0.0;
 };
 
 namespace {
@@ -660,6 +696,7 @@ void free_all(pcfd_fr& f) {
   dfree(f.nslot_d); dfree(f.he_ptr); dfree(f.he_id); dfree(f.he_tag); dfree(f.bc_type); dfree(f.bnd);
   dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount); dfree(f.eoff);
   dfree(f.piv); dfree(f.Md); dfree(f.Dd); dfree(f.xd); dfree(f.Mf); dfree(f.Df); dfree(f.xf); dfree(f.dmodel);
+  dfree(f.s_rhoi); dfree(f.s_T); dfree(f.s_w); dfree(f.s_lst); dfree(f.s_cnt); dfree(f.s_off);
   if (f.e0) (void)hipEventDestroy(f.e0);
   if (f.e1) (void)hipEventDestroy(f.e1);
   if (f.e2) (void)hipEventDestroy(f.e2);
@@ -727,10 +764,27 @@ void run_residual(pcfd_fr& f, double* norms) {
     hcheck(hipGetLastError(), "k_fr_edge_flux");
     send(f, 7);
   }
+  const double* wr = nullptr;
+  if (f.mlp && f.prm.reactions) {
+    hipLaunchKernelGGL(k_fr_chem_inputs<NS5>, dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream,
+                       chem_device_model(f.chem), f.nnode, f.q, f.s_rhoi, f.s_T);
+    hcheck(hipGetLastError(), "k_fr_chem_inputs");
+    hcheck(hipStreamSynchronize(f.stream), "sync");
+    int nf = 0;
+    double ms = 0.0;
+    const int rc = surrogate_launch(const_cast<pcfd_chem*>(f.chem), *f.mlp, &f.sdesc, f.nnode, f.s_rhoi, f.s_T,
+                                    f.s_w, f.s_lst, f.s_cnt, f.s_off, &nf, &ms);
+    if (rc != PCFD_OK) throw std::runtime_error("surrogate production rates failed");
+    f.s_last_fallback = nf;
+    f.s_fallback_total += nf;
+    f.s_calls++;
+    f.s_ms += ms;
+    wr = f.s_w;
+  }
   const int nb = nblocks(f.nnode);
   hipLaunchKernelGGL(k_fr_residual<NS5>, dim3(nb), dim3(kFB), 0, f.stream, f.dmodel, chem_device_model(f.chem),
                      f.nnode, f.nptr, f.nslot_d, f.eflux, f.he_ptr, f.he_id, f.he_a, f.q, f.vol, f.prm.reactions,
-                     f.prm.beta, f.b, f.red);
+                     wr, f.prm.beta, f.b, f.red);
   hcheck(hipGetLastError(), "k_fr_residual");
   if (norms) {
     const int ne = f.ne;
@@ -1425,6 +1479,40 @@ int pcfd_fr_solve(pcfd_fr* fp, int maxiter, double drop, double* history, int* i
     if (iters) *iters = it;
     return PCFD_OK;
   });
+}
+
+int pcfd_fr_set_surrogate(pcfd_fr* fp, pcfd_mlp* mlp, const pcfd_surrogate_desc* d) {
+  return guarded(fp, [&](pcfd_fr& f) -> int {
+    if (mlp && !d) {
+      f.err = "pcfd_fr_set_surrogate: a surrogate needs its descriptor";
+      return PCFD_ERR_ARG;
+    }
+    f.mlp = mlp;
+    if (!mlp) return PCFD_OK;
+    f.sdesc = *d;
+    if (!f.s_rhoi) {
+      const size_t npass = ((size_t)f.nnode + 63) / 64;
+      dalloc(f, &f.s_rhoi, (size_t)f.nnode * f.ns);
+      dalloc(f, &f.s_T, (size_t)f.nnode);
+      dalloc(f, &f.s_w, (size_t)f.nnode * f.ns);
+      dalloc(f, &f.s_lst, npass * 64);
+      dalloc(f, &f.s_cnt, npass);
+      dalloc(f, &f.s_off, npass + 1);
+    }
+    f.s_calls = f.s_fallback_total = 0;
+    f.s_ms = 0.0;
+    return PCFD_OK;
+  });
+}
+
+int pcfd_fr_surrogate_stats(const pcfd_fr* f, int* last_fallback, long long* calls, long long* fallback_total,
+                            double* ms_total) {
+  if (!f) return PCFD_ERR_ARG;
+  if (last_fallback) *last_fallback = f->s_last_fallback;
+  if (calls) *calls = f->s_calls;
+  if (fallback_total) *fallback_total = f->s_fallback_total;
+  if (ms_total) *ms_total = f->s_ms;
+  return PCFD_OK;
 }
 
 int pcfd_fr_phase_times(pcfd_fr* f, double* ms, int reset) {

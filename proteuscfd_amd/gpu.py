@@ -589,7 +589,7 @@ FR_SYMBOLS = ["pcfd_fr_create", "pcfd_fr_destroy", "pcfd_fr_last_error", "pcfd_f
               "pcfd_fr_upload_q", "pcfd_fr_download_q", "pcfd_fr_update_bcs", "pcfd_fr_residual", "pcfd_fr_timestep",
               "pcfd_fr_set_timestep", "pcfd_fr_jacobian", "pcfd_fr_download_jacobian", "pcfd_fr_pattern",
               "pcfd_fr_sgs", "pcfd_fr_sgs_order", "pcfd_fr_apply_dq", "pcfd_fr_init_iterations", "pcfd_fr_step",
-              "pcfd_fr_solve", "pcfd_fr_phase_times"]
+              "pcfd_fr_solve", "pcfd_fr_phase_times", "pcfd_fr_set_surrogate", "pcfd_fr_surrogate_stats"]
 FR_PHASES = ["jacobian", "timestep", "bcs", "residual", "sgs", "update", "k_jac_edge", "k_edge_flux"]
 
 
@@ -626,6 +626,8 @@ def _fr_lib():
         L.pcfd_fr_step.argtypes = [vp, dp, ip]
         L.pcfd_fr_solve.argtypes = [vp, C.c_int, C.c_double, dp, ip]
         L.pcfd_fr_phase_times.argtypes = [vp, dp, C.c_int]
+        L.pcfd_fr_set_surrogate.argtypes = [vp, vp, C.c_void_p]
+        L.pcfd_fr_surrogate_stats.argtypes = [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp]
         L._fr_ready = True
     return L
 
@@ -770,6 +772,19 @@ class FrContext:
         n = C.c_int()
         self._chk(_fr_lib().pcfd_fr_solve(self.h, int(max_iter), float(drop), _d(hist), C.byref(n)), "pcfd_fr_solve")
         return hist[:n.value]
+
+    def set_surrogate(self, mlp, desc):
+        """mlp: a gpu.MLP (kept alive here) and its SurrogateDesc; None for the exact rates."""
+        self._mlp, self._sdesc = mlp, desc
+        self._chk(_fr_lib().pcfd_fr_set_surrogate(self.h, mlp.h if mlp is not None else None,
+                                                  C.byref(desc) if mlp is not None else None),
+                  "pcfd_fr_set_surrogate")
+
+    def surrogate_stats(self):
+        last, calls, tot, ms = C.c_int(), C.c_longlong(), C.c_longlong(), C.c_double()
+        self._chk(_fr_lib().pcfd_fr_surrogate_stats(self.h, C.byref(last), C.byref(calls), C.byref(tot),
+                                                    C.byref(ms)), "pcfd_fr_surrogate_stats")
+        return dict(last_fallback=last.value, calls=calls.value, fallback_total=tot.value, ms=ms.value)
 
     def phase_times(self, reset=False):
         ms = np.zeros(len(FR_PHASES))

@@ -316,3 +316,22 @@ def test_qinf_partial_pressures_from_mass_fractions():
     assert abs(q[NS + 4] - want) < 1e-14 and abs(fr.f.Pref - q[NS + 4]) == 0
     np.testing.assert_array_equal(q, d["qinf"])
     assert abs(float(d["Pref"]) - want) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["fr_wedge_phases", "fr_bump_phases", "fr_bump_solve"])
+def test_case_scales_and_freestream_match_reference(name):
+    """The host-side case set-up (Param::PostCompute's scales, UpdateQinf's
+    freestream and Pref) against the values the reference reported for the
+    fixture's case."""
+    d = load(name)
+    setup_ = {"fr_wedge_phases": (None, 300.0, 8.0, dict(N2=0.767, O2=0.233)),
+              "fr_bump_phases": (101325.0, 600.0, 0.5, dict(N2=0.767, O2=0.233)),
+              "fr_bump_solve": (101325.0, 3000.0, 0.5, dict(O2=0.10, O=0.10, N=0.05, N2=0.70, NO=0.05))}[name]
+    p0, T0, mach, Y = setup_
+    p0 = float(d["ref_pressure"]) if p0 is None else p0
+    sc = chem.fr_case_scales(p0, T0, float(d["ref_pressure"]))
+    for k in SCALES:
+        assert abs(sc[k] / float(d[k]) - 1) < 1e-15, k
+    q, Pref = chem.fr_freestream(MODEL, sc, chem.mass_fractions(MODEL, **Y), p0, T0, [mach, 0.0, 0.0])
+    assert np.abs(q - d["qinf"][:NS + 4]).max() <= 1e-15 * np.abs(d["qinf"][:NS + 4]).max()
+    assert abs(Pref - float(d["Pref"])) < 1e-15
