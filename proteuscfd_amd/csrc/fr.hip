@@ -311,15 +311,24 @@ __global__ void __launch_bounds__(kFB) k_fr_timestep(const Model* __restrict__ m
 // Kernel_NumJac (ucs/jacobian.tcc:253-304): one-sided differences (h = 1e-8)
 // of the edge's HLLC flux in each native variable of either side, aux
 // recomputed for every perturbed state.  Block (n0,n1) = dF/dq_n1, block
-// (n1,n0) = -dF/dq_n0; both are written into their SELL slots.
+// (n1,n0) = -dF/dq_n0; both are written into their SELL slots, and their
+// negations, the two diagonal contributions of Kernel_Diag_NumJac
+// (:436-456; the n0 row's diagonal gets minus block (n1,n0)), into ediag.
+// One thread per (edge, column): thread 9e + i perturbs variable i on
+// either side (two fluxes) against the edge's unperturbed flux, computed
+// once beforehand; nine threads share an edge's loads, at twice the
+// occupancy of one thread per edge.
 template <int NS, typename TM>
 __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ mp, int nedge,
                                                     const int* __restrict__ edges, const double* __restrict__ ea,
                                                     const double* __restrict__ q, double beta,
-                                                    const long long* __restrict__ eoff, TM* __restrict__ M) {
+                                                    const double* __restrict__ fs,
+                                                    const long long* __restrict__ eoff, TM* __restrict__ M,
+                                                    double* __restrict__ ediag) {
   constexpr int NE = NS + 4;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nedge) return;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nedge * NE) return;
+  const int e = (int)(t / NE), i = (int)(t % NE);
   const Model& m = *mp;
   State<NS> L, R;
   load_state<NS>(q, edges[2 * e], L, m);
@@ -327,26 +336,41 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
   double a[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k) a[k] = ea[4LL * e + k];
-  double fS[NE];
-  fr::hllc<NS>(m, L, R, a, beta, fS);
+  // the unperturbed flux, evaluated once per edge by k_fr_edge_flux
+  double fS[NE], fX[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) fS[k] = fs[(long long)e * NE + k];
   const double h = 1.0e-8;
   TM* oLR = M + eoff[2LL * e];
   TM* oRL = M + eoff[2LL * e + 1];
-  for (int i = 0; i < NE; ++i) {
+  double* dL = ediag + (long long)e * 2 * NE * NE;
+  double* dR = dL + NE * NE;
+  {
     State<NS> P = L;
-    P.q[i] += h;
+    // P.q[i] += h without a dynamic register index
+#pragma unroll
+    for (int k = 0; k < NE; ++k) P.q[k] = (k == i) ? P.q[k] + h : P.q[k];
     fr::aux(m, P);
-    double fL[NE], fR[NE];
-    fr::hllc<NS>(m, P, R, a, beta, fL);
-    P = R;
-    P.q[i] += h;
-    fr::aux(m, P);
-    fr::hllc<NS>(m, L, P, a, beta, fR);
+    fr::hllc<NS>(m, P, R, a, beta, fX);
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       // added to a zeroed block, as the reference's scatter does
-      oRL[(long long)(j * NE + i) * kLane] = (TM)(0.0 + (fS[j] - fL[j]) / h);
-      oLR[(long long)(j * NE + i) * kLane] = (TM)(0.0 + (fR[j] - fS[j]) / h);
+      const double tl = 0.0 + (fS[j] - fX[j]) / h;
+      oRL[(long long)(j * NE + i) * kLane] = (TM)tl;
+      dL[j * NE + i] = -tl;
+    }
+  }
+  {
+    State<NS> P = R;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) P.q[k] = (k == i) ? P.q[k] + h : P.q[k];
+    fr::aux(m, P);
+    fr::hllc<NS>(m, L, P, a, beta, fX);
+#pragma unroll
+    for (int j = 0; j < NE; ++j) {
+      const double tr = 0.0 + (fX[j] - fS[j]) / h;
+      oLR[(long long)(j * NE + i) * kLane] = (TM)tr;
+      dR[j * NE + i] = -tr;
     }
   }
 }
@@ -404,64 +428,102 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_bnd(const Model* __restrict__ mp
   if (lchanged) store_state<NS>(q, i, L);
 }
 
-// Diagonal block of every row (Kernel_Diag_NumJac, ucs/jacobian.tcc:436-456:
-// minus the transposed off-diagonal blocks in edge order, after the boundary
-// part), minus the source Jacobian (:198-209, EqnSet::SourceTermJacobian,
-// ucs/eqnset.tcc:162-187), plus the temporal terms (:213-250).  One thread
-// per SELL row; the block is built in LDS.
+// Per node: the temporal block (ContributeTemporalTerms applied to a zero
+// block, ucs/jacobian.tcc:213-250) and the source Jacobian's species rows
+// (EqnSet::SourceTermJacobian, ucs/eqnset.tcc:162-187: forward differences,
+// h = 1e-8; the velocity columns are exactly zero, the source not reading
+// u, v, w).  nterm[i] = [temporal NE*NE | source NS x (NS+1), columns
+// rho_1..rho_NS, T].
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_node_terms(const Model* __restrict__ mp, const ChemDev* __restrict__ cm,
+                                                      int nnode, const double* __restrict__ q,
+                                                      const double* __restrict__ vol, const double* __restrict__ dt,
+                                                      int rxn, double beta, double* __restrict__ nterm) {
+  constexpr int NE = NS + 4, NE2 = NE * NE, NT = NE2 + NS * (NS + 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  const Model& m = *mp;
+  double* o = nterm + (long long)i * NT;
+  State<NS> S;
+  load_state<NS>(q, i, S, m);
+  const double v = vol[i];
+  double T[NE2];
+#pragma unroll
+  for (int k = 0; k < NE2; ++k) T[k] = 0.0;
+  fr::temporal<NS>(m, S, v, dt[i], beta, T);
+#pragma unroll
+  for (int k = 0; k < NE2; ++k) o[k] = T[k];
+  if (!rxn) return;
+  double s0[NS], s1[NS], Q[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) Q[k] = S.q[k];
+  source_of(*cm, Q, v, s0);
+  const double h = 1.0e-8;
+  for (int c = 0; c <= NS; ++c) {
+    const int col = c < NS ? c : NS + 3;
+    const double keep = Q[col];
+    Q[col] += h;
+    source_of(*cm, Q, v, s1);
+    Q[col] = keep;
+    for (int j = 0; j < NS; ++j) o[NE2 + j * (NS + 1) + c] = (s1[j] - s0[j]) / h;
+  }
+}
+
+// whether ContributeTemporalTerms adds to entry (r, c) of the diagonal block
+template <int NS>
+__device__ __forceinline__ bool temporal_entry(int r, int c) {
+  constexpr int tloc = NS + 3;
+  if (r < NS) return c == r || c == tloc;
+  if (r < NS + 3) return c < NS || c == r || c == tloc;
+  return true;
+}
+
+// Diagonal block of every row: the boundary part, then minus the transposed
+// off-diagonal blocks in edge order (Kernel_Diag_NumJac, ucs/jacobian.tcc:
+// 436-456), minus the source Jacobian (:198-209), plus the temporal terms
+// (:213-250).  A workgroup takes one 64-row SELL chunk; wave w builds block
+// row w of those 64 rows (so every store is 64 consecutive rows' entry).
 template <int NS, typename TM>
-__global__ void __launch_bounds__(64) k_fr_jac_diag(const Model* __restrict__ mp, const ChemDev* __restrict__ cm,
-                                                   int nrow_pad, const int* __restrict__ srow,
-                                                   const int* __restrict__ nptr, const int* __restrict__ nslot,
-                                                   const long long* __restrict__ eoff, const TM* __restrict__ M,
-                                                   const int* __restrict__ bidx, const double* __restrict__ jdb,
-                                                   const double* __restrict__ q, const double* __restrict__ vol,
-                                                   const double* __restrict__ dt, int rxn, double beta,
-                                                   TM* __restrict__ D) {
-  constexpr int NE = NS + 4, NE2 = NE * NE;
-  __shared__ double sh[NE2][64];
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, const int* __restrict__ srow,
+                                                             const int* __restrict__ nptr,
+                                                             const int* __restrict__ nslot,
+                                                             const double* __restrict__ ediag,
+                                                             const int* __restrict__ bidx,
+                                                             const double* __restrict__ jdb,
+                                                             const double* __restrict__ nterm, int rxn,
+                                                             TM* __restrict__ D) {
+  constexpr int NE = NS + 4, NE2 = NE * NE, NT = NE2 + NS * (NS + 1);
+  const int lane = threadIdx.x & 63, rr = threadIdx.x >> 6;
+  const int r = blockIdx.x * 64 + lane;
   if (r >= nrow_pad) return;
   const int i = srow[r];
   TM* Dr = D + (long long)(r >> 6) * NE2 * kLane + (r & 63);
   if (i < 0) {
     // padding rows: identity, so the factorisation of the chunk stays finite
-    for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)((k % (NE + 1)) == 0 ? 1.0 : 0.0);
+#pragma unroll
+    for (int c = 0; c < NE; ++c) Dr[(long long)(rr * NE + c) * kLane] = (TM)(c == rr ? 1.0 : 0.0);
     return;
   }
-  const Model& m = *mp;
-  double(*A)[64] = sh;
+  double A[NE];
   const int bi = bidx[i];
-  for (int k = 0; k < NE2; ++k) A[k][lane] = bi >= 0 ? jdb[(long long)bi * NE2 + k] : 0.0;
-  for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
-    const int s = nslot[p];
-    // the block of the other row in this node's column
-    const TM* o = M + eoff[(long long)(s >> 1) * 2 + ((s & 1) ? 0 : 1)];
-    for (int k = 0; k < NE2; ++k) A[k][lane] += -(double)o[(long long)k * kLane];
-  }
-  State<NS> S;
-  load_state<NS>(q, i, S, m);
-  const double v = vol[i];
-  if (rxn) {
-    double s0[NS], s1[NS], Q[NE];
 #pragma unroll
-    for (int k = 0; k < NE; ++k) Q[k] = S.q[k];
-    source_of(*cm, Q, v, s0);
-    const double h = 1.0e-8;
-    for (int c = 0; c < NE; ++c) {
-      if (c >= NS && c < NS + 3) continue;  // the source does not read u, v, w: (S - S) / h = 0
-      const double keep = Q[c];
-      Q[c] += h;
-      source_of(*cm, Q, v, s1);
-      Q[c] = keep;
-      for (int j = 0; j < NS; ++j) A[j * NE + c][lane] -= (s1[j] - s0[j]) / h;
-    }
+  for (int c = 0; c < NE; ++c) A[c] = bi >= 0 ? jdb[(long long)bi * NE2 + rr * NE + c] : 0.0;
+  for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
+    const double* e = ediag + (long long)nslot[p] * NE2 + rr * NE;
+#pragma unroll
+    for (int c = 0; c < NE; ++c) A[c] += e[c];
   }
-  double T[NE2];
-  for (int k = 0; k < NE2; ++k) T[k] = A[k][lane];
-  fr::temporal<NS>(m, S, v, dt[i], beta, T);
-  for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)T[k];
+  const double* nt = nterm + (long long)i * NT;
+  if (rxn && rr < NS) {
+#pragma unroll
+    for (int c = 0; c < NS; ++c) A[c] -= nt[NE2 + rr * (NS + 1) + c];
+    A[NS + 3] -= nt[NE2 + rr * (NS + 1) + NS];
+  }
+#pragma unroll
+  for (int c = 0; c < NE; ++c)
+    if (temporal_entry<NS>(rr, c)) A[c] += nt[rr * NE + c];
+#pragma unroll
+  for (int c = 0; c < NE; ++c) Dr[(long long)(rr * NE + c) * kLane] = (TM)A[c];
 }
 
 // PrepareSGS (ucs/crsmatrix.tcc:839-876): LU with logical pivoting of every
@@ -503,46 +565,81 @@ __global__ void __launch_bounds__(64) k_fr_prepare(int nrow_pad, TM* __restrict_
   long long pk = 0;
   for (int k = 0; k < NE; ++k) pk |= (long long)p[k] << (4 * k);
   piv[r] = pk;
-  for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = (TM)A[k];
+  // stored with the rows in pivot order (row k holds factor row p[k]), so the
+  // solve indexes the factor statically
+  for (int k = 0; k < NE; ++k)
+    for (int c = 0; c < NE; ++c) Dr[(long long)(k * NE + c) * kLane] = (TM)A[p[k] * NE + c];
+}
+
+// LuSolve (ucs/matrix.h:237-264) on a factor stored with its rows in pivot
+// order: the same products and sums as the logical-pivot form, with only
+// the right-hand side picked by the pivot
+template <int N, typename T>
+__device__ __forceinline__ void lu_solve_rows(const T* a, T* b, const int* p) {
+  T x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    T sum = 0.0;
+#pragma unroll
+    for (int j = 0; j < i; ++j) sum += a[i * N + j] * x[j];
+    T bp = b[0];
+#pragma unroll
+    for (int k = 1; k < N; ++k) bp = (p[i] == k) ? b[k] : bp;
+    x[i] = bp - sum;
+  }
+#pragma unroll
+  for (int i = N - 1; i >= 0; --i) {
+    T sum = 0.0;
+#pragma unroll
+    for (int j = N - 1; j > i; --j) sum += a[i * N + j] * b[j];
+    b[i] = (x[i] - sum) / a[i * N + i];
+  }
 }
 
 // One colour of a symmetric Gauss-Seidel sweep (CRS::SGS, ucs/crs.tcc:61-173):
 // rows of a colour share no block, so they update together; per row the
 // neighbours' products are subtracted in slot (edge) order, then LuSolve.
+// A workgroup takes one 64-row SELL chunk; wave w forms component w of the
+// 64 rows' right-hand sides (block row w of every slot: each block-entry load
+// is 64 consecutive rows' entry), the nine components meet in LDS, and wave
+// 0 solves the diagonal blocks.  Per component the arithmetic is the
+// reference's matrix-vector product row and subtraction, in its order.
 template <int NS, typename TM>
-__global__ void __launch_bounds__(kFB) k_fr_sgs_color(int row0, int nrows, const int* __restrict__ srow,
-                                                     const int* __restrict__ swidth, const int* __restrict__ sbase,
-                                                     const int* __restrict__ scol, const TM* __restrict__ Moff,
-                                                     const TM* __restrict__ D, const long long* __restrict__ piv,
-                                                     const double* __restrict__ b, TM* __restrict__ x) {
+__global__ void __launch_bounds__(64 * (NS + 4)) k_fr_sgs_color(int chunk0, const int* __restrict__ srow,
+                                                              const int* __restrict__ swidth,
+                                                              const int* __restrict__ sbase,
+                                                              const int* __restrict__ scol, const TM* __restrict__ Moff,
+                                                              const TM* __restrict__ D,
+                                                              const long long* __restrict__ piv,
+                                                              const double* __restrict__ b, TM* __restrict__ x) {
   constexpr int NE = NS + 4, NE2 = NE * NE;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrows) return;
-  const int r = row0 + t;
+  __shared__ TM sh[NE][64];
+  const int lane = threadIdx.x & 63, rr = threadIdx.x >> 6;
+  const int chunk = chunk0 + blockIdx.x;
+  const int r = chunk * 64 + lane;
   const int i = srow[r];
-  if (i < 0) return;
-  const int chunk = r >> 6, lane = r & 63;
-  TM rhs[NE];
+  if (i >= 0) {
+    TM rhs = (TM)b[(long long)i * NE + rr];
+    const int w = swidth[chunk];
+    const long long cb = (long long)sbase[chunk] * kLane + lane;
+    const TM* mb = Moff + (long long)sbase[chunk] * NE2 * kLane + (long long)(rr * NE) * kLane + lane;
+    for (int s = 0; s < w; ++s) {
+      const int j = scol[cb + (long long)s * kLane];
+      if (j < 0) break;
+      const TM* xj = x + (long long)j * NE;
+      const TM* blk = mb + (long long)s * NE2 * kLane;
+      TM v = blk[0] * xj[0];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) rhs[k] = (TM)b[(long long)i * NE + k];
-  const int w = swidth[chunk];
-  const long long cb = (long long)sbase[chunk] * kLane + lane;
-  const TM* mb = Moff + (long long)sbase[chunk] * NE2 * kLane + lane;
-  for (int s = 0; s < w; ++s) {
-    const int j = scol[cb + (long long)s * kLane];
-    if (j < 0) break;
-    TM xj[NE];
-#pragma unroll
-    for (int k = 0; k < NE; ++k) xj[k] = x[(long long)j * NE + k];
-    const TM* blk = mb + (long long)s * NE2 * kLane;
-#pragma unroll
-    for (int rr = 0; rr < NE; ++rr) {
-      TM v = blk[(long long)(rr * NE) * kLane] * xj[0];
-#pragma unroll
-      for (int c = 1; c < NE; ++c) v += blk[(long long)(rr * NE + c) * kLane] * xj[c];
-      rhs[rr] -= v;
+      for (int c = 1; c < NE; ++c) v += blk[(long long)c * kLane] * xj[c];
+      rhs -= v;
     }
+    sh[rr][lane] = rhs;
   }
+  __syncthreads();
+  if (rr != 0 || i < 0) return;
+  TM rv[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) rv[k] = sh[k][lane];
   const TM* dg = D + (long long)chunk * NE2 * kLane + lane;
   TM A[NE2];
 #pragma unroll
@@ -551,9 +648,9 @@ __global__ void __launch_bounds__(kFB) k_fr_sgs_color(int row0, int nrows, const
   int p[NE];
 #pragma unroll
   for (int k = 0; k < NE; ++k) p[k] = (int)((pk >> (4 * k)) & 15);
-  fr::lu_solve<NE, TM>(A, rhs, p);
+  lu_solve_rows<NE, TM>(A, rv, p);
 #pragma unroll
-  for (int k = 0; k < NE; ++k) x[(long long)i * NE + k] = rhs[k];
+  for (int k = 0; k < NE; ++k) x[(long long)i * NE + k] = rv[k];
 }
 
 template <typename T>
@@ -658,6 +755,7 @@ struct pcfd_fr {
       *bc_type = nullptr, *bnd = nullptr, *bidx = nullptr, *srow = nullptr, *swidth = nullptr, *sbase = nullptr,
       *scol = nullptr, *icount = nullptr;
   long long *eoff = nullptr, *piv = nullptr;
+  double *ediag = nullptr, *nterm = nullptr;  // Jacobian diagonal parts: per edge side, per node
   double *Md = nullptr, *Dd = nullptr, *xd = nullptr;
   float *Mf = nullptr, *Df = nullptr, *xf = nullptr;
   // production-rate surrogate in the residual (null: exact rates)
@@ -697,6 +795,7 @@ void free_all(pcfd_fr& f) {
   dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount); dfree(f.eoff);
   dfree(f.piv); dfree(f.Md); dfree(f.Dd); dfree(f.xd); dfree(f.Mf); dfree(f.Df); dfree(f.xf); dfree(f.dmodel);
   dfree(f.s_rhoi); dfree(f.s_T); dfree(f.s_w); dfree(f.s_lst); dfree(f.s_cnt); dfree(f.s_off);
+  dfree(f.ediag); dfree(f.nterm);
   if (f.e0) (void)hipEventDestroy(f.e0);
   if (f.e1) (void)hipEventDestroy(f.e1);
   if (f.e2) (void)hipEventDestroy(f.e2);
@@ -821,8 +920,11 @@ template <typename TM>
 void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
   if (f.nedge > 0) {
     sbeg(f);
-    hipLaunchKernelGGL((k_fr_jac_edge<NS5, TM>), dim3(nblocks(f.nedge)), dim3(kFB), 0, f.stream, f.dmodel, f.nedge,
-                       f.edges, f.edge_a, f.q, f.prm.beta, f.eoff, M);
+    hipLaunchKernelGGL(k_fr_edge_flux<NS5>, dim3(nblocks(f.nedge)), dim3(kFB), 0, f.stream, f.dmodel, f.nedge,
+                       f.edges, f.edge_a, f.q, f.prm.beta, f.eflux);
+    hcheck(hipGetLastError(), "k_fr_edge_flux");
+    hipLaunchKernelGGL((k_fr_jac_edge<NS5, TM>), dim3(nblocks((long long)f.nedge * (NS5 + 4))), dim3(kFB), 0,
+                       f.stream, f.dmodel, f.nedge, f.edges, f.edge_a, f.q, f.prm.beta, f.eflux, f.eoff, M, f.ediag);
     hcheck(hipGetLastError(), "k_fr_jac_edge");
     send(f, 6);
   }
@@ -831,9 +933,11 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
                        f.he_ptr, f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref, f.nnode, f.prm.beta, f.q, f.jdb);
     hcheck(hipGetLastError(), "k_fr_jac_bnd");
   }
-  hipLaunchKernelGGL((k_fr_jac_diag<NS5, TM>), dim3(nblocks(f.nrow_pad, 64)), dim3(64), 0, f.stream, f.dmodel,
-                     chem_device_model(f.chem), f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.eoff, M, f.bidx, f.jdb, f.q,
-                     f.vol, f.dt, f.prm.reactions, f.prm.beta, D);
+  hipLaunchKernelGGL(k_fr_node_terms<NS5>, dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream, f.dmodel,
+                     chem_device_model(f.chem), f.nnode, f.q, f.vol, f.dt, f.prm.reactions, f.prm.beta, f.nterm);
+  hcheck(hipGetLastError(), "k_fr_node_terms");
+  hipLaunchKernelGGL((k_fr_jac_diag<NS5, TM>), dim3(f.nrow_pad / kLane), dim3(kLane * (NS5 + 4)), 0, f.stream,
+                     f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.ediag, f.bidx, f.jdb, f.nterm, f.prm.reactions, D);
   hcheck(hipGetLastError(), "k_fr_jac_diag");
   f.lu_done = false;
 }
@@ -874,15 +978,20 @@ void run_sgs_t(pcfd_fr& f, int nsgs, const TM* M, const TM* D, TM* x, double* de
   hipLaunchKernelGGL(k_fr_zero<TM>, dim3(nblocks(n)), dim3(kFB), 0, f.stream, n, x);
   hcheck(hipGetLastError(), "k_fr_zero");
   double xold = 0.0, xn = 0.0;
+  // A colour pass directly after a pass over the same colour (the turn of a
+  // symmetric sweep, and the start of the next) reads the same neighbour x
+  // and rewrites the same values to the bit -- a colour's rows never read
+  // each other -- so it is skipped.
+  int last = -1;
   for (int it = 0; it < nsgs; ++it) {
     for (int dir = 0; dir < 2; ++dir)
       for (int cc = 0; cc < f.ncolor; ++cc) {
         const int col = dir == 0 ? cc : f.ncolor - 1 - cc;
-        const int row0 = f.color_chunk[col] * kLane;
-        const int nrows = (f.color_chunk[col + 1] - f.color_chunk[col]) * kLane;
-        if (nrows == 0) continue;
-        hipLaunchKernelGGL((k_fr_sgs_color<NS5, TM>), dim3(nblocks(nrows)), dim3(kFB), 0, f.stream, row0, nrows,
-                           f.srow, f.swidth, f.sbase, f.scol, M, D, f.piv, f.b, x);
+        const int nch = f.color_chunk[col + 1] - f.color_chunk[col];
+        if (nch == 0 || col == last) continue;
+        last = col;
+        hipLaunchKernelGGL((k_fr_sgs_color<NS5, TM>), dim3(nch), dim3(kLane * (NS5 + 4)), 0, f.stream,
+                           f.color_chunk[col], f.srow, f.swidth, f.sbase, f.scol, M, D, f.piv, f.b, x);
         hcheck(hipGetLastError(), "k_fr_sgs_color");
       }
     if (delta && it >= nsgs - 2) {
@@ -1166,6 +1275,8 @@ int pcfd_fr_create(const pcfd_mesh_desc* m, pcfd_chem* chem, const double* qref,
     dalloc(*f, &f->red, (size_t)(NE + 1) * std::max(nblocks(nn), 256));
     dalloc(*f, &f->icount, 4);
     dalloc(*f, &f->piv, (size_t)f->nrow_pad);
+    dalloc(*f, &f->ediag, 2 * (size_t)ne * NE2);
+    dalloc(*f, &f->nterm, (size_t)nn * (NE2 + ns * (ns + 1)));
     hcheck(hipMemset(f->q, 0, sizeof(double) * (size_t)f->ntot * NE), "hipMemset");
   } catch (const std::exception& e) {
     g_fr_create_err = e.what();
