@@ -399,7 +399,7 @@ class OFr(C.Structure):
 
 class OFrCrs(C.Structure):
     _fields_ = [("nnode", C.c_int), ("nblocks", C.c_int), ("neqn", C.c_int), ("ia", ip), ("ja", ip), ("iau", ip),
-                ("e_lr", ip), ("e_rl", ip), ("pv", ip), ("M", dp)]
+                ("e_lr", ip), ("e_rl", ip), ("pv", ip), ("M", dp), ("ludiag", C.c_int)]
 
 
 class OFrSolver(C.Structure):

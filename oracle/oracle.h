@@ -170,6 +170,7 @@ typedef struct {
   int nnode, nblocks, neqn;
   int *ia, *ja, *iau, *e_lr, *e_rl, *pv;
   double* M; /* [nblocks*neqn*neqn] */
+  int ludiag; /* diagonal blocks hold their LU factors (CRSMatrix::ludiag) */
 } o_fr_crs;
 
 typedef struct {

@@ -704,6 +704,7 @@ int o_fr_crs_init(const o_mesh* m, int neqn, o_fr_crs* A) {
   free(base.M);
   A->pv = (int*)malloc(sizeof(int) * neqn * (size_t)A->nnode);
   A->M = (double*)calloc((size_t)A->nblocks * neqn * neqn, sizeof(double));
+  A->ludiag = 0;
   return 0;
 }
 
@@ -733,6 +734,7 @@ void o_fr_jacobian(const o_fr* f, const o_mesh* m, const double* qref, double* q
   double fS[O_CHEM_MAXS + 4], fL[O_CHEM_MAXS + 4], fR[O_CHEM_MAXS + 4];
   double tL[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)], tR[(O_CHEM_MAXS + 4) * (O_CHEM_MAXS + 4)];
   memset(A->M, 0, sizeof(double) * (size_t)A->nblocks * n2);
+  A->ludiag = 0; /* CRSMatrix::Blank (ucs/crsmatrix.tcc:506-510) */
   /* Kernel_NumJac (ucs/jacobian.tcc:253-304) */
   for (int e = 0; e < m->nedge; e++) {
     const int L = m->edges[2 * e], R = m->edges[2 * e + 1];
@@ -828,7 +830,10 @@ static void fr_sgs_row(o_fr_crs* A, int node, const double* b, double* x) {
 
 double o_fr_sgs(o_fr_crs* A, int nsgs, const double* b, double* x, const int* order) {
   const int n = A->neqn, nn = A->nnode;
-  for (int i = 0; i < nn; i++) o_lu(blk(A, A->iau[i]), &A->pv[(size_t)n * i], n);
+  /* PrepareSGS factors once per assembly (ludiag, ucs/crsmatrix.tcc:840-876) */
+  if (!A->ludiag)
+    for (int i = 0; i < nn; i++) o_lu(blk(A, A->iau[i]), &A->pv[(size_t)n * i], n);
+  A->ludiag = 1;
   memset(x, 0, sizeof(double) * (size_t)n * nn);
   double xOld = 0.0, xNorm = 0.0;
   for (int it = 0; it < nsgs; it++) {

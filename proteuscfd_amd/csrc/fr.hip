@@ -1051,9 +1051,9 @@ double step(pcfd_fr& f, double* norms, int* nzeroed) {
   int nz = 0;
   if (f.prm.nsgs > 0) {
     tbeg(f);
-    // PrepareSGS factors the stored diagonal blocks every iteration, as the
-    // reference's NewtonIterate does
-    run_prepare(f);
+    // PrepareSGS factors the diagonal blocks once per assembly (its ludiag
+    // flag, reset when the matrix is rebuilt, ucs/crsmatrix.tcc:505-510,839-876)
+    if (!f.lu_done) run_prepare(f);
     run_sgs(f, f.prm.nsgs, nullptr);
     tend(f, 4);
     tbeg(f);

@@ -562,11 +562,32 @@ def make_fr_wedge_nocvbc():
     return fr_out(run, out, meta)
 
 
+def make_fr_bump_jacfreq():
+    """The bump case of make_fr_bump_solve with jacobianUpdateFrequency 3:
+    40 iterations of the reference's loop in which the Jacobian (and its
+    factored diagonal) is rebuilt on iterations 1, 3, 6, 9, ... and reused
+    between them.  History and the final field."""
+    case = cases.bump_case(n=4, rcm=False)
+    keys = fr_keys(velocity=0.5, initialTemperature=3000.0, initialPressure=101325, refPressure=101325,
+                   massFractions="[0.10, 0.10, 0.05, 0.70, 0.05]", numberSGS=6, CFL=20.0, startingCFL=1.0,
+                   rampCFL=30, jacobianUpdateFrequency=3)
+    run = FRRun(case, keys, bump_bc(), nsteps=40)
+    meta, qinf = run.ref_scales()
+    run("trace", 1)
+    ia, ja = run.arr("ia", np.int32), run.arr("ja", np.int32)
+    run("solve", 40, 1e-14)
+    m = case.mesh
+    out = {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 21),
+           "qinf": qinf, "ia": ia, "ja": ja}
+    return fr_out(run, out, meta)
+
+
 MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wedge_phases": make_wedge_phases,
           "pointwise": make_pointwise, "bump6_solve": make_bump6_solve, "plate_phases": make_plate_phases,
           "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure,
           "fr_wedge_phases": make_fr_wedge_phases, "fr_bump_phases": make_fr_bump_phases,
-          "fr_bump_solve": make_fr_bump_solve, "fr_wedge_nocvbc": make_fr_wedge_nocvbc}
+          "fr_bump_solve": make_fr_bump_solve, "fr_wedge_nocvbc": make_fr_wedge_nocvbc,
+          "fr_bump_jacfreq": make_fr_bump_jacfreq}
 
 
 def main(names):

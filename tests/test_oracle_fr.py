@@ -19,7 +19,8 @@ NS = MODEL.ns
 SCALES = ["ref_pressure", "ref_velocity", "ref_density", "ref_time", "ref_temperature"]
 # BC ids per face tag of the fixtures' meshes (ucs/bc_defines.h)
 BCT = {"fr_wedge_phases": [0, 7, 8, 3, 3, 9, 9], "fr_bump_phases": [0, 6, 6, 6, 6, 3, 6],
-       "fr_bump_solve": [0, 6, 6, 6, 6, 3, 6], "fr_wedge_nocvbc": [0, 7, 8, 3, 3, 9, 9]}
+       "fr_bump_solve": [0, 6, 6, 6, 6, 3, 6], "fr_wedge_nocvbc": [0, 7, 8, 3, 3, 9, 9],
+       "fr_bump_jacfreq": [0, 6, 6, 6, 6, 3, 6]}
 
 
 def load(name):
@@ -103,6 +104,21 @@ def test_fr_steady_solve_matches_reference_bitwise():
     np.testing.assert_array_equal(s.q[:pb.nnode], d["q_final"][:pb.nnode])
     T = s.q[:pb.nnode, NS + 3] * float(d["ref_temperature"])
     assert 3000.0 < T.min() and T.max() < 4000.0  # recombination heats the gas
+
+
+def test_fr_reused_jacobian_matches_reference_bitwise():
+    """jacobianUpdateFrequency 3: the Jacobian is assembled on iterations 1,
+    3, 6, ... and its factored diagonal reused by the SGS sweeps between
+    them (PrepareSGS factors once per assembly).  40 iterations of the
+    reference's loop, every residGlobal and the final field bit for bit;
+    factoring the already factored blocks again would change both."""
+    d, fr, pb = setup("fr_bump_jacfreq")
+    q0 = np.tile(d["qinf"], (pb.nnode + pb.nbnode, 1))
+    s = O.FRSolver(pb, q0, cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6, jac_freq=3, ja=d["ja"])
+    hist = np.array([s.step()[0] for _ in range(len(d["history"]))])
+    np.testing.assert_array_equal(hist, d["history"])
+    np.testing.assert_array_equal(s.q[:pb.nnode], d["q_final"][:pb.nnode])
+    assert hist[-1] < 1e-3 * hist[0]
 
 
 def test_fr_wedge_nocvbc_matches_reference_bitwise():
