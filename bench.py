@@ -66,8 +66,10 @@ def sgs_pass_bytes(m, nsgs, prec_bytes):
     return float(nbytes), len(run), len(seq)
 
 
-def phase_profile(ctx, case, iters):
-    """Run `iters` implicit iterations with per-phase HIP-event timing."""
+def phase_profile(ctx, case, iters, jac_freq=1):
+    """Run `iters` implicit iterations with per-phase HIP-event timing: ms
+    per iteration, and for the Jacobian also ms per assembly (iterations
+    0, 1 and the multiples of jac_freq assemble it)."""
     ctx.upload_q(case.initial_state())
     ctx.reset_iteration()
     ctx.enable_phase_timing(True)
@@ -76,7 +78,12 @@ def phase_profile(ctx, case, iters):
         ctx.iterate()
     pt = ctx.phase_times(reset=True)
     ctx.enable_phase_timing(False)
-    return {k: v / iters for k, v in pt.items()}
+    out = {k: v / iters for k, v in pt.items()}
+    builds = sum(1 for it in range(iters) if it % jac_freq == 0 or it == 1)
+    out["jacobian_builds"] = builds
+    out["jacobian_per_build"] = pt["jacobian"] / builds
+    out["k_jacobian_per_build"] = pt["k_jacobian"] / builds
+    return out
 
 
 def timed_solves(ctx, case, k, max_iter, drop):
@@ -235,13 +242,19 @@ def make_case(kind, args, parts=1):
     return case, base, workload
 
 
-def run_forms(case, base, warmup, profile_iters, iters_only, steps, max_iter, drop_goal):
-    """Warm up, profile and solve the case in the plain and the chosen form."""
+def run_forms(case, base, warmup, profile_iters, iters_only, steps, max_iter, drop_goal, chosen_jac_freq=1,
+              forms=((0, "plain_fp64"), (1, "chosen_fp32"))):
+    """Warm up, profile and solve the case in the plain and the chosen form.
+    The plain form is the reference's method as configured (Jacobian every
+    iteration); the chosen form stores the Jacobian and SGS in FP32 and
+    refreshes the Jacobian every chosen_jac_freq iterations (the reference's
+    jacobianUpdateFrequency): both change only the preconditioner."""
     from proteuscfd_amd import gpu
     from proteuscfd_amd.cases import Params
     results = {}
-    for prec, name in [(0, "plain_fp64"), (1, "chosen_fp32")]:
-        p = Params(**base, precision=prec)
+    for prec, name in forms:
+        jf = chosen_jac_freq if (prec == 1 or name.endswith("same_refresh")) else 1
+        p = Params(**base, precision=prec, jac_freq=jf)
         ctx = gpu.Context(case, p, gradient_terms=9)
         info = ctx.info()
         ctx.upload_q(case.initial_state())
@@ -249,7 +262,7 @@ def run_forms(case, base, warmup, profile_iters, iters_only, steps, max_iter, dr
         for _ in range(warmup):
             ctx.iterate()
         ctx.synchronize()
-        prof = phase_profile(ctx, case, profile_iters)
+        prof = phase_profile(ctx, case, profile_iters, jf)
         log(f"[bench] {name} phase ms/iter: " + ", ".join(f"{k}={v:.3f}" for k, v in prof.items()))
         if iters_only:
             ctx.upload_q(case.initial_state())
@@ -266,7 +279,7 @@ def run_forms(case, base, warmup, profile_iters, iters_only, steps, max_iter, dr
         drop = float(h[-1] / h[0]) if len(h) > 1 else 1.0
         log(f"[bench] {name}: {iters} iterations, {['%.3f' % x for x in times]} s, drop {drop:.2e}")
         qfinal = ctx.download_q()
-        results[name] = dict(times=times, iters=iters, prof=prof, info=info, drop=drop, q=qfinal)
+        results[name] = dict(times=times, iters=iters, prof=prof, info=info, drop=drop, q=qfinal, jac_freq=jf)
         ctx.close()
     return results
 
@@ -297,6 +310,10 @@ def main():
                     help="plate: 8M-cell laminar flat plate (the headline workload); bump: 1.03M-node Euler bump")
     ap.add_argument("--plate", default="200,40,180", help="plate hexes nx,ny,nz (the lowest 20 become prism layers)")
     ap.add_argument("--cfl", type=float, default=0.0, help="final CFL (default 50 bump, 100 plate)")
+    ap.add_argument("--chosen-jac-freq", type=int, default=4,
+                    help="chosen form: Jacobian refreshed every N iterations (jacobianUpdateFrequency)")
+    ap.add_argument("--no-attribution", action="store_true",
+                    help="skip the plain FP64 solve with the chosen form's Jacobian refresh")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the record of the other workload that follows the headline")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
@@ -321,7 +338,16 @@ def main():
     log(f"[bench] mesh {args.case}: {m.nnode} nodes {m.nedge} edges {m.nbedge} half-edges "
         f"({time.time() - t0:.1f}s)")
     results = run_forms(case, base, args.warmup, args.profile_iters, args.iters_only, args.steps, args.max_iter,
-                        args.drop)
+                        args.drop, args.chosen_jac_freq)
+    same = None
+    if not args.iters_only and args.chosen_jac_freq > 1 and not args.no_attribution:
+        # attribution: the plain FP64 path with the chosen form's Jacobian
+        # refresh, one solve (how much of the speed-up is the FP32 storage)
+        r = run_forms(case, base, 2, 2, 0, 1, args.max_iter, args.drop, args.chosen_jac_freq,
+                      forms=((0, "plain_fp64_same_refresh"),))["plain_fp64_same_refresh"]
+        same = {"s": float(np.mean(r["times"])), "iterations": r["iters"][0], "jac_freq": r["jac_freq"],
+                "field_max_rel_err_vs_plain": field_error({"plain_fp64": results["plain_fp64"],
+                                                           "chosen_fp32": r}, m.nnode)}
 
     plain, chosen = results["plain_fp64"], results["chosen_fp32"]
     t_plain = float(np.mean(plain["times"]))
@@ -345,8 +371,9 @@ def main():
         # per symmetric sweep (2 per SGS iteration); bytes of the passes actually run
         "sgs_sweep_fp64": dict(ms=pp["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs64, pp["k_sgs_sweeps"])),
         "sgs_sweep_fp32": dict(ms=pc["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs32, pc["k_sgs_sweeps"])),
-        "jacobian_fp64": dict(ms=pp["k_jacobian"], gbs=gbs(ab64["jacobian"], pp["k_jacobian"])),
-        "jacobian_fp32": dict(ms=pc["k_jacobian"], gbs=gbs(ab32["jacobian"], pc["k_jacobian"])),
+        # per assembly
+        "jacobian_fp64": dict(ms=pp["k_jacobian_per_build"], gbs=gbs(ab64["jacobian"], pp["k_jacobian_per_build"])),
+        "jacobian_fp32": dict(ms=pc["k_jacobian_per_build"], gbs=gbs(ab32["jacobian"], pc["k_jacobian_per_build"])),
         "gradient9": dict(ms=pp["gradient"], gbs=gbs(ab64["gradient9"], pp["gradient"])),
     }
     if base.get("limiter"):
@@ -399,6 +426,8 @@ def main():
         "chosen_fp32_s": t_chosen,
         "speedup_chosen_vs_plain": speedup,
         "speedup_target": 2.0,
+        "jacobian_update_frequency": {"plain": 1, "chosen": args.chosen_jac_freq},
+        "plain_fp64_same_refresh": same,
         "iterations": {"plain": plain["iters"], "chosen": chosen["iters"]},
         "residual_drop": {"plain": plain["drop"], "chosen": chosen["drop"]},
         "edge_fluxes_per_s": edge_flux,
@@ -429,7 +458,7 @@ def extra_record(kind, args):
     case, base, workload = make_case(kind, args)
     m = case.mesh
     log(f"[bench] mesh {kind}: {m.nnode} nodes {m.nedge} edges ({time.time() - t0:.1f}s)")
-    res = run_forms(case, base, 2, 5, 0, 1, 4000, args.drop)
+    res = run_forms(case, base, 2, 5, 0, 1, 4000, args.drop, args.chosen_jac_freq)
     tp = float(np.mean(res["plain_fp64"]["times"]))
     tc = float(np.mean(res["chosen_fp32"]["times"]))
     ip, ic = res["plain_fp64"]["iters"][0], res["chosen_fp32"]["iters"][0]
@@ -438,6 +467,7 @@ def extra_record(kind, args):
         "plain_fp64_s": tp,
         "chosen_fp32_s": tc,
         "speedup_chosen_vs_plain": tp / tc if tc > 0 else None,
+        "jacobian_update_frequency": {"plain": 1, "chosen": args.chosen_jac_freq},
         "iterations": {"plain": ip, "chosen": ic},
         "residual_drop": {"plain": res["plain_fp64"]["drop"], "chosen": res["chosen_fp32"]["drop"]},
         "wall_ms_per_iteration": {"plain": 1e3 * tp / max(ip, 1), "chosen": 1e3 * tc / max(ic, 1)},

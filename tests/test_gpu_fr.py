@@ -210,8 +210,10 @@ def test_fr_reused_jacobian_steps():
     """jacobianUpdateFrequency 3 (the Jacobian and its factored diagonal
     reused between assemblies).  Reactions off: 10 device steps and the
     oracle's 10 in the device's row orders, identical to the bit.  Reactions
-    on: 40 steps against the reference's own 40 (natural-order SGS), the
-    residual history to 1e-3 and the field to 1e-6 per variable."""
+    on: 40 steps against the reference's own 40 (natural-order SGS, so the
+    iterates differ in the last bits from the first colour sweep on), the
+    residual history to 1e-2 relative down to its 1e-14 floor and the
+    solved field to 1e-10 per variable."""
     d = load("fr_bump_jacfreq")
     kw = dict(cfl=20.0, cfl_start=1.0, cfl_ramp=30, nsgs=6, jac_freq=3)
     _, f, fr = context("fr_bump_jacfreq", reactions=False, **kw)
@@ -233,8 +235,8 @@ def test_fr_reused_jacobian_steps():
     ref = d["history"]
     err = _field_error(f.download_q()[:f.nnode, :NE], d["q_final"][:f.nnode, :NE])
     print("history", float(np.abs(h / ref - 1).max()), "field", err, flush=True)
-    assert np.abs(h / ref - 1).max() < 1e-3
-    assert max(err) < 1e-6
+    assert np.abs(h / ref - 1).max() < 1e-2
+    assert max(err) < 1e-10
 
 
 def test_fr_rerun_is_bitwise_identical():
