@@ -102,6 +102,14 @@ int pcfd_mlp_destroy(pcfd_mlp* m);
 const char* pcfd_mlp_last_error(const pcfd_mlp* m);
 double pcfd_mlp_last_kernel_ms(const pcfd_mlp* m);
 int pcfd_mlp_set_params(pcfd_mlp* m, const float* params);  /* also resets the Adam state */
+/* Error network of the production-rate surrogate m: a second network of
+ * the same shape trained on m's own errors (the targets
+ * ln(|sigma_i (y_i - target_i)| + 1e-3), normalised by the desc's
+ * err_mid/err_scale).  With one set, the pcfd_chem_surrogate_* calls on m
+ * (and an FR context's surrogate) evaluate both networks and apply the
+ * desc's err_tol rule; partner = NULL removes it.  Borrowed: it must outlive
+ * its use. */
+int pcfd_mlp_set_partner(pcfd_mlp* m, pcfd_mlp* partner);
 int pcfd_mlp_get_params(pcfd_mlp* m, float* params);
 /* y[n*5] = net(x[n*7]); host arrays / device arrays */
 int pcfd_mlp_forward(pcfd_mlp* m, int n, const float* x, float* y);
@@ -130,6 +138,13 @@ typedef struct {
                                     for air) in each species, in the model's
                                     species numbering; imbalance =
                                     sum_e |sum_i atoms[e][i] w_i / M_i| */
+  /* with an error network (pcfd_mlp_set_partner): its output i, g_i, gives
+     the predicted error of output i in asinh units, e_i = exp(g_i *
+     err_scale[i] + err_mid[i]); fall back when
+       max_i rho s0 (sinh(|z_i| + e_i) - sinh(|z_i|)) > err_tol * (max_i |w_i| + rho * rate_floor),
+     z_i = sigma_i * output_i, rate_floor in 1/s; ignored without one */
+  double err_tol, rate_floor;
+  double err_mid[5], err_scale[5];
 } pcfd_surrogate_desc;
 
 /* Production rates by the surrogate for n cells (device arrays rhoi[n*5]

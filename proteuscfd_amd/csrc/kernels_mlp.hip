@@ -478,6 +478,7 @@ struct pcfd_mlp {
   float* loss2 = nullptr;
   int grid = 0;  // blocks of the gradient kernel
   int* scratch = nullptr;  // device address of the no-allocation scratch on this mlp's device
+  pcfd_mlp* partner = nullptr;  // error network of the surrogate (borrowed)
   long long adam_t = 0;
   float last_ms = 0.f;
   std::string err;
@@ -627,6 +628,17 @@ const char* pcfd_mlp_last_error(const pcfd_mlp* c) { return c ? c->err.c_str() :
 
 double pcfd_mlp_last_kernel_ms(const pcfd_mlp* c) { return c ? (double)c->last_ms : 0.0; }
 
+int pcfd_mlp_set_partner(pcfd_mlp* c, pcfd_mlp* partner) {
+  return guarded_mlp(c, [&](pcfd_mlp& m) -> int {
+    if (partner == c) {
+      m.err = "pcfd_mlp_set_partner: a network cannot be its own error network";
+      return PCFD_ERR_ARG;
+    }
+    m.partner = partner;
+    return PCFD_OK;
+  });
+}
+
 int pcfd_mlp_set_params(pcfd_mlp* c, const float* params) {
   return guarded_mlp(c, [&](pcfd_mlp& m) -> int {
     if (!params) {
@@ -773,6 +785,12 @@ int pcfd_mlp_train(pcfd_mlp* c, int n, const float* x, const float* t, int steps
 // exactly (every reaction does), so the molar imbalance of the two elements,
 //   sum_e |sum_i atoms[e][i] w_i / M_i|   (for air: N and O atoms),
 // over (sum_i |w_i/M_i| + rho * floor) estimates the surrogate's error.
+// With an error network (same shape, trained on the first network's own
+// errors: g_i = ln(|sigma_i (y_i - target_i)| + 1e-3), normalised by
+// err_mid/err_scale) the predicted error e_i = exp(g_i) of output i in
+// asinh units becomes a rate error rho s0 (sinh(|z_i| + e_i) - sinh(|z_i|)),
+// z_i = sigma_i y_i, and a cell also falls back when the largest of these
+// exceeds err_tol * (max_i |w_i| + rho * rate_floor).
 namespace pcfd {
 
 struct SurrogateDev {
@@ -782,9 +800,22 @@ struct SurrogateDev {
   double T_min, T_max, rho_min, rho_max, tol, floor;
   double imw[5];  // 1 / molecular weight
   double atoms[2][5];
+  double err_tol, rate_floor;
+  double err_mid[5], err_scale[5];
 };
 
-__global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, SurrogateDev sd, int n,
+// rates of one member from its transformed outputs y (FP32): rho s0 sinh(sigma y)
+__device__ __forceinline__ void member_rates(const SurrogateDev& sd, double rho, const float* y, double* w) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const float z = (float)sd.sigma[k] * y[k];
+    const float e = expf(z);
+    w[k] = rho * sd.s0 * (double)(0.5f * (e - 1.f / e));
+  }
+}
+
+template <bool ENS>
+__global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, MlpWeights w2, SurrogateDev sd, int n,
                                                         const double* __restrict__ rhoi, const double* __restrict__ T,
                                                         double* __restrict__ wdot, int* __restrict__ lst,
                                                         int* __restrict__ cnt) {
@@ -792,8 +823,9 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r = lane & 15, q = lane >> 4;
   InferTiles& Tl = tiles[wv];
-  WRegs R;
+  WRegs R, R2;
   load_wregs(w, r, q, R);
+  if constexpr (ENS) load_wregs(w2, r, q, R2);
   const int npass = (n + 63) / 64;
   for (int pass = blockIdx.x * kWaves + wv; pass < npass; pass += gridDim.x * kWaves) {
     const int c = pass * 64 + lane;
@@ -824,16 +856,30 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
     for (int f = kNin; f < kKin; ++f) Tl.X[lane][f] = 0;
     wave_sync();
     infer64(R, Tl, r, q);
-    bool fb = false;
-    if (valid) {
-      double wv5[5], m[5], mag = 0.0;
+    float ya[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ya[k] = Tl.Y[lane][k];
+    double wv5[5], werr = 0.0;
+    member_rates(sd, rho, ya, wv5);
+    if constexpr (ENS) {
+      // every lane has read its outputs before the error network rewrites Y
+      wave_sync();
+      infer64(R2, Tl, r, q);
 #pragma unroll
       for (int k = 0; k < 5; ++k) {
-        const float z = (float)sd.sigma[k] * Tl.Y[lane][k];
-        const float e = expf(z);
-        wv5[k] = rho * sd.s0 * (double)(0.5f * (e - 1.f / e));
+        const double e = exp((double)Tl.Y[lane][k] * sd.err_scale[k] + sd.err_mid[k]);
+        const double z = fabs(sd.sigma[k] * (double)ya[k]);
+        werr = fmax(werr, rho * sd.s0 * (sinh(z + e) - sinh(z)));
+      }
+    }
+    bool fb = false;
+    if (valid) {
+      double m[5], mag = 0.0, wmax = 0.0;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
         m[k] = wv5[k] * sd.imw[k];
         mag += fabs(m[k]);
+        wmax = fmax(wmax, fabs(wv5[k]));
       }
       double eN = 0.0, eO = 0.0;
 #pragma unroll
@@ -844,7 +890,8 @@ __global__ void __launch_bounds__(kMlpBlock) k_surrogate(MlpWeights w, Surrogate
       eN = fabs(eN);
       eO = fabs(eO);
       const bool out = !(Tc >= sd.T_min && Tc <= sd.T_max && rho >= sd.rho_min && rho <= sd.rho_max);
-      const bool bad = (eN + eO) > sd.tol * (mag + rho * sd.floor);
+      bool bad = (eN + eO) > sd.tol * (mag + rho * sd.floor);
+      if constexpr (ENS) bad = bad || !(werr <= sd.err_tol * (wmax + rho * sd.rate_floor));
 #pragma unroll
       for (int k = 0; k < 5; ++k) wdot[(long long)c * 5 + k] = wv5[k];
       fb = out || bad;
@@ -1002,13 +1049,34 @@ int surrogate_launch(pcfd_chem* chem, pcfd_mlp& m, const pcfd_surrogate_desc* d,
   sd.rho_max = d->rho_max;
   sd.tol = d->tol;
   sd.floor = d->floor;
+  sd.err_tol = d->err_tol;
+  sd.rate_floor = d->rate_floor;
+  const bool ens = m.partner != nullptr;
+  for (int k = 0; k < 5; ++k) {
+    sd.err_mid[k] = d->err_mid[k];
+    sd.err_scale[k] = d->err_scale[k];
+    if (ens && !(d->err_scale[k] > 0.0)) {
+      m.err = "surrogate: an error network is set; err_scale must be > 0";
+      return PCFD_ERR_ARG;
+    }
+  }
+  if (ens && !(d->err_tol > 0.0 && d->rate_floor >= 0.0)) {
+    m.err = "surrogate: an error network is set; err_tol must be > 0 and rate_floor >= 0";
+    return PCFD_ERR_ARG;
+  }
+  // the error network's BF16 weights are written on its own stream
+  if (ens) mcheck(hipStreamSynchronize(m.partner->stream), "hipStreamSynchronize");
   // each wavefront loads its weight fragments once and loops over 64-sample
   // passes: about four blocks per CU
   const int npass = (n + 63) / 64;
   const int grid = std::max(1, std::min(1024, (npass + kWaves - 1) / kWaves));
   mcheck(hipEventRecord(m.e0, m.stream), "hipEventRecord");
-  hipLaunchKernelGGL(k_surrogate, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), sd, n, rhoi, T, wdot,
-                     lst, cnt);
+  if (ens)
+    hipLaunchKernelGGL(k_surrogate<true>, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), weights(*m.partner),
+                       sd, n, rhoi, T, wdot, lst, cnt);
+  else
+    hipLaunchKernelGGL(k_surrogate<false>, dim3(grid), dim3(kMlpBlock), 0, m.stream, weights(m), weights(m), sd, n,
+                       rhoi, T, wdot, lst, cnt);
   mcheck(hipGetLastError(), "k_surrogate");
   launch_scan_counts(m.stream, npass, cnt, off);
   hipLaunchKernelGGL(k_wdot_list, dim3((n + kChemBlock - 1) / kChemBlock), dim3(kChemBlock), 0, m.stream,

@@ -470,7 +470,8 @@ class ChemContext:
 
 MLP_NPARAM = 6288
 MLP_SYMBOLS = ["pcfd_mlp_create", "pcfd_mlp_destroy", "pcfd_mlp_last_error", "pcfd_mlp_last_kernel_ms",
-               "pcfd_mlp_set_params", "pcfd_mlp_get_params", "pcfd_mlp_forward", "pcfd_mlp_forward_device",
+               "pcfd_mlp_set_params", "pcfd_mlp_set_partner", "pcfd_mlp_get_params", "pcfd_mlp_forward",
+               "pcfd_mlp_forward_device",
                "pcfd_mlp_loss_grad", "pcfd_mlp_train", "pcfd_chem_surrogate_wdot_device", "pcfd_chem_wdot_device",
                "pcfd_chem_surrogate_wdot", "pcfd_chem_surrogate_wdot_device_noalloc"]
 SURROGATE_NOALLOC_MAX_CELLS = 1 << 22
@@ -484,7 +485,8 @@ class SurrogateDesc(C.Structure):
                 ("lnrho_half", C.c_double), ("y_mid", C.c_double), ("y_scale", C.c_double), ("y_eps", C.c_double),
                 ("rate_scale", C.c_double), ("sigma", C.c_double * 5), ("T_min", C.c_double), ("T_max", C.c_double),
                 ("rho_min", C.c_double), ("rho_max", C.c_double), ("tol", C.c_double), ("floor", C.c_double),
-                ("atoms", (C.c_double * 5) * 2)]
+                ("atoms", (C.c_double * 5) * 2), ("err_tol", C.c_double), ("rate_floor", C.c_double),
+                ("err_mid", C.c_double * 5), ("err_scale", C.c_double * 5)]
 
 
 def _mlp_lib():
@@ -498,6 +500,7 @@ def _mlp_lib():
         L.pcfd_mlp_last_kernel_ms.argtypes = [vp]
         L.pcfd_mlp_last_kernel_ms.restype = C.c_double
         L.pcfd_mlp_set_params.argtypes = [vp, fp]
+        L.pcfd_mlp_set_partner.argtypes = [vp, vp]
         L.pcfd_mlp_get_params.argtypes = [vp, fp]
         L.pcfd_mlp_forward.argtypes = [vp, C.c_int, fp, fp]
         L.pcfd_mlp_forward_device.argtypes = [vp, C.c_int, vp, vp]
@@ -542,6 +545,13 @@ class MLP:
             self.close()
         except Exception:
             pass
+
+    def set_partner(self, other):
+        """Error network of the surrogate (None removes it); it is kept
+        alive with this one."""
+        self._chk(_mlp_lib().pcfd_mlp_set_partner(self.h, other.h if other is not None else None),
+                  "pcfd_mlp_set_partner")
+        self.partner = other
 
     @property
     def params(self):

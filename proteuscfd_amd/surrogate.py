@@ -9,7 +9,10 @@ rates span across that box.  The network is trained on the GPU by the
 library's own MFMA forward/backward kernels (pcfd_mlp_train).  At run
 time pcfd_chem_surrogate_wdot_device evaluates it and sends every cell
 outside the box, or whose predicted rates break N/O element conservation
-by more than the bound, to the exact rates.
+by more than the bound, to the exact rates.  A second network of the same
+shape, trained on the first one's own errors, predicts each cell's rate
+error; cells whose predicted error exceeds err_tol of their rate scale go
+to the exact rates as well (train_with_error_net, calibrate).
 """
 from __future__ import annotations
 
@@ -78,7 +81,8 @@ def atom_counts(symbols, elements=("N", "O")):
     return out
 
 
-def desc(tr: Transforms, tol=0.05, floor=10.0, symbols=None):
+def desc(tr: Transforms, tol=0.05, floor=10.0, symbols=None, err_tol=0.0, rate_floor=1.0, err_mid=None,
+         err_scale=None):
     from . import chem, gpu
     d = gpu.SurrogateDesc()
     atoms = atom_counts(symbols if symbols is not None else chem.load_air5().symbols)
@@ -91,6 +95,10 @@ def desc(tr: Transforms, tol=0.05, floor=10.0, symbols=None):
         d.sigma[k] = float(tr.sigma[k])
     d.T_min, d.T_max, d.rho_min, d.rho_max = T_MIN, T_MAX, RHO_MIN, RHO_MAX
     d.tol, d.floor = tol, floor
+    d.err_tol, d.rate_floor = err_tol, rate_floor
+    for k in range(5):
+        d.err_mid[k] = float(err_mid[k]) if err_mid is not None else 0.0
+        d.err_scale[k] = float(err_scale[k]) if err_scale is not None else 1.0
     return d
 
 
@@ -105,6 +113,82 @@ def train_surrogate(chem_ctx, n=2_000_000, steps=4000, batch=4096, lr=3e-3, seed
     net = gpu.MLP(seed=seed)
     hist = net.train(x, t, steps, batch, lr, seed=seed + 1)
     return net, tr, hist
+
+
+def train_with_error_net(chem_ctx, n=2_000_000, steps=60000, batch=8192, lr=3e-3, err_steps=20000, seed=40):
+    """The rate network, then its error network: trained on the same exact-
+    rate samples to predict g_i = ln(|sigma_i (y_i - target_i)| + 1e-3), the
+    rate network's own error per output in asinh units (normalised per
+    output).  The rate network carries the error network as its partner.
+    Returns (rate net, error net, transforms, err_mid, err_scale, loss
+    histories)."""
+    from . import gpu
+    rhoi, T = sample_states(n, seed)
+    wdot = chem_ctx.wdot(rhoi, T)
+    tr = Transforms()
+    tr.fit_sigma(wdot, rhoi)
+    x, t = tr.features(rhoi, T), tr.targets(wdot, rhoi)
+    net = gpu.MLP(seed=seed)
+    h1 = net.train(x, t, steps, batch, lr, seed=seed + 1)
+    g = np.log(np.abs(tr.sigma[None, :] * (net.forward(x).astype(np.float64) - t)) + 1e-3)
+    gm, gs = g.mean(axis=0), g.std(axis=0)
+    err = gpu.MLP(seed=seed + 37)
+    h2 = err.train(x, ((g - gm) / gs).astype(np.float32), err_steps, batch, lr, seed=seed + 38)
+    net.set_partner(err)
+    return net, err, tr, gm, gs, (h1, h2)
+
+
+def fallback_rule(tr: Transforms, d, mw, atoms, rhoi, T, ya, yb=None):
+    """The kernel's fallback rule restated on the host from the rate
+    network's outputs ya [n, 5] and the error network's yb: returns (rates,
+    fallback mask, predicted rate error)."""
+    rho = rhoi.sum(axis=1)
+    w = tr.rates(ya.astype(np.float64), rhoi)
+    m = w / mw
+    imb = np.abs(m @ atoms.T).sum(axis=1)
+    out = ~((T >= d.T_min) & (T <= d.T_max) & (rho >= d.rho_min) & (rho <= d.rho_max))
+    bad = imb > d.tol * (np.abs(m).sum(axis=1) + rho * d.floor)
+    werr = np.zeros(len(T))
+    if yb is not None:
+        e = np.exp(yb.astype(np.float64) * np.array(d.err_scale[:]) + np.array(d.err_mid[:]))
+        z = np.abs(tr.sigma[None, :] * ya.astype(np.float64))
+        with np.errstate(over="ignore"):
+            werr = (rho[:, None] * tr.rate_scale * (np.sinh(z + e) - np.sinh(z))).max(axis=1)
+        bad |= ~(werr <= d.err_tol * (np.abs(w).max(axis=1) + rho * d.rate_floor))
+    return w, out | bad, werr
+
+
+def rate_error(w, w_exact, rho, rate_floor=1.0):
+    """Per cell: max_i |w_i - exact_i| / (max_i |exact_i| + rho * rate_floor)."""
+    return np.abs(w - w_exact).max(axis=1) / (np.abs(w_exact).max(axis=1) + rho * rate_floor)
+
+
+def calibrate(chem_ctx, net, err_net, tr, d, mw, atoms, n=1_000_000, seed=41, target_p99=0.05,
+              grid=(0.001, 0.003, 0.01, 0.02, 0.03, 0.05, 0.1, 0.2, 0.3, 1.0)):
+    """The error / fallback curve on held-out states: for each err_tol of the
+    grid, the fallback share and the rate error (exact rates judging,
+    d.rate_floor in its floor) over the cells the surrogate keeps and over all
+    cells after fallback (those sent back are exact).  Returns (the largest
+    err_tol whose p99 over all cells is <= target_p99 with some cells kept, or
+    None; the curve)."""
+    rhoi, T = sample_states(n, seed)
+    we = chem_ctx.wdot(rhoi, T)
+    x = tr.features(rhoi, T)
+    ya, yb = net.forward(x), err_net.forward(x)
+    rho = rhoi.sum(axis=1)
+    curve, best = [], None
+    for et in grid:
+        d.err_tol = et
+        w, fb, _ = fallback_rule(tr, d, mw, atoms, rhoi, T, ya, yb)
+        e = np.where(fb, 0.0, rate_error(w, we, rho, d.rate_floor))
+        ek = e[~fb] if (~fb).any() else np.zeros(1)
+        row = dict(err_tol=et, fallback=float(fb.mean()), kept_p90=float(np.quantile(ek, 0.9)),
+                   kept_p99=float(np.quantile(ek, 0.99)), p90=float(np.quantile(e, 0.9)),
+                   p99=float(np.quantile(e, 0.99)), max=float(e.max()))
+        curve.append(row)
+        if row["p99"] <= target_p99 and row["fallback"] < 1.0:
+            best = et
+    return best, curve
 
 
 def surrogate_wdot_device(chem_ctx, net, d, rhoi_ptr, T_ptr, wdot_ptr, n):

@@ -68,32 +68,66 @@ def test_training_fits_a_smooth_map():
 
 
 def test_surrogate_gain_over_exact_chemistry():
-    """Train on exact rates from the GPU chemistry, then evaluate new cells
-    with and without the network.  Reported: device time of the exact
-    rates, of the surrogate with its exact fallback, the fallback share and
-    the error of the rates against the exact ones.  Checked: training
-    converges, the fallback rule catches the surrogate's large errors (the
-    cells it keeps are within the element-balance bound's reach), and the
-    surrogate path never returns rates further than that from exact."""
+    """The rate network (60000 steps on 2M exact-rate samples: T
+    log-uniform in [300, 12000] K, rho log-uniform in [1e-5, 1] kg/m^3,
+    Dirichlet mass fractions) and its error network (trained on the rate
+    network's own errors) form the surrogate.  Two error scales: max_i
+    |wdot_i| + rho * 1/s, and + rho / t_ref with t_ref = 1/350 s (the flow
+    time of the Mach 8 wedge, below which a rate cannot move the mass
+    fractions within a flow time).  For each, err_tol is calibrated on 1M
+    held-out states so that the p99 error over all cells after fallback is
+    at most 5% (those sent back are exact); on another 1M states the device
+    path must then fall back on the cells the host restatement of its rule
+    names (to 1% of the count) and meet the stated 5%.  The estimate must
+    rank the errors (cells kept at err_tol 0.03: at most half the p90 error
+    of those kept at 1).  Reported: the error / fallback curves and the
+    device time beside the exact rates -- at the 5% bound nearly every cell
+    goes back to the exact rates (the rate network's median error is ~10%),
+    so there the surrogate costs more than exact chemistry; the rate network
+    alone (element-balance rule, ~30% fallback, p99 ~30%) runs 1.7x faster."""
     from proteuscfd_amd import chem, gpu, surrogate as S
-    ctx = gpu.ChemContext(chem.load_air5())
-    print("[mlp] surrogate training", flush=True)
-    net, tr, hist = S.train_surrogate(ctx, n=1_000_000, steps=20000, batch=8192, lr=3e-3, seed=40)
-    assert hist[-200:].mean() < 0.05 * hist[:50].mean()
+    model = chem.load_air5()
+    ctx = gpu.ChemContext(model)
+    print("[mlp] rate network (60000 steps) and error network (20000 steps) on 2M samples", flush=True)
+    net, enet, tr, gm, gs, (h1, h2) = S.train_with_error_net(ctx, n=2_000_000)
+    assert h1[-500:].mean() < 0.05 * h1[:50].mean()
+    assert h2[-500:].mean() < h2[:50].mean()
+    atoms = S.atom_counts(model.symbols)
     n = 1_000_000
-    rhoi, T = S.sample_states(n, seed=41)
-    d = S.desc(tr, tol=0.1, floor=10.0)
-    runs_e = [S.exact_wdot(ctx, rhoi, T) for _ in range(3)]
-    runs_s = [S.surrogate_wdot(ctx, net, d, rhoi, T) for _ in range(3)]
-    we_h, ms_e = runs_e[-1][0], min(r[1] for r in runs_e)
-    ws_h, nf, ms_s = runs_s[-1][0], runs_s[-1][1], min(r[2] for r in runs_s)
+    rhoi, T = S.sample_states(n, seed=42)
     rho = rhoi.sum(axis=1)
-    err = np.abs(ws_h - we_h).max(axis=1) / (np.abs(we_h).max(axis=1) + rho * 1.0)
-    raw = S.Transforms()
-    print(f"[mlp] 1M cells: exact {ms_e:.3f} ms, surrogate+fallback {ms_s:.3f} ms (gain {ms_e / ms_s:.2f}x), "
-          f"fallback {nf / n:.1%}, rate error median {np.median(err):.2e} p90 {np.quantile(err, 0.9):.2e} "
-          f"p99 {np.quantile(err, 0.99):.2e}; final training loss {hist[-200:].mean():.2e}", flush=True)
-    assert 0 < nf < 0.6 * n
-    assert np.quantile(err, 0.9) < 0.5
+    runs_e = [S.exact_wdot(ctx, rhoi, T) for _ in range(3)]
+    we, ms_e = runs_e[-1][0], min(r[1] for r in runs_e)
+    x = tr.features(rhoi, T)
+    ya, yb = net.forward(x), enet.forward(x)
+    for floor in (1.0, 350.0):
+        d = S.desc(tr, tol=0.1, floor=10.0, rate_floor=floor, err_mid=gm, err_scale=gs)
+        best, curve = S.calibrate(ctx, net, enet, tr, d, model.mw, atoms, n=1_000_000, seed=41, target_p99=0.05)
+        for row in curve:
+            print("[mlp] floor {fl:g}/s err_tol {err_tol:<6g} fallback {fallback:6.1%}  kept p90 {kept_p90:.2e} "
+                  "p99 {kept_p99:.2e}  all p90 {p90:.2e} p99 {p99:.2e}".format(fl=floor, **row), flush=True)
+        assert best is not None
+        # the estimate ranks the errors: the cells kept at err_tol 0.03 have
+        # at most half the p90 error of the cells kept without it
+        loose = next(r for r in curve if r["err_tol"] == 1.0)
+        tight = next(r for r in curve if r["err_tol"] == 0.03)
+        assert tight["kept_p90"] <= 0.5 * loose["kept_p90"]
+        d.err_tol = best
+        runs_s = [S.surrogate_wdot(ctx, net, d, rhoi, T) for _ in range(3)]
+        ws, nf, ms_s = runs_s[-1][0], runs_s[-1][1], min(r[2] for r in runs_s)
+        _, fb_host, _ = S.fallback_rule(tr, d, model.mw, atoms, rhoi, T, ya, yb)
+        err = S.rate_error(ws, we, rho, floor)
+        print(f"[mlp] floor {floor:g}/s, err_tol {best:g}: 1M cells exact {ms_e:.3f} ms, surrogate+fallback "
+              f"{ms_s:.3f} ms (gain {ms_e / ms_s:.2f}x), fallback {nf / n:.1%} (host rule {fb_host.mean():.1%}), "
+              f"error p90 {np.quantile(err, 0.9):.2e} p99 {np.quantile(err, 0.99):.2e}", flush=True)
+        assert abs(nf - int(fb_host.sum())) <= 0.01 * max(nf, 1000)
+        assert np.quantile(err, 0.99) <= 0.05
+    net.set_partner(None)
+    d1 = S.desc(tr, tol=0.1, floor=10.0)
+    w1, nf1, ms_1 = S.surrogate_wdot(ctx, net, d1, rhoi, T)
+    err1 = S.rate_error(w1, we, rho)
+    print(f"[mlp] rate network with the element-balance rule only: {ms_1:.3f} ms, fallback {nf1 / n:.1%}, "
+          f"error p90 {np.quantile(err1, 0.9):.2e} p99 {np.quantile(err1, 0.99):.2e}", flush=True)
+    enet.close()
     net.close()
     ctx.close()
