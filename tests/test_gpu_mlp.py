@@ -101,7 +101,7 @@ def test_surrogate_gain_over_exact_chemistry():
     x = tr.features(rhoi, T)
     ya, yb = net.forward(x), enet.forward(x)
     for floor in (1.0, 350.0):
-        d = S.desc(tr, tol=0.1, floor=10.0, rate_floor=floor, err_mid=gm, err_scale=gs)
+        d = S.desc(tr, tol=0.1, floor=max(10.0, floor / 0.0289), rate_floor=floor, err_mid=gm, err_scale=gs)
         best, curve = S.calibrate(ctx, net, enet, tr, d, model.mw, atoms, n=1_000_000, seed=41, target_p99=0.05)
         for row in curve:
             print("[mlp] floor {fl:g}/s err_tol {err_tol:<6g} fallback {fallback:6.1%}  kept p90 {kept_p90:.2e} "
