@@ -519,11 +519,11 @@ def fr_record(args):
     t_train = time.time() - t0
     log(f"[bench] surrogate trained in {t_train:.1f}s")
     runs, fields = {}, {}
-    jf = args.chosen_jac_freq
+    # the chosen form refreshes the Jacobian every step here: with the stiff
+    # source Jacobian lagged over four steps the reacting solve diverges
     for name, prec, sur, jfreq in [("plain_fp64", gpu.PLAIN_FP64, False, 1),
-                                   ("chosen_fp32", gpu.CHOSEN_FP32, False, jf),
-                                   ("chosen_fp32_surrogate", gpu.CHOSEN_FP32, True, jf),
-                                   ("chosen_fp32_jacobian_every_step", gpu.CHOSEN_FP32, False, 1)]:
+                                   ("chosen_fp32", gpu.CHOSEN_FP32, False, 1),
+                                   ("chosen_fp32_surrogate", gpu.CHOSEN_FP32, True, 1)]:
         f = gpu.FrContext(m, bct, np.tile(qinf, (len(bct), 1)), cc)
         f.set_params(sc["ref_pressure"], sc["ref_velocity"], Pref, 1.0, cfl=5.0, cfl_start=1.0, cfl_ramp=50, nsgs=6,
                      precision=prec, no_cvbc=True, jac_freq=jfreq)
@@ -575,9 +575,7 @@ def fr_record(args):
         "speedup_chosen_vs_plain": runs["plain_fp64"]["s"] / runs["chosen_fp32"]["s"],
         "field_max_rel_err_vs_plain": {
             "chosen_fp32": dict(zip(names, ferr(fields["chosen_fp32"], base))),
-            "chosen_fp32_surrogate": dict(zip(names, ferr(fields["chosen_fp32_surrogate"], base))),
-            "chosen_fp32_jacobian_every_step": dict(zip(names, ferr(fields["chosen_fp32_jacobian_every_step"],
-                                                                    base)))},
+            "chosen_fp32_surrogate": dict(zip(names, ferr(fields["chosen_fp32_surrogate"], base)))},
         "surrogate_vs_exact_field_max_rel_err": dict(zip(names, ferr(fields["chosen_fp32_surrogate"],
                                                                      fields["chosen_fp32"]))),
         "surrogate": {"train_s": t_train, "samples": 2_000_000, "err_tol": args.fr_err_tol,

@@ -2,7 +2,8 @@
 
 The training set is made by the repository's own exact chemistry on the
 GPU (pcfd_chem_wdot): T log-uniform in [300, 12000] K, rho log-uniform in
-[1e-5, 1] kg/m^3, mass fractions from a symmetric Dirichlet(1).  Inputs
+[1e-5, 1] kg/m^3, mass fractions from a symmetric Dirichlet(1) for half of
+the states and partly dissociated air for the other half.  Inputs
 are (ln T, ln rho, Y_1..Y_5) scaled to about [-1, 1]; targets are
 asinh(wdot_i / (rho s0)) / sigma_i, which compresses the ~90 decades the
 rates span across that box.  The network is trained on the GPU by the
@@ -26,11 +27,39 @@ T_MIN, T_MAX = 300.0, 12000.0
 RHO_MIN, RHO_MAX = 1e-5, 1.0
 
 
-def sample_states(n, seed):
+# air's element mass fractions and the molar masses (kg/mol) of O, N, NO
+Y_O_AIR, Y_N_AIR = 0.233, 0.767
+M_O, M_N = 0.0159994, 0.0140067
+
+
+def air_compositions(rng, n, symbols=("O2", "O", "N", "N2", "NO")):
+    """Mass fractions of partly dissociated air with air's N/O element
+    ratio: NO takes a mass fraction log-uniform in [1e-10, 0.1], the rest of
+    the O atoms a share a of them as O and the rest of the N atoms a share c
+    as N (a, c log-uniform in [1e-10, 1])."""
+    x_no = np.exp(rng.uniform(np.log(1e-10), np.log(0.1), n))
+    a = np.exp(rng.uniform(np.log(1e-10), 0.0, n))
+    c = np.exp(rng.uniform(np.log(1e-10), 0.0, n))
+    o_left = Y_O_AIR - x_no * M_O / (M_O + M_N)
+    n_left = Y_N_AIR - x_no * M_N / (M_O + M_N)
+    y = {"O2": o_left * (1 - a), "O": o_left * a, "N2": n_left * (1 - c), "N": n_left * c, "NO": x_no}
+    return np.stack([y[s] for s in symbols], axis=1)
+
+
+def sample_states(n, seed, air_share=0.5):
+    """T log-uniform in [T_MIN, T_MAX], rho log-uniform in [RHO_MIN,
+    RHO_MAX]; mass fractions from a symmetric Dirichlet(1) for a share
+    1 - air_share of the states and partly dissociated air (the compositions
+    an air flow visits, which the Dirichlet draws leave at the simplex's
+    corner) for the rest."""
     rng = np.random.default_rng(seed)
     T = np.exp(rng.uniform(np.log(T_MIN), np.log(T_MAX), n))
     rho = np.exp(rng.uniform(np.log(RHO_MIN), np.log(RHO_MAX), n))
     Y = rng.dirichlet(np.ones(5), n)
+    na = int(round(air_share * n))
+    if na:
+        Y[:na] = air_compositions(rng, na)
+        rng.shuffle(Y, axis=0)
     return rho[:, None] * Y, T
 
 
