@@ -70,7 +70,7 @@ def test_training_fits_a_smooth_map():
 def test_surrogate_gain_over_exact_chemistry():
     """The rate network (60000 steps on 2M exact-rate samples: T
     log-uniform in [300, 12000] K, rho log-uniform in [1e-5, 1] kg/m^3,
-    Dirichlet mass fractions) and its error network (trained on the rate
+    mass fractions half Dirichlet, half partly dissociated air) and its error network (trained on the rate
     network's own errors) form the surrogate.  Two error scales: max_i
     |wdot_i| + rho * 1/s, and + rho / t_ref with t_ref = 1/350 s (the flow
     time of the Mach 8 wedge, below which a rate cannot move the mass
@@ -81,10 +81,9 @@ def test_surrogate_gain_over_exact_chemistry():
     names (to 1% of the count) and meet the stated 5%.  The estimate must
     rank the errors (cells kept at err_tol 0.03: at most half the p90 error
     of those kept at 1).  Reported: the error / fallback curves and the
-    device time beside the exact rates -- at the 5% bound nearly every cell
-    goes back to the exact rates (the rate network's median error is ~10%),
-    so there the surrogate costs more than exact chemistry; the rate network
-    alone (element-balance rule, ~30% fallback, p99 ~30%) runs 1.7x faster."""
+    device time beside the exact rates -- at the 5% bound most cells go back
+    to the exact rates (the rate network's median error is ~10%), so there
+    the surrogate costs more than exact chemistry."""
     from proteuscfd_amd import chem, gpu, surrogate as S
     model = chem.load_air5()
     ctx = gpu.ChemContext(model)
