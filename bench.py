@@ -315,6 +315,7 @@ def main():
     ap.add_argument("--no-attribution", action="store_true",
                     help="skip the plain FP64 solve with the chosen form's Jacobian refresh")
     ap.add_argument("--no-fr", action="store_true", help="skip the finite-rate reacting wedge record")
+    ap.add_argument("--only-fr", action="store_true", help="run the finite-rate reacting wedge record alone")
     ap.add_argument("--fr-wedge", default="200,100,17", help="FR wedge hexes nx,ny,nz (6 tets each)")
     ap.add_argument("--fr-steps", type=int, default=300)
     ap.add_argument("--fr-err-tol", type=float, default=0.05,
@@ -336,6 +337,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.only_fr:
+        print(json.dumps({"reacting_wedge_fr": fr_record(args)}))
+        return
 
     t0 = time.time()
     case, base, workload = make_case(args.case, args)
