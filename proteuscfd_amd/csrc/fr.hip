@@ -614,27 +614,43 @@ __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_sgs_color(int chunk0, cons
                                                               const double* __restrict__ b, TM* __restrict__ x) {
   constexpr int NE = NS + 4, NE2 = NE * NE;
   __shared__ TM sh[NE][64];
+  // the neighbours' x for a group of SB slots, double-buffered by group
+  // parity: wave rr gathers component rr for the 64 rows, every wave reads
+  // all NE (one gather per component instead of one per wave)
+  constexpr int SB = 4;
+  __shared__ TM xs[2][SB][NE][64];
   const int lane = threadIdx.x & 63, rr = threadIdx.x >> 6;
   const int chunk = chunk0 + blockIdx.x;
   const int r = chunk * 64 + lane;
   const int i = srow[r];
-  if (i >= 0) {
-    TM rhs = (TM)b[(long long)i * NE + rr];
-    const int w = swidth[chunk];
-    const long long cb = (long long)sbase[chunk] * kLane + lane;
-    const TM* mb = Moff + (long long)sbase[chunk] * NE2 * kLane + (long long)(rr * NE) * kLane + lane;
-    for (int s = 0; s < w; ++s) {
-      const int j = scol[cb + (long long)s * kLane];
-      if (j < 0) break;
-      const TM* xj = x + (long long)j * NE;
-      const TM* blk = mb + (long long)s * NE2 * kLane;
-      TM v = blk[0] * xj[0];
+  TM rhs = i >= 0 ? (TM)b[(long long)i * NE + rr] : (TM)0;
+  const int w = swidth[chunk];
+  const long long cb = (long long)sbase[chunk] * kLane + lane;
+  const TM* mb = Moff + (long long)sbase[chunk] * NE2 * kLane + (long long)(rr * NE) * kLane + lane;
+  // every wave runs the chunk's w slots (the barriers need all of them);
+  // a row's padding slots (j < 0) contribute nothing
+  for (int s0 = 0; s0 < w; s0 += SB) {
+    TM* xg = &xs[(s0 / SB) & 1][0][0][0];
+    int jj[SB];
 #pragma unroll
-      for (int c = 1; c < NE; ++c) v += blk[(long long)c * kLane] * xj[c];
-      rhs -= v;
+    for (int t = 0; t < SB; ++t) {
+      jj[t] = (i >= 0 && s0 + t < w) ? scol[cb + (long long)(s0 + t) * kLane] : -1;
+      xg[(t * NE + rr) * 64 + lane] = jj[t] >= 0 ? x[(long long)jj[t] * NE + rr] : (TM)0;
     }
-    sh[rr][lane] = rhs;
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < SB; ++t) {
+      if (jj[t] >= 0) {
+        const TM* blk = mb + (long long)(s0 + t) * NE2 * kLane;
+        const TM* xb = xg + t * NE * 64;
+        TM v = blk[0] * xb[lane];
+#pragma unroll
+        for (int c = 1; c < NE; ++c) v += blk[(long long)c * kLane] * xb[c * 64 + lane];
+        rhs -= v;
+      }
+    }
   }
+  if (i >= 0) sh[rr][lane] = rhs;
   __syncthreads();
   if (rr != 0 || i < 0) return;
   TM rv[NE];
