@@ -311,9 +311,11 @@ __global__ void __launch_bounds__(kFB) k_fr_timestep(const Model* __restrict__ m
 // Kernel_NumJac (ucs/jacobian.tcc:253-304): one-sided differences (h = 1e-8)
 // of the edge's HLLC flux in each native variable of either side, aux
 // recomputed for every perturbed state.  Block (n0,n1) = dF/dq_n1, block
-// (n1,n0) = -dF/dq_n0; both are written into their SELL slots, and their
-// negations, the two diagonal contributions of Kernel_Diag_NumJac
-// (:436-456; the n0 row's diagonal gets minus block (n1,n0)), into ediag.
+// (n1,n0) = -dF/dq_n0; their negations, the two diagonal contributions of
+// Kernel_Diag_NumJac (:436-456; the n0 row's diagonal gets minus block
+// (n1,n0)), are written to ediag, contiguously per edge side, and
+// k_fr_jac_diag places the blocks in their SELL slots (scattered 8-byte
+// stores from here cost more than the second read there).
 // One thread per (edge, column): thread 9e + i perturbs variable i on
 // either side (two fluxes) against the edge's unperturbed flux, computed
 // once beforehand; nine threads share an edge's loads, at twice the
@@ -323,7 +325,6 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
                                                     const int* __restrict__ edges, const double* __restrict__ ea,
                                                     const double* __restrict__ q, double beta,
                                                     const double* __restrict__ fs,
-                                                    const long long* __restrict__ eoff, TM* __restrict__ M,
                                                     double* __restrict__ ediag) {
   constexpr int NE = NS + 4;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,8 +342,6 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
 #pragma unroll
   for (int k = 0; k < NE; ++k) fS[k] = fs[(long long)e * NE + k];
   const double h = 1.0e-8;
-  TM* oLR = M + eoff[2LL * e];
-  TM* oRL = M + eoff[2LL * e + 1];
   double* dL = ediag + (long long)e * 2 * NE * NE;
   double* dR = dL + NE * NE;
   {
@@ -354,9 +353,9 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
     fr::hllc<NS>(m, P, R, a, beta, fX);
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
-      // added to a zeroed block, as the reference's scatter does
+      // added to a zeroed block, as the reference's scatter does; the
+      // off-diagonal block A[R][L] is -dL (written by k_fr_jac_diag)
       const double tl = 0.0 + (fS[j] - fX[j]) / h;
-      oRL[(long long)(j * NE + i) * kLane] = (TM)tl;
       dL[j * NE + i] = -tl;
     }
   }
@@ -369,8 +368,7 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const double tr = 0.0 + (fX[j] - fS[j]) / h;
-      oLR[(long long)(j * NE + i) * kLane] = (TM)tr;
-      dR[j * NE + i] = -tr;
+      dR[j * NE + i] = -tr;  // A[L][R] = -dR
     }
   }
 }
@@ -481,7 +479,8 @@ __device__ __forceinline__ bool temporal_entry(int r, int c) {
 // Diagonal block of every row: the boundary part, then minus the transposed
 // off-diagonal blocks in edge order (Kernel_Diag_NumJac, ucs/jacobian.tcc:
 // 436-456), minus the source Jacobian (:198-209), plus the temporal terms
-// (:213-250).  A workgroup takes one 64-row SELL chunk; wave w builds block
+// (:213-250); and the row's off-diagonal blocks from the other sides of
+// its edges.  A workgroup takes one 64-row SELL chunk; wave w builds block
 // row w of those 64 rows (so every store is 64 consecutive rows' entry).
 template <int NS, typename TM>
 __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, const int* __restrict__ srow,
@@ -491,6 +490,7 @@ __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, con
                                                              const int* __restrict__ bidx,
                                                              const double* __restrict__ jdb,
                                                              const double* __restrict__ nterm, int rxn,
+                                                             const int* __restrict__ sbase, TM* __restrict__ M,
                                                              TM* __restrict__ D) {
   constexpr int NE = NS + 4, NE2 = NE * NE, NT = NE2 + NS * (NS + 1);
   const int lane = threadIdx.x & 63, rr = threadIdx.x >> 6;
@@ -508,10 +508,21 @@ __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, con
   const int bi = bidx[i];
 #pragma unroll
   for (int c = 0; c < NE; ++c) A[c] = bi >= 0 ? jdb[(long long)bi * NE2 + rr * NE + c] : 0.0;
+  // slot s of the row is its CSR slot nptr[i] + s: the row's own side of
+  // the edge adds to the diagonal, the other side, negated, is the row's
+  // off-diagonal block (A[L][R] = -dR, A[R][L] = -dL), written here with
+  // contiguous lanes
+  TM* mo = M + (long long)sbase[r >> 6] * NE2 * kLane + (long long)(rr * NE) * kLane + (r & 63);
   for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
-    const double* e = ediag + (long long)nslot[p] * NE2 + rr * NE;
+    const int sd = nslot[p];
+    const double* e = ediag + (long long)sd * NE2 + rr * NE;
+    const double* o = ediag + (long long)(sd ^ 1) * NE2 + rr * NE;
+    TM* mb = mo + (long long)(p - nptr[i]) * NE2 * kLane;
 #pragma unroll
-    for (int c = 0; c < NE; ++c) A[c] += e[c];
+    for (int c = 0; c < NE; ++c) {
+      A[c] += e[c];
+      mb[(long long)c * kLane] = (TM)(-o[c]);
+    }
   }
   const double* nt = nterm + (long long)i * NT;
   if (rxn && rr < NS) {
@@ -940,7 +951,7 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
                        f.edges, f.edge_a, f.q, f.prm.beta, f.eflux);
     hcheck(hipGetLastError(), "k_fr_edge_flux");
     hipLaunchKernelGGL((k_fr_jac_edge<NS5, TM>), dim3(nblocks((long long)f.nedge * (NS5 + 4))), dim3(kFB), 0,
-                       f.stream, f.dmodel, f.nedge, f.edges, f.edge_a, f.q, f.prm.beta, f.eflux, f.eoff, M, f.ediag);
+                       f.stream, f.dmodel, f.nedge, f.edges, f.edge_a, f.q, f.prm.beta, f.eflux, f.ediag);
     hcheck(hipGetLastError(), "k_fr_jac_edge");
     send(f, 6);
   }
@@ -953,7 +964,8 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
                      chem_device_model(f.chem), f.nnode, f.q, f.vol, f.dt, f.prm.reactions, f.prm.beta, f.nterm);
   hcheck(hipGetLastError(), "k_fr_node_terms");
   hipLaunchKernelGGL((k_fr_jac_diag<NS5, TM>), dim3(f.nrow_pad / kLane), dim3(kLane * (NS5 + 4)), 0, f.stream,
-                     f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.ediag, f.bidx, f.jdb, f.nterm, f.prm.reactions, D);
+                     f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.ediag, f.bidx, f.jdb, f.nterm, f.prm.reactions,
+                     f.sbase, M, D);
   hcheck(hipGetLastError(), "k_fr_jac_diag");
   f.lu_done = false;
 }
