@@ -325,7 +325,7 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
                                                     const int* __restrict__ edges, const double* __restrict__ ea,
                                                     const double* __restrict__ q, double beta,
                                                     const double* __restrict__ fs,
-                                                    double* __restrict__ ediag) {
+                                                    TM* __restrict__ ediag) {
   constexpr int NE = NS + 4;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nedge * NE) return;
@@ -342,8 +342,9 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
 #pragma unroll
   for (int k = 0; k < NE; ++k) fS[k] = fs[(long long)e * NE + k];
   const double h = 1.0e-8;
-  double* dL = ediag + (long long)e * 2 * NE * NE;
-  double* dR = dL + NE * NE;
+  // in the storage precision: the FP32 form's diagonal sums FP32 terms
+  TM* dL = ediag + (long long)e * 2 * NE * NE;
+  TM* dR = dL + NE * NE;
   {
     State<NS> P = L;
     // P.q[i] += h without a dynamic register index
@@ -356,7 +357,7 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
       // added to a zeroed block, as the reference's scatter does; the
       // off-diagonal block A[R][L] is -dL (written by k_fr_jac_diag)
       const double tl = 0.0 + (fS[j] - fX[j]) / h;
-      dL[j * NE + i] = -tl;
+      dL[j * NE + i] = (TM)(-tl);
     }
   }
   {
@@ -368,7 +369,7 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
 #pragma unroll
     for (int j = 0; j < NE; ++j) {
       const double tr = 0.0 + (fX[j] - fS[j]) / h;
-      dR[j * NE + i] = -tr;  // A[L][R] = -dR
+      dR[j * NE + i] = (TM)(-tr);  // A[L][R] = -dR
     }
   }
 }
@@ -486,7 +487,7 @@ template <int NS, typename TM>
 __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, const int* __restrict__ srow,
                                                              const int* __restrict__ nptr,
                                                              const int* __restrict__ nslot,
-                                                             const double* __restrict__ ediag,
+                                                             const TM* __restrict__ ediag,
                                                              const int* __restrict__ bidx,
                                                              const double* __restrict__ jdb,
                                                              const double* __restrict__ nterm, int rxn,
@@ -515,13 +516,13 @@ __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, con
   TM* mo = M + (long long)sbase[r >> 6] * NE2 * kLane + (long long)(rr * NE) * kLane + (r & 63);
   for (int p = nptr[i]; p < nptr[i + 1]; ++p) {
     const int sd = nslot[p];
-    const double* e = ediag + (long long)sd * NE2 + rr * NE;
-    const double* o = ediag + (long long)(sd ^ 1) * NE2 + rr * NE;
+    const TM* e = ediag + (long long)sd * NE2 + rr * NE;
+    const TM* o = ediag + (long long)(sd ^ 1) * NE2 + rr * NE;
     TM* mb = mo + (long long)(p - nptr[i]) * NE2 * kLane;
 #pragma unroll
     for (int c = 0; c < NE; ++c) {
-      A[c] += e[c];
-      mb[(long long)c * kLane] = (TM)(-o[c]);
+      A[c] += (double)e[c];
+      mb[(long long)c * kLane] = -o[c];
     }
   }
   const double* nt = nterm + (long long)i * NT;
@@ -951,7 +952,8 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
                        f.edges, f.edge_a, f.q, f.prm.beta, f.eflux);
     hcheck(hipGetLastError(), "k_fr_edge_flux");
     hipLaunchKernelGGL((k_fr_jac_edge<NS5, TM>), dim3(nblocks((long long)f.nedge * (NS5 + 4))), dim3(kFB), 0,
-                       f.stream, f.dmodel, f.nedge, f.edges, f.edge_a, f.q, f.prm.beta, f.eflux, f.ediag);
+                       f.stream, f.dmodel, f.nedge, f.edges, f.edge_a, f.q, f.prm.beta, f.eflux,
+                       reinterpret_cast<TM*>(f.ediag));
     hcheck(hipGetLastError(), "k_fr_jac_edge");
     send(f, 6);
   }
@@ -964,7 +966,8 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
                      chem_device_model(f.chem), f.nnode, f.q, f.vol, f.dt, f.prm.reactions, f.prm.beta, f.nterm);
   hcheck(hipGetLastError(), "k_fr_node_terms");
   hipLaunchKernelGGL((k_fr_jac_diag<NS5, TM>), dim3(f.nrow_pad / kLane), dim3(kLane * (NS5 + 4)), 0, f.stream,
-                     f.nrow_pad, f.srow, f.nptr, f.nslot_d, f.ediag, f.bidx, f.jdb, f.nterm, f.prm.reactions,
+                     f.nrow_pad, f.srow, f.nptr, f.nslot_d, reinterpret_cast<const TM*>(f.ediag), f.bidx, f.jdb,
+                     f.nterm, f.prm.reactions,
                      f.sbase, M, D);
   hcheck(hipGetLastError(), "k_fr_jac_diag");
   f.lu_done = false;
