@@ -335,6 +335,29 @@ def make_plate_solve():
             "ref_temperature": np.float64(run.meta["ref_temperature"])}
 
 
+def make_plate_bench_solve():
+    """The reference's own steady loop on a small channel with the bench
+    plate's boundary set: freestream exactly along the floor (no
+    incidence), farfield inflow and outflow, symmetry top and sides,
+    isothermal noSlip floor -- where the farfield's theta == 0 branch point
+    sits at the outflow corner of the floor (laminar NS, second order,
+    unlimited MUSCL, 8 SGS, CFL 1 -> 50 over 30 iterations), to an 8-order
+    drop: history and converged field."""
+    case = cases.plate_case(nx=12, ny=3, nz=10, nlayers=4, first_h=0.01, growth=1.3, twall=1.0, incidence_deg=0.0,
+                            outflow_bc=cases.BC_FARFIELD, top_bc=cases.BC_SYMMETRY)
+    bc = {1: 6, 2: 6, 3: 9, 4: 9, 5: 4, 6: 9}
+    extra = {5: f" twall = [{300.0:.17g}]"}
+    keys = euler_keys(equationSet="compressibleNS", flowDirection="[1.0, 0.0, 0.0]", initialTemperature=300,
+                      refViscosity=0.01225, velocity=0.3, spatialOrder=2, limiter=0, numberSGS=8, CFL=50.0,
+                      startingCFL=1.0, rampCFL=30)
+    run = Run(case, keys, bc, nsteps=3000, extra=extra)
+    run("solve", 3000, 1e-8)
+    m = case.mesh
+    return {"history": run.arr("history"), "q_final": run.arr("q_final").reshape(m.nnode + m.nbedge, 10),
+            "gamma": np.float64(run.meta["gamma"]), "qinf": run.arr("qinf"), "Re": np.float64(run.meta["Re"]),
+            "ref_temperature": np.float64(run.meta["ref_temperature"])}
+
+
 def make_binding():
     """The descriptors the reference-side binding (proteuscfd_amd/examples/
     ucs_binding.hpp, compiled against the reference's headers) builds from
@@ -587,7 +610,7 @@ MAKERS = {"bump5_phases": make_bump5_phases, "bump6_clip": make_bump6_clip, "wed
           "plate_solve": make_plate_solve, "binding": make_binding, "bump5_failure": make_bump5_failure,
           "fr_wedge_phases": make_fr_wedge_phases, "fr_bump_phases": make_fr_bump_phases,
           "fr_bump_solve": make_fr_bump_solve, "fr_wedge_nocvbc": make_fr_wedge_nocvbc,
-          "fr_bump_jacfreq": make_fr_bump_jacfreq}
+          "fr_bump_jacfreq": make_fr_bump_jacfreq, "plate_bench_solve": make_plate_bench_solve}
 
 
 def main(names):

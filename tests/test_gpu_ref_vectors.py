@@ -186,6 +186,44 @@ def test_viscous_converged_field_matches_reference(precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision,jac_freq", [(0, 1), (1, 1), (1, 4)])
+def test_bench_plate_boundary_set_converges_to_a_reference_steady_state(precision, jac_freq):
+    """The bench plate's boundary set on a small channel (no incidence,
+    farfield in- and outflow, symmetry top and sides, isothermal floor), in
+    the plain form, the chosen form and the chosen form with the bench's
+    Jacobian refresh every fourth iteration, to an 8-order drop.  The
+    farfield state at the floor's outflow corner is degenerate there (any
+    phantom with zero normal velocity is kept, ucs/compressible.tcc:
+    1318-1321), so the steady state is not unique and is not compared with
+    the reference's field; instead the device's converged state (with its
+    phantoms) must be a steady state of the oracle's restatement of the
+    reference's equations: after the oracle's boundary update, residGlobal
+    <= 1e-8 of the reference's first residual."""
+    import oracle as O
+    from proteuscfd_amd import gpu
+    d = load("plate_bench_solve")
+    case = with_ref_ns(cases.plate_case(nx=12, ny=3, nz=10, nlayers=4, first_h=0.01, growth=1.3, twall=1.0,
+                                        incidence_deg=0.0, outflow_bc=cases.BC_FARFIELD,
+                                        top_bc=cases.BC_SYMMETRY), d, sorder=2, limiter=0, cfl=50.0, cfl_start=1.0,
+                       cfl_ramp=30, nsgs=8, precision=precision, jac_freq=jac_freq)
+    m = case.mesh
+    print(f"[ref] bench plate set precision={precision} jac_freq={jac_freq}", flush=True)
+    ctx = gpu.Context(case)
+    ctx.upload_q(case.initial_state())
+    ctx.reset_iteration()
+    hist, nit, _ = ctx.solve(2000, 1e-8)
+    q = ctx.download_q()
+    ctx.close()
+    assert hist[-1] <= 1e-8 * hist[0]
+    assert nit <= 1.2 * len(d["history"])
+    pb = O.Problem(m, case.bc_type, case.qref, case.params, twall=case.twall)
+    qb = pb.update_bcs(q)
+    _, nrm = pb.residual(qb, pb.gradient(qb), np.ones((m.nnode, 5)), it=nit)
+    print(f"[ref] iterations gpu {nit} reference {len(d['history'])}; oracle residual at the device's state "
+          f"{nrm[0]:.3e} (reference first residual {d['history'][0]:.3e})", flush=True)
+    assert nrm[0] <= 1e-8 * d["history"][0]
+
+
 def test_failure_paths_match_reference():
     """Negative pressure / negative density nodes (NaN Roe fluxes zeroed
     component-wise, ApplyDQ's density clip) and a NaN node (every correction

@@ -264,6 +264,48 @@ def test_viscous_steady_solve_matches_reference():
     s.close()
 
 
+def bench_plate():
+    """A small channel with the bench plate's boundary set: no incidence,
+    farfield inflow and outflow, symmetry top and sides, isothermal floor."""
+    return cases.plate_case(nx=12, ny=3, nz=10, nlayers=4, first_h=0.01, growth=1.3, twall=1.0, incidence_deg=0.0,
+                            outflow_bc=cases.BC_FARFIELD, top_bc=cases.BC_SYMMETRY)
+
+
+def test_bench_plate_boundary_set_is_degenerate_at_the_outflow_corner():
+    """The bench plate's boundary set (no incidence, farfield in- and
+    outflow, symmetry top and sides, isothermal floor) makes the farfield
+    state at the floor's outflow corner degenerate: there the wall node's
+    velocity is 0, the characteristic fixed point drives the phantom's
+    normal velocity to 0, and at theta == 0 exactly the reference keeps
+    whatever phantom it has (ucs/compressible.tcc:1318-1321).  So the
+    steady state is not unique, and which one a run reaches depends on the
+    round-off of its history.  Checked: the oracle's loop converges to the
+    8-order drop in about the reference's iteration count (159 vs 161), the
+    two converged fields differ (by percents near the outflow corner), and
+    the reference's converged state is a steady state of the oracle's
+    equations too (residual after the oracle's own boundary update below
+    the 8-order drop): both are solutions of the same discrete problem."""
+    d = load("plate_bench_solve")
+    case = bench_plate()
+    m = case.mesh
+    pb = ns_problem(case, d, sorder=2, limiter=0, cfl=50.0, cfl_start=1.0, cfl_ramp=30, nsgs=8)
+    s = O.Solver(pb, np.tile(d["qinf"], (m.nnode + m.nbedge, 1)))
+    href = d["history"]
+    hist = [s.step()[0]]
+    while hist[-1] > 1e-8 * hist[0] and len(hist) < 400:
+        hist.append(s.step()[0])
+    assert hist[-1] <= 1e-8 * hist[0]
+    assert abs(len(hist) - len(href)) <= 0.1 * len(href)
+    np.testing.assert_allclose(hist[:2], href[:2], rtol=1e-8)
+    qn, qr = s.q[:m.nnode, :5], d["q_final"][:m.nnode, :5]
+    diff = max(float(np.abs(qn[:, k] - qr[:, k]).max() / np.abs(qr[:, k]).max()) for k in range(5))
+    assert diff > 1e-3
+    s.close()
+    qb = pb.update_bcs(d["q_final"])
+    _, nrm = pb.residual(qb, pb.gradient(qb), np.ones((m.nnode, 5)), it=len(href))
+    assert nrm[0] <= 1e-8 * href[0]
+
+
 @pytest.mark.parametrize("tag", ["bump", "plate"])
 def test_reference_side_binding_descriptors(tag):
     """The binding a ucs.x maintainer adds (proteuscfd_amd/examples/
