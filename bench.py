@@ -547,8 +547,19 @@ def fr_record(args):
             h.append(f.step()[0])
         el = time.time() - t
         pt = f.phase_times()
+        info = f.info()
         r = {"s": el, "ms_per_step": 1e3 * el / args.fr_steps, "residual_ratio": h[-1] / h[0], "jac_freq": jfreq,
              "phase_ms_per_step": {k: v / args.fr_steps for k, v in pt.items()}}
+        # SGS against the HBM roofline: every pass reads its rows' blocks
+        # (81 values each) and moves b, the neighbours' x and the new x (27
+        # values per row); 2 nsgs C - (2 nsgs - 1) passes of the C colours
+        nc, nsgs, sz = info["ncolor"], 6, 4 if prec == gpu.CHOSEN_FP32 else 8
+        npass = 2 * nsgs * nc - (2 * nsgs - 1)
+        sgs_bytes = npass * (info["nblocks"] * 81 * sz + m.nnode * 27 * sz) / nc
+        sgs_gbs = sgs_bytes / (r["phase_ms_per_step"]["sgs"] * 1e-3) / 1e9
+        r["sgs_roofline"] = {"bound": "hbm", "passes_per_step": npass, "algorithmic_bytes_per_step": sgs_bytes,
+                             "achieved": sgs_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sgs_gbs / HBM_PEAK_GBS,
+                             "note": "the SGS phase time includes PrepareSGS (LU of the diagonal blocks)"}
         if sur:
             st = f.surrogate_stats()
             r["surrogate_fallback_share_last_step"] = st["last_fallback"] / m.nnode
