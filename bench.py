@@ -3,14 +3,17 @@ SGS, FP64 residual/gradients/norms) vs the plain FP64 form, on the 8M-cell
 laminar flat plate (the headline line: the case the chosen form's 2x target
 is quoted on), followed by the same two solves of the 1.03M-node
 second-order Euler bump as a "bump_euler" record (--no-extra skips it;
---case bump makes the bump the headline), on one MI355X.
+--case bump makes the bump the headline) and a finite-rate reacting Mach 8
+wedge record (--no-fr skips it; --only-fr runs it alone), on one MI355X.
 
 A "step" here is one complete steady solve from freestream until
 residGlobal <= 1e-8 * residGlobal(iteration 1); --steps K times K solves of
 each form, --warmup W runs W untimed implicit iterations first.  Prints one
 JSON line (rank 0).  Per-kernel device times come from HIP events on the
-context stream (pcfd_phase_times); the CPU baseline times the oracle on the
-host on a bounded sample (see cpu_baseline.sample).
+context stream (pcfd_phase_times); the CPU baseline times the reference
+solver itself (its checker build under oracle/_ref; the oracle where that
+is absent) on the host's cores on a bounded sample (see
+cpu_baseline.sample).
 """
 from __future__ import annotations
 
@@ -146,34 +149,74 @@ def cpu_iteration(case):
     return it_s, {k: t1[k] - t0[k] for k in t1}
 
 
+def reference_keys(kind, base):
+    """The workload as the reference's .param keywords (ucs/param.tcc) and
+    .bc surfaces: BC types per face tag, and the noSlip wall temperature."""
+    k = {"timeOrder": 0, "reorderMesh": 0, "refPressure": 101325, "flowDirection": "[1.0, 0.0, 0.0]",
+         "spatialOrder": base["sorder"], "limiter": base["limiter"], "numberSGS": base["nsgs"], "CFL": base["cfl"],
+         "startingCFL": base["cfl_start"], "rampCFL": base["cfl_ramp"]}
+    if kind == "plate":
+        # refViscosity gives Re ~ 1e4 per unit length at Mach 0.3 and 300 K
+        k.update(equationSet="compressibleNS", initialTemperature=300, refViscosity=0.01225, velocity=0.3)
+        return k, {1: 6, 2: 6, 3: 9, 4: 9, 5: 4, 6: 9}, {5: " twall = [300.0]"}
+    k.update(equationSet="compressibleEuler", velocity=0.5)
+    return k, {1: 6, 2: 6, 3: 6, 4: 6, 5: 3, 6: 6}, {}
+
+
 def cpu_worker(args):
     """--cpu-worker: one rank's share of the CPU baseline (a partition-sized
-    mesh of the workload), printed as one JSON line."""
-    case, _, _ = make_case(args.case, args, parts=args.cpu_parts)
+    mesh of the workload), printed as one JSON line: the reference solver
+    itself (oracle/_ref/ref_harness, iterations of its own steady loop timed
+    on the wall clock) or the oracle."""
+    case, base, _ = make_case(args.case, args, parts=args.cpu_parts)
+    if args.cpu_kind == "reference":
+        from oracle import ref_case
+        keys, bc, extra = reference_keys(args.case, base)
+        d = ref_case.write_case(case.mesh, keys, bc, nsteps=args.cpu_iters, extra=extra)
+        ref_case.run(d, "c", "time", args.cpu_iters)
+        secs = ref_case.read(d, "iter_seconds")
+        print(json.dumps({"s": float(secs[-1]), "iter_s": [float(x) for x in secs], "nnode": case.mesh.nnode}))
+        return
     it_s, tm = cpu_iteration(case)
     print(json.dumps({"s": it_s, "phases": tm, "nnode": case.mesh.nnode}))
 
 
 def cpu_baseline(args, n_iter_plain):
     """The reference's CPU layout: one MPI rank per core, each iterating its
-    own partition.  Estimated by running one oracle process per core at once
-    (so they share the host's memory bandwidth), each on a mesh of the
-    workload cut to 1/cores of its size in the two directions along the
-    wall, and timing one iteration; the slowest rank sets the iteration
-    time, times the plain GPU path's iteration count.  Halo exchanges are
-    not modelled."""
+    own partition.  Estimated by running one process per core at once (so
+    they share the host's memory bandwidth), each on a mesh of the workload
+    cut to 1/cores of its size in the two directions along the wall: the
+    reference solver itself where its checker build (oracle/_ref) is present
+    (the last of a few iterations of its own loop timed), else the oracle
+    (one iteration); the slowest rank sets the iteration time, times the
+    plain GPU path's iteration count.  Halo exchanges are not modelled."""
     import subprocess
     cores = max(1, min(16, os.cpu_count() or 1))
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--case", args.case, "--n", str(args.n),
-           "--plate", args.plate, "--cpu-parts", str(cores)]
+
+    def run_workers(kind):
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", "--case", args.case, "--n", str(args.n),
+               "--plate", args.plate, "--cpu-parts", str(cores), "--cpu-kind", kind]
+        procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True)
+                 for _ in range(cores)]
+        outs = []
+        for pr in procs:
+            o, _ = pr.communicate()
+            if pr.returncode != 0:
+                raise RuntimeError(f"cpu baseline worker ({kind}) failed")
+            outs.append(json.loads(o.strip().splitlines()[-1]))
+        return outs
+
     t0 = time.time()
-    procs = [subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True) for _ in range(cores)]
-    outs = []
-    for pr in procs:
-        o, _ = pr.communicate()
-        if pr.returncode != 0:
-            raise RuntimeError("cpu baseline worker failed")
-        outs.append(json.loads(o.strip().splitlines()[-1]))
+    kind = "reference"
+    try:
+        from oracle import ref_case
+        if not os.path.exists(ref_case.harness_path()):
+            raise RuntimeError("the reference harness is not built")
+        outs = run_workers("reference")
+    except Exception as e:  # the reference checker is optional: time the oracle instead
+        log(f"[bench] reference CPU baseline unavailable ({e}); timing the oracle")
+        kind = "port"
+        outs = run_workers("port")
     wall = time.time() - t0
     worst = max(outs, key=lambda r: r["s"])
     model = ""
@@ -184,16 +227,23 @@ def cpu_baseline(args, n_iter_plain):
                 break
     except OSError:
         pass
-    ph = worst["phases"]
+    if kind == "reference":
+        sample = (f"{cores} concurrent processes of the reference solver itself (ProteusCFD ucs, built from its own "
+                  f"sources as oracle/_ref/ref_harness; one per core, as MPI ranks), each running "
+                  f"{args.cpu_iters} iterations of its own steady loop on a {worst['nnode']}-node partition-sized "
+                  f"mesh of the workload; the last iteration (second order) timed: slowest rank "
+                  f"{worst['s']:.2f} s (its iterations {', '.join(f'{x:.2f}' for x in worst['iter_s'])} s)")
+    else:
+        ph = worst["phases"]
+        sample = (f"{cores} concurrent oracle processes (one per core, as MPI ranks), each one implicit iteration "
+                  f"on a {worst['nnode']}-node partition-sized mesh of the workload: slowest {worst['s']:.2f} s "
+                  f"(jacobian {ph['jacobian']:.2f} s, residual {ph['residual']:.2f} s, gradient+limiter "
+                  f"{ph['gradient']:.2f} s, SGS {ph['linear']:.2f} s)")
     return {
         "value": worst["s"] * n_iter_plain,
         "unit": "s (estimated: slowest rank's s/iteration x the plain GPU path's iteration count)",
-        "cores": cores, "kind": "port",
-        "sample": (f"{cores} concurrent oracle processes (one per core, as MPI ranks), each one implicit iteration "
-                   f"on a {worst['nnode']}-node partition-sized mesh of the workload: slowest {worst['s']:.2f} s "
-                   f"(jacobian {ph['jacobian']:.2f} s, residual {ph['residual']:.2f} s, gradient+limiter "
-                   f"{ph['gradient']:.2f} s, SGS {ph['linear']:.2f} s); {wall:.0f} s wall incl. mesh set-up; "
-                   f"host {model}; halo exchange not modelled"),
+        "cores": cores, "kind": kind,
+        "sample": sample + f"; {wall:.0f} s wall incl. mesh set-up; host {model}; halo exchange not modelled",
         "s_per_iteration": worst["s"],
         "nproc_host": os.cpu_count(),
     }
@@ -324,6 +374,8 @@ def main():
                     help="skip the record of the other workload that follows the headline")
     ap.add_argument("--cpu-worker", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-parts", type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-kind", default="reference", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-iters", type=int, default=2, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.cpu_worker:

@@ -20,10 +20,13 @@
 //        final state.
 //   mode "trace" <n>: the same loop for n iterations, writing the last
 //        iteration's Jacobian, time step, residual, correction and state.
+//   mode "time" <n>: n iterations of that loop, each timed on the wall clock
+//        (ref_iter_seconds.bin): the CPU baseline of bench.py.
 //   mode "chemdb": writes the species database file <dir>/<casename> from
 //        <dir>/chemdb.txt (see mode_chemdb), for the finite-rate eqnset.
 // Arrays are written raw (little-endian float64 or int32) to
 // <dir>/ref_<name>.bin; scalars go to <dir>/ref_meta.txt.
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -279,6 +282,24 @@ int mode_solve(SolutionSpace<Real>* s, Int maxiter, Real drop) {
   return 0;
 }
 
+// time <n>: n iterations of the reference's own steady loop, each timed on
+// the wall clock (PreTimeAdvance + NewtonIterate + PostTimeAdvance): the CPU
+// baseline's per-iteration cost of the reference solver itself.
+int mode_time(SolutionSpace<Real>* s, Int n) {
+  std::vector<Real> secs;
+  s->PreIterate();
+  for (Int it = 1; it <= n; ++it) {
+    const auto t0 = std::chrono::steady_clock::now();
+    s->PreTimeAdvance();
+    s->NewtonIterate();
+    s->PostTimeAdvance();
+    secs.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+  }
+  meta("iterations", secs.size());
+  dump("iter_seconds", secs.data(), secs.size());
+  return 0;
+}
+
 // trace <n>: the steady loop for n iterations; the last one's Jacobian and
 // time step (after PreTimeAdvance) and residual, correction and state (after
 // NewtonIterate) are written.
@@ -386,6 +407,7 @@ int main(int argc, char** argv) {
   else if (mode == "describe") rc = mode_describe(s);
   else if (mode == "solve" && argc >= 6) rc = mode_solve(s, std::atoi(argv[4]), std::atof(argv[5]));
   else if (mode == "trace" && argc >= 5) rc = mode_trace(s, std::atoi(argv[4]));
+  else if (mode == "time" && argc >= 5) rc = mode_time(s, std::atoi(argv[4]));
   else std::cerr << "unknown mode or missing arguments" << std::endl;
   g_meta.close();
   MPI_Finalize();

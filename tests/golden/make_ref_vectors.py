@@ -15,7 +15,6 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
-import tempfile
 
 import numpy as np
 
@@ -23,22 +22,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from oracle import build_ref  # noqa: E402
+from oracle import build_ref, ref_case  # noqa: E402
 from proteuscfd_amd import cases  # noqa: E402
 
 # BCTypes names as the reference's .bc parser knows them (ucs/bc_defines.h:31-53)
-BC_NAMES = {1: "dirichlet", 2: "neumann", 3: "impermeableWall", 4: "noSlip", 6: "farField", 7: "sonicInflow",
-            8: "sonicOutflow", 9: "symmetry"}
-
 MESH_KEYS = ["edges", "edge_a", "bedges", "bedge_a", "bfactag", "xyz", "vol"]
-
-
-def param_text(name, keys, nsteps=1):
-    lines = [f"{k} = {v}" for k, v in keys.items()]
-    return (f"<<<BEGIN TEMPORAL CONTROL>>>\nnumTimeSteps = {nsteps}\nnewtonIterations = 1\n"
-            f"<<<END TEMPORAL CONTROL>>>\n\n<<<BEGIN SOLUTION ORDERING>>>\nIterate {name}\n"
-            f"<<<END SOLUTION ORDERING>>>\n\n<<<BEGIN SPACE {name}>>>\n" + "\n".join(lines) +
-            "\n<<<END SPACE>>>\n")
 
 
 def euler_keys(**kw):
@@ -53,13 +41,7 @@ class Run:
 
     def __init__(self, case, keys, bc, nsteps=1, name="c", extra=None):
         self.case, self.name = case, name
-        self.dir = tempfile.mkdtemp(prefix="pcfd_ref_") + "/"
-        case.mesh.write_h5(self.dir + name, 1)
-        with open(self.dir + name + ".param", "w") as f:
-            f.write(param_text(name, keys, nsteps))
-        with open(self.dir + name + ".bc", "w") as f:
-            for tag, typ in bc.items():
-                f.write(f"surface #{tag} = {BC_NAMES[typ]}{(extra or {}).get(tag, '')}\n")
+        self.dir = ref_case.write_case(case.mesh, keys, bc, nsteps=nsteps, name=name, extra=extra)
 
     def __call__(self, mode, *args):
         exe = build_ref.harness_path()
