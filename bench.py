@@ -460,7 +460,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        log("[bench] cpu baseline: one oracle process per core, each on a partition-sized mesh ...")
+        log("[bench] cpu baseline: one reference (or oracle) process per core, each on a partition-sized mesh ...")
         cpu = cpu_baseline(args, int(np.mean(plain["iters"])))
         log(f"[bench] cpu: {cpu['s_per_iteration']:.2f} s/iteration on {cpu['cores']} cores")
 
