@@ -102,6 +102,11 @@ typedef struct {
                             ucs/jacobian.tcc:538-640, with jacobianBoundaryEval 0: the +h evaluation
                             re-evaluates the boundary state, the -h one keeps the unperturbed one);
                             2 (complex step) gives PCFD_ERR_UNSUPPORTED */
+  int rk_stages;         /* explicit path (nsgs 0): 0 or 1 -- the reference's single-stage ExplicitSolve
+                            (ucs/solve.tcc:70-140, with NewtonIterate's second application of the
+                            correction); 4 -- Jameson's four stages, Q_k = Q_0 + a_k dt/vol b(Q_{k-1}) with
+                            a = 1/4, 1/3, 1/2, 1, boundary states (and gradient, limiter) refreshed per
+                            stage, dt of the step (no counterpart in the reference) */
 } pcfd_params;
 
 #define PCFD_FLUX_ROE 0

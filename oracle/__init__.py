@@ -34,7 +34,7 @@ class OParams(C.Structure):
                 ("prt", C.c_double), ("sutherland_s", C.c_double), ("gradient_type", C.c_int),
                 ("global_dt", C.c_int), ("fixed_dt", C.c_double), ("enable_vnn", C.c_int), ("vnn", C.c_double),
                 ("dynamic_cfl", C.c_int), ("jac_field_type", C.c_int), ("flux_type", C.c_int),
-                ("jac_boundary_type", C.c_int)]
+                ("jac_boundary_type", C.c_int), ("rk_stages", C.c_int)]
 
 
 class OChem(C.Structure):
@@ -156,7 +156,7 @@ class Problem:
                          int(params.viscous), params.re_tilde, params.prandtl, params.prandtl_t,
                          params.sutherland_s, params.gradient_type, params.global_dt, params.fixed_dt,
                          params.enable_vnn, params.vnn, params.dynamic_cfl, params.jac_field_type,
-                         params.flux_type, params.jac_boundary_type)
+                         params.flux_type, params.jac_boundary_type, getattr(params, "rk_stages", 1))
 
     @property
     def bnormal(self):

@@ -1441,6 +1441,53 @@ double o_solver_step(o_solver* s, double* norms, int* nzeroed) {
   s->res = nrm[0];
   double t4 = now_s();
   s->t_res += t4 - t3;
+  if (p->nsgs == 0 && p->rk_stages == 4) {
+    /* Four-stage Runge-Kutta with Jameson's coefficients (an extension: the
+     * reference's explicit path is the single stage below): stage k sets
+     * Q = Q0 + a_k dt/vol b(Q_{k-1}) through ApplyDQ, rows with a NaN/Inf
+     * correction keep Q0 (counted); boundary states, and at second order the
+     * gradient and limiter, are refreshed before each later stage's
+     * residual; dt is the step's.  The residual norms are the first stage's. */
+    static const double a[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
+    const size_t nq = (size_t)NV * (size_t)(nn + m->nbnode);
+    double* q0 = (double*)malloc(sizeof(double) * nq);
+    memcpy(q0, s->q, sizeof(double) * nq);
+    int zc = 0;
+    for (int k = 0; k < 4; k++) {
+      if (k > 0) {
+        o_update_bcs(m, p, s->q);
+        if ((p->sorder > 1 && s->iter > p->nfirst) || p->viscous) {
+          if (p->gradient_type == 1) o_gradient_gg(m, s->q, s->grad);
+          else o_gradient(m, s->q, s->sw, s->grad);
+        }
+        if ((s->iter % lr == 0) && p->sorder > 1 && p->limiter && s->iter > p->nfirst &&
+            s->iter < p->limiter_freeze)
+          o_limiter(m, p, s->q, s->grad, s->l);
+        double tmp[6];
+        o_residual(m, p, s->iter, s->q, s->grad, s->l, s->b, tmp);
+      }
+      for (int i = 0; i < nn; i++) {
+        double x[NE];
+        int z = 0;
+        for (int j = 0; j < NE; j++) {
+          x[j] = a[k] * (s->b[NE * i + j] * s->dt[i] / m->vol[i]);
+          if (isnan(x[j]) || isinf(x[j])) z = 1;
+        }
+        if (z) {
+          zc++;
+          for (int j = 0; j < NE; j++) x[j] = 0.0;
+        }
+        memcpy(&s->q[NV * i], &q0[NV * i], sizeof(double) * NV);
+        o_apply_dq(x, &s->q[NV * i], p->gamma);
+      }
+    }
+    free(q0);
+    s->t_lin += now_s() - t4;
+    if (nzeroed) *nzeroed = zc;
+    if (norms) memcpy(norms, nrm, sizeof(nrm));
+    s->iter++;
+    return nrm[0];
+  }
   if (p->nsgs > 0) {
     o_prepare_sgs(&s->A);
     memset(s->x, 0, sizeof(double) * NE * (size_t)nn);

@@ -70,6 +70,7 @@ typedef struct {
   int jac_field_type;           /* jacobianFieldType: 0 one-sided FD, 1 centred FD */
   int flux_type;                /* 0 Roe, 1 algebraic */
   int jac_boundary_type;        /* jacobianBoundaryType: 0 one-sided FD, 1 centred FD */
+  int rk_stages;                /* explicit path: 0/1 the reference's single stage, 4 Jameson's four stages */
 } o_params;
 
 typedef struct {

@@ -89,6 +89,7 @@ class Params:
     jac_field_type: int = 0       # jacobianFieldType: 0 one-sided, 1 centred
     flux_type: int = 0            # 0 Roe (roeFlux), 1 algebraic (algebraicFlux)
     jac_boundary_type: int = 0    # jacobianBoundaryType: 0 one-sided, 1 centred
+    rk_stages: int = 1            # explicit path: 1 the reference's single stage, 4 Jameson's four stages
 
     @property
     def re_tilde(self) -> float:  # Re / Mach (ucs/compressible.tcc:753-755)

@@ -72,6 +72,7 @@ struct Dev {
   double* dt = nullptr;     // nnode
   double* dt_next = nullptr;  // nnode: the time step being computed while the Jacobian rows read dt
   double* qjac = nullptr;   // 5 x ntot: states the Jacobian rows read when hard-set nodes are committed early
+  double* qrk = nullptr;    // 5 x ntot: the state a four-stage Runge-Kutta step starts from (allocated on first use)
   // SELL block matrix
   int* srow = nullptr;      // nrow_pad: node of each padded row (-1 = padding)
   int* swidth = nullptr;    // nchunk_s
@@ -215,5 +216,7 @@ void jacobian_join(Ctx& c);
 void launch_prepare_sgs(Ctx& c);
 void launch_sgs(Ctx& c, int nsgs, double* delta_xnorm);
 void launch_apply_dq(Ctx& c, int explicit_mode, int* nzeroed);
+void launch_zero_count(Ctx& c);                          // the zeroed-row counter
+void launch_rk_stage(Ctx& c, double a, const double* q0);  // q = q0 + a dt/vol b (ApplyDQ)
 
 }  // namespace pcfd

@@ -40,7 +40,7 @@ void dfree(T*& p) {
 
 void free_all(Ctx& c) {
   Dev& d = c.d;
-  dfree(d.xyz); dfree(d.vol); dfree(d.vol23); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm);
+  dfree(d.xyz); dfree(d.vol); dfree(d.vol23); dfree(d.sw); dfree(d.gwidth); dfree(d.gbase); dfree(d.gcol); dfree(d.gnrm); dfree(d.qrk);
   dfree(d.he_ptr); dfree(d.he_id); dfree(d.he_tag); dfree(d.he_a); dfree(d.bnd_nodes); dfree(d.bidx);
   dfree(d.jdb); dfree(d.jqb); dfree(d.bc_type); dfree(d.twall); dfree(d.he_nrm); dfree(d.spos);
   dfree(d.qref); dfree(d.q); dfree(d.grad); dfree(d.lim); dfree(d.lim2); dfree(d.edges); dfree(d.clip_flag); dfree(d.clip_cnt);
@@ -458,9 +458,11 @@ int pcfd_set_params(pcfd_ctx* ctx, const pcfd_params* p) {
         (p->enable_vnn != 0 && p->enable_vnn != 1) || (p->enable_vnn && !(p->vnn > 0.0)) ||
         (p->dynamic_cfl != 0 && p->dynamic_cfl != 1) || (p->jac_field_type != 0 && p->jac_field_type != 1) ||
         (p->jac_boundary_type != 0 && p->jac_boundary_type != 1) ||
-        (p->flux_type != PCFD_FLUX_ROE && p->flux_type != PCFD_FLUX_ALGEBRAIC)) {
+        (p->flux_type != PCFD_FLUX_ROE && p->flux_type != PCFD_FLUX_ALGEBRAIC) ||
+        (p->rk_stages != 0 && p->rk_stages != 1 && p->rk_stages != 4)) {
       c.err = "pcfd_set_params: option out of range (gradient_type, jac_field_type, jac_boundary_type, global_dt, "
-              "enable_vnn, dynamic_cfl 0/1; flux_type PCFD_FLUX_ROE/PCFD_FLUX_ALGEBRAIC; vnn > 0 when enabled)";
+              "enable_vnn, dynamic_cfl 0/1; flux_type PCFD_FLUX_ROE/PCFD_FLUX_ALGEBRAIC; vnn > 0 when enabled; "
+              "rk_stages 0, 1 or 4)";
       return PCFD_ERR_ARG;
     }
     c.prm = *p;

@@ -64,6 +64,45 @@ __global__ void __launch_bounds__(kBlock) k_explicit(int nnode, int ntot, double
   for (int k = 0; k < 5; ++k) q[k * ntot + i] = Q[k];
 }
 
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+// One stage of the four-stage Runge-Kutta extension: Q = Q0 + a dt/vol b
+// through ApplyDQ; a row whose correction is NaN/Inf keeps Q0 (counted).
+__global__ void __launch_bounds__(kBlock) k_rk_stage(int nnode, int ntot, double gamma, double a,
+                                                    const double* __restrict__ b, const double* __restrict__ dt,
+                                                    const double* __restrict__ vol, const double* __restrict__ q0,
+                                                    double* __restrict__ q, int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnode) return;
+  double dq[5], Q[5];
+  bool zero = false;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    dq[k] = a * (b[k * nnode + i] * dt[i] / vol[i]);
+    if (__builtin_isnan(dq[k]) || __builtin_isinf(dq[k])) zero = true;
+    Q[k] = q0[k * ntot + i];
+  }
+  if (zero) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dq[k] = 0.0;
+    atomicAdd(count, 1);
+  }
+  apply_dq(dq, Q, gamma);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) q[k * ntot + i] = Q[k];
+}
+
+void launch_zero_count(Ctx& c) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c.stream, c.d.icount);
+  PCFD_HIP(hipGetLastError());
+}
+
+void launch_rk_stage(Ctx& c, double a, const double* q0) {
+  hipLaunchKernelGGL(k_rk_stage, dim3(nblk(c.nnode)), dim3(kBlock), 0, c.stream, c.nnode, c.ntot, c.prm.gamma, a,
+                     c.d.b, c.d.dt, c.d.vol, q0, c.d.q, c.d.icount);
+  PCFD_HIP(hipGetLastError());
+}
+
 // Exclusive prefix sum of per-pass counts (one workgroup of 1024, fixed
 // order): off[p] = sum of cnt[0..p-1], off[npass] = total.  Used to compact
 // lists in order without atomics (the surrogate's fallback cells, the
@@ -116,7 +155,6 @@ void launch_scan_counts(hipStream_t stream, int n, const int* cnt, int* off) {
   PCFD_HIP(hipGetLastError());
 }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
 
 void launch_apply_dq(Ctx& c, int explicit_mode, int* nzeroed) {
   hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c.stream, c.d.icount);

@@ -116,6 +116,29 @@ void iterate(Ctx& c, double* resid) {
     phase_begin(c);
     launch_apply_dq(c, 0, nullptr);
     phase_end(c, 8);
+  } else if (p.rk_stages == 4) {
+    // four-stage Runge-Kutta (Jameson's coefficients): the first stage uses
+    // the residual above; the state it starts from is kept in qrk
+    phase_begin(c);
+    if (!c.d.qrk) PCFD_HIP(hipMalloc((void**)&c.d.qrk, sizeof(double) * 5 * (size_t)c.ntot));
+    PCFD_HIP(hipMemcpyAsync(c.d.qrk, c.d.q, sizeof(double) * 5 * (size_t)c.ntot, hipMemcpyDeviceToDevice,
+                            c.stream));
+    launch_zero_count(c);
+    static const double a[4] = {0.25, 1.0 / 3.0, 0.5, 1.0};
+    for (int k = 0; k < 4; ++k) {
+      if (k > 0) {
+        launch_update_bcs(c);
+        if (grad_now && lim_now && p.gradient_type == 0) {
+          launch_gradient_limiter(c, p.gradient_terms);
+        } else {
+          if (grad_now) launch_gradient(c, p.gradient_terms);
+          if (lim_now) launch_limiter(c);
+        }
+        launch_residual(c, ho ? 1 : 0);
+      }
+      launch_rk_stage(c, a[k], c.d.qrk);
+    }
+    phase_end(c, 8);
   } else {
     phase_begin(c);
     launch_apply_dq(c, 1, nullptr);

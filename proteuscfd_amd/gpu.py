@@ -39,7 +39,8 @@ class CParams(C.Structure):
                 ("prandtl", C.c_double), ("prandtl_t", C.c_double), ("sutherland_s", C.c_double),
                 ("gradient_type", C.c_int), ("global_dt", C.c_int), ("fixed_dt", C.c_double),
                 ("enable_vnn", C.c_int), ("vnn", C.c_double), ("dynamic_cfl", C.c_int),
-                ("jac_field_type", C.c_int), ("flux_type", C.c_int), ("jac_boundary_type", C.c_int)]
+                ("jac_field_type", C.c_int), ("flux_type", C.c_int), ("jac_boundary_type", C.c_int),
+                ("rk_stages", C.c_int)]
 
 
 SYMBOLS = [
@@ -121,7 +122,7 @@ def to_cparams(p: Params, gradient_terms: int = 9) -> CParams:
                    p.cfl, p.cfl_start, p.cfl_ramp, p.nsgs, p.symmetry2d, p.jac_freq, p.precision,
                    9 if p.viscous else gradient_terms, int(p.viscous), p.re_tilde, p.prandtl, p.prandtl_t,
                    p.sutherland_s, p.gradient_type, p.global_dt, p.fixed_dt, p.enable_vnn, p.vnn, p.dynamic_cfl,
-                   p.jac_field_type, p.flux_type, p.jac_boundary_type)
+                   p.jac_field_type, p.flux_type, p.jac_boundary_type, p.rk_stages)
 
 
 class Context:

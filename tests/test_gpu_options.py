@@ -159,8 +159,36 @@ def test_unbuilt_and_out_of_range_options_rejected(setup):
     name, case, ctx, q0 = setup
     for kw, code in [(dict(flux_type=2), "(-4)"), (dict(jac_field_type=2), "(-4)"), (dict(jac_boundary_type=2), "(-4)"),
                      (dict(jac_boundary_type=3), "(-1)"), (dict(gradient_type=2), "(-1)"),
-                     (dict(enable_vnn=1, vnn=0.0), "(-1)"), (dict(dynamic_cfl=3), "(-1)")]:
+                     (dict(enable_vnn=1, vnn=0.0), "(-1)"), (dict(dynamic_cfl=3), "(-1)"),
+                     (dict(rk_stages=3), "(-1)")]:
         with pytest.raises(RuntimeError) as e:
             ctx.set_params(with_(case.params, **kw))
         assert code in str(e.value), (kw, str(e.value))
     ctx.set_params(case.params)
+
+
+@pytest.mark.parametrize("sorder", [1, 2])
+def test_four_stage_runge_kutta(sorder):
+    """The explicit four-stage Runge-Kutta extension (rk_stages 4; the
+    configuration of the supersonic wedge channel: Mach 2.5, local time
+    step at CFL 0.9) against the oracle's, first order and second order with
+    the Venkatakrishnan limiter: 10 steps (40 residual evaluations, each
+    matching the oracle's to 1e-12), residual history to 1e-8 and the states
+    to 1e-9 at first order, 1e-8 at second (the limiter amplifies the
+    residuals' last-bit differences over the stages)."""
+    from proteuscfd_amd import gpu
+    case = cases.wedge_case(params=Params(sorder=sorder, limiter=2, cfl=0.9, cfl_start=0.9, nsgs=0, rk_stages=4))
+    print(f"[options] wedge rk4 sorder={sorder}", flush=True)
+    ctx = gpu.Context(case)
+    q = case.initial_state()
+    ctx.upload_q(q)
+    ctx.reset_iteration()
+    so = O.Solver(problem(case, case.params), q)
+    for _ in range(10):
+        rg = ctx.iterate()
+        ro, _, _ = so.step()
+        assert abs(rg[0] - ro) <= 1e-8 * ro
+    qg = ctx.download_q()
+    assert rel(qg[:, :5], so.q[:, :5]) < (1e-9 if sorder == 1 else 1e-8)
+    so.close()
+    ctx.close()

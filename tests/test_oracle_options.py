@@ -379,3 +379,55 @@ def test_centred_boundary_jacobian_is_half_reevaluated_half_frozen(kind, flux):
         np.testing.assert_allclose(Dc[rows], want[rows], atol=2e-5 * np.abs(want).max(), err_msg=f"node {i}")
         checked += np.abs(Dc - Do).max() > 1e-3 * np.abs(Do).max()
     assert checked > 0
+
+
+def test_four_stage_runge_kutta_is_the_rk4_polynomial():
+    """The explicit path's four-stage Runge-Kutta extension (rk_stages 4,
+    Jameson's coefficients 1/4, 1/3, 1/2, 1).  Near a steady state the
+    scaled residual g(e) = dt/vol * b(Q_s + e) is affine, g(e) = g0 + G e,
+    and such a scheme advances e by exactly the fourth-order Taylor
+    polynomial: e4 = e0 + g0 + G g0 / 2 + G^2 g0 / 6 + G^3 g0 / 24.  The
+    oracle's step from a 1e-6 perturbation of a converged bump, at the local
+    CFL 0.9 time step, matches it to 1e-5 of the change (a wrong coefficient
+    or stage order moves the G^2 and G^3 terms by percents of it)."""
+    case = cases.bump_case(n=4, rcm=False)
+    m = case.mesh
+    base = dict(sorder=1, limiter=0, cfl=0.9, cfl_start=0.9, cfl_ramp=0)
+    imp = O.Problem(m, case.bc_type, case.qref, Params(sorder=1, limiter=0, cfl=50.0, cfl_start=1.0, cfl_ramp=10,
+                                                       nsgs=6))
+    s = O.Solver(imp, case.initial_state())
+    r0 = s.step()[0]
+    for _ in range(300):
+        if s.step()[0] <= 1e-13 * r0:
+            break
+    qs = s.q
+    s.close()
+    nn = m.nnode
+    rng = np.random.default_rng(7)
+    q0 = qs.copy()
+    q0[:nn, :5] += 1e-6 * rng.standard_normal((nn, 5)) * np.abs(qs[:nn, :5]).max(axis=0)
+    q0 = cases.aux(q0, 1.4)
+    pb = O.Problem(m, case.bc_type, case.qref, Params(**base, nsgs=0, rk_stages=4))
+    dt, _ = pb.timestep(q0, 0.9)
+    rk = O.Solver(pb, q0)
+    rk.step()
+    e4 = rk.q[:nn, :5] - qs[:nn, :5]
+    rk.close()
+    D = (dt / np.asarray(m.vol))[:, None]
+
+    def g(e):
+        q = qs.copy()
+        q[:nn, :5] += e
+        q = pb.update_bcs(cases.aux(q, 1.4))
+        return D * pb.residual(q)[0]
+
+    e0 = q0[:nn, :5] - qs[:nn, :5]
+    g0 = g(e0)
+    Gv = lambda v: g(e0 + v) - g0
+    t1 = Gv(g0)
+    t2 = Gv(t1)
+    t3 = Gv(t2)
+    pred = e0 + g0 + t1 / 2 + t2 / 6 + t3 / 24
+    change = np.abs(pred - e0).max()
+    assert np.abs(t2).max() > 1e-3 * change  # the higher terms are resolved
+    assert np.abs(e4 - pred).max() <= 1e-5 * change
