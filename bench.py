@@ -42,6 +42,9 @@ def algorithmic_bytes(m, nblocks, prec_bytes):
         # compulsory bytes per launch (indices int32, state FP64)
         "residual_2nd": 264 * nn + 40 * ne + 84 * nb,
         "residual_1st": 80 * nn + 40 * ne + 84 * nb,
+        # the viscous pass that continues the residual's node sums: q, the
+        # T, u, v, w gradient terms, xyz and b read, b written per node
+        "viscous": 240 * nn + 40 * ne + 84 * nb,
         "sgs_sweep": nblocks * 25 * prec_bytes + 8 * 15 * nn,
         "jacobian": nblocks * 25 * prec_bytes + 80 * nn + 40 * ne,
         "gradient9": nn * 8 * (10 + 6 + 3 + 27) + 8 * ne,
@@ -428,7 +431,10 @@ def main():
     # kernel-only times (HIP events on the library stream around the launches)
     res_key = "residual_2nd" if base["sorder"] > 1 else "residual_1st"
     kern = {
-        "residual_fp64": dict(ms=pp["k_residual"], gbs=gbs(ab64[res_key], pp["k_residual"])),
+        # k_residual's window covers the viscous pass too when the case is viscous
+        "residual_fp64": dict(ms=pp["k_residual"],
+                              gbs=gbs(ab64[res_key] + (ab64["viscous"] if base.get("viscous") else 0),
+                                      pp["k_residual"])),
         # per symmetric sweep (2 per SGS iteration); bytes of the passes actually run
         "sgs_sweep_fp64": dict(ms=pp["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs64, pp["k_sgs_sweeps"])),
         "sgs_sweep_fp32": dict(ms=pc["k_sgs_sweeps"] / sweeps, gbs=gbs(sgs32, pc["k_sgs_sweeps"])),
