@@ -14,8 +14,8 @@
 //                       block of row r slot s at ((sbase[chunk]+s)*NE2 + k)*64
 //                       + lane, k = 0..NE2-1 (row-major entry); diagonal at
 //                       (chunk*NE2 + k)*64 + lane; a row's slots follow its
-//                       CSR (edge) order.  eoff[2e], eoff[2e+1]: where edge
-//                       e's blocks (n0,n1) and (n1,n0) start.
+//                       CSR (edge) order.  h_eoff[2e], h_eoff[2e+1] (host):
+//                       where edge e's blocks (n0,n1) and (n1,n0) start.
 //
 // Every flux and residual sum runs in the reference's order (edges in edge
 // order, then half-edges, then the source), as node gathers with no atomics,
@@ -782,7 +782,7 @@ struct pcfd_fr {
   int *edges = nullptr, *nptr = nullptr, *nslot_d = nullptr, *he_ptr = nullptr, *he_id = nullptr, *he_tag = nullptr,
       *bc_type = nullptr, *bnd = nullptr, *bidx = nullptr, *srow = nullptr, *swidth = nullptr, *sbase = nullptr,
       *scol = nullptr, *icount = nullptr;
-  long long *eoff = nullptr, *piv = nullptr;
+  long long* piv = nullptr;
   double *ediag = nullptr, *nterm = nullptr;  // Jacobian diagonal parts: per edge side, per node
   double *Md = nullptr, *Dd = nullptr, *xd = nullptr;
   float *Mf = nullptr, *Df = nullptr, *xf = nullptr;
@@ -820,7 +820,7 @@ void free_all(pcfd_fr& f) {
   dfree(f.q); dfree(f.qref); dfree(f.vol); dfree(f.edge_a); dfree(f.he_a); dfree(f.eflux); dfree(f.elam);
   dfree(f.b); dfree(f.dt); dfree(f.jdb); dfree(f.red); dfree(f.aux); dfree(f.edges); dfree(f.nptr);
   dfree(f.nslot_d); dfree(f.he_ptr); dfree(f.he_id); dfree(f.he_tag); dfree(f.bc_type); dfree(f.bnd);
-  dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount); dfree(f.eoff);
+  dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount);
   dfree(f.piv); dfree(f.Md); dfree(f.Dd); dfree(f.xd); dfree(f.Mf); dfree(f.Df); dfree(f.xf); dfree(f.dmodel);
   dfree(f.s_rhoi); dfree(f.s_T); dfree(f.s_w); dfree(f.s_lst); dfree(f.s_cnt); dfree(f.s_off);
   dfree(f.ediag); dfree(f.nterm);
@@ -1296,7 +1296,6 @@ int pcfd_fr_create(const pcfd_mesh_desc* m, pcfd_chem* chem, const double* qref,
     dput(*f, &f->swidth, swidth);
     dput(*f, &f->sbase, sbase);
     dput(*f, &f->scol, scol);
-    dput(*f, &f->eoff, eoff);
     dalloc(*f, &f->q, (size_t)f->ntot * NE);
     dalloc(*f, &f->eflux, (size_t)ne * NE);
     dalloc(*f, &f->elam, (size_t)ne);
