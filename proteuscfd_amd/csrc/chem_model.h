@@ -112,5 +112,93 @@ __device__ inline void source_of(const ChemDev& m, const double* Q, double vol, 
   }
 }
 
+// wdot_of of a state and of its ns copies with one species density raised
+// (copy c holds rhop[c] in place of rhoi[c]).  The rate constants are
+// functions of T alone and a reaction's mass-action factors of the species
+// a copy leaves alone are the state's, so both are evaluated once per
+// reaction and shared; every copy's products and sums, in their order, are
+// wdot_of's, so w[(1 + c) * ns + i] equals wdot_of of copy c bit for bit
+// and w[i] that of the state.  w [(ns + 1) * ns].
+__device__ inline void wdot_rho_columns(const ChemDev& m, const double* rhoi, const double* rhop, double T,
+                                        double* w) {
+  const int ns = m.ns;
+  double X[PCFD_CHEM_MAXS], Xp[PCFD_CHEM_MAXS];
+  for (int k = 0; k < ns; ++k) {
+    X[k] = rhoi[k] / m.mw[k];
+    Xp[k] = rhop[k] / m.mw[k];
+  }
+  for (int k = 0; k < (ns + 1) * ns; ++k) w[k] = 0.0;
+  for (int r = 0; r < m.nr; ++r) {
+    const double Kf = kf_of(m, r, T);
+    const double Kb = Kf / kc_of(m, r, T);
+    const int nl = m.nloc[r];
+    double ff[PCFD_CHEM_MAXS], fd[PCFD_CHEM_MAXS], nu[PCFD_CHEM_MAXS];
+    for (int k = 0; k < nl; ++k) {
+      const double x = X[m.loc[r][k]];
+      ff[k] = powr(x, m.nup[r][k]);
+      fd[k] = powr(x, m.nupp[r][k]);
+    }
+    for (int i = 0; i < ns; ++i) {
+      int kk = -1;
+      for (int k = 0; k < nl; ++k)
+        if (m.loc[r][k] == i) {
+          kk = k;
+          break;
+        }
+      nu[i] = kk >= 0 ? m.nupp[r][kk] - m.nup[r][kk] : 0.0;
+    }
+    for (int s = 0; s <= ns; ++s) {
+      const int c = s - 1;  // -1: the state itself
+      double pf = 1.0, pd = 1.0;
+      for (int k = 0; k < nl; ++k) {
+        if (m.loc[r][k] == c) {
+          pf *= powr(Xp[c], m.nup[r][k]);
+          pd *= powr(Xp[c], m.nupp[r][k]);
+        } else {
+          pf *= ff[k];
+          pd *= fd[k];
+        }
+      }
+      double Mc = 1.0;
+      if (m.tb[r]) {
+        Mc = 0.0;
+        for (int k = 0; k < ns; ++k) Mc += k == c ? Xp[k] : X[k];
+        for (int k = 0; k < nl; ++k) {
+          const int l = m.loc[r][k];
+          Mc += (m.eff[r][k] - 1.0) * (l == c ? Xp[l] : X[l]);
+        }
+      }
+      const double net = Kf * pf - Kb * pd;
+      double* ws = w + s * ns;
+      for (int i = 0; i < ns; ++i) {
+        if (nu[i] == 0.0) continue;
+        double v = nu[i] * Mc * net;
+        v *= m.mw[i];
+        ws[i] += v;
+      }
+    }
+  }
+}
+
+// source_of of the FR state Q and of its ns copies with Q[c] raised by h
+// (the species columns of the forward-difference source Jacobian):
+// src[i] is the state's, src[(1 + c) * ns + i] copy c's, each bit for bit
+// what source_of gives.  src [(ns + 1) * ns].
+__device__ inline void source_rho_columns(const ChemDev& m, const double* Q, double h, double vol, double* src) {
+  const int ns = m.ns;
+  double rhoi[PCFD_CHEM_MAXS], rhop[PCFD_CHEM_MAXS];
+  const double T = Q[ns + 3] * m.ref_temperature;
+  for (int i = 0; i < ns; ++i) {
+    rhoi[i] = Q[i] * m.ref_density;
+    rhop[i] = (Q[i] + h) * m.ref_density;
+  }
+  wdot_rho_columns(m, rhoi, rhop, T, src);
+  for (int k = 0; k < (ns + 1) * ns; ++k) {
+    double v = src[k];
+    v /= (m.ref_density / m.ref_time);
+    src[k] = vol * v;
+  }
+}
+
 
 }  // namespace pcfd

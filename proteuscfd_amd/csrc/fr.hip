@@ -453,19 +453,20 @@ __global__ void __launch_bounds__(kFB) k_fr_node_terms(const Model* __restrict__
 #pragma unroll
   for (int k = 0; k < NE2; ++k) o[k] = T[k];
   if (!rxn) return;
-  double s0[NS], s1[NS], Q[NE];
+  double sc[(NS + 1) * NS], s1[NS], Q[NE];
 #pragma unroll
   for (int k = 0; k < NE; ++k) Q[k] = S.q[k];
-  source_of(*cm, Q, v, s0);
   const double h = 1.0e-8;
-  for (int c = 0; c <= NS; ++c) {
-    const int col = c < NS ? c : NS + 3;
-    const double keep = Q[col];
-    Q[col] += h;
-    source_of(*cm, Q, v, s1);
-    Q[col] = keep;
-    for (int j = 0; j < NS; ++j) o[NE2 + j * (NS + 1) + c] = (s1[j] - s0[j]) / h;
-  }
+  // the state and its species columns (Q[c] + h) share the T-only rate
+  // constants: the same values as NS + 1 separate source evaluations
+  source_rho_columns(*cm, Q, h, v, sc);
+  const double* s0 = sc;
+  for (int c = 0; c < NS; ++c)
+    for (int j = 0; j < NS; ++j) o[NE2 + j * (NS + 1) + c] = (sc[(1 + c) * NS + j] - s0[j]) / h;
+  // the T column
+  Q[NS + 3] += h;
+  source_of(*cm, Q, v, s1);
+  for (int j = 0; j < NS; ++j) o[NE2 + j * (NS + 1) + NS] = (s1[j] - s0[j]) / h;
 }
 
 // whether ContributeTemporalTerms adds to entry (r, c) of the diagonal block
