@@ -380,24 +380,30 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_edge(const Model* __restrict__ m
 // then each native variable of the node perturbed, the BC re-evaluated from
 // the perturbed state and the boundary flux differenced.  The node's
 // boundary blocks are summed in half-edge order into jdb.
+// Two passes: k_fr_jac_bnd_base walks each node's half-edges once, applies
+// the BC updates to q and records, per half-edge, the node's and the
+// boundary state's native values right after that half-edge's BC call and
+// the unperturbed flux (bhe, 3 NE values); k_fr_jac_bnd_cols gives each
+// (node, column) a thread that differences from those records and sums its
+// column over the half-edges in their order.  A State's auxiliary values are
+// always aux() of its native ones (load_state and bc_state recompute them),
+// so the records carry every bit the perturbed evaluations read.
 template <int NS>
-__global__ void __launch_bounds__(kFB) k_fr_jac_bnd(const Model* __restrict__ mp, int nbnd,
-                                                   const int* __restrict__ bnd, const int* __restrict__ he_ptr,
-                                                   const int* __restrict__ he_id, const int* __restrict__ he_tag,
-                                                   const double* __restrict__ he_a, const int* __restrict__ bc_type,
-                                                   const double* __restrict__ qref, int nnode, double beta,
-                                                   double* __restrict__ q, double* __restrict__ jdb) {
-  constexpr int NE = NS + 4, NE2 = NE * NE;
+__global__ void __launch_bounds__(kFB) k_fr_jac_bnd_base(const Model* __restrict__ mp, int nbnd,
+                                                        const int* __restrict__ bnd, const int* __restrict__ he_ptr,
+                                                        const int* __restrict__ he_id, const int* __restrict__ he_tag,
+                                                        const double* __restrict__ he_a,
+                                                        const int* __restrict__ bc_type,
+                                                        const double* __restrict__ qref, int nnode, double beta,
+                                                        double* __restrict__ q, double* __restrict__ bhe) {
+  constexpr int NE = NS + 4;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nbnd) return;
   const Model& m = *mp;
   const int i = bnd[t];
-  double* J = jdb + (long long)t * NE2;
-  for (int k = 0; k < NE2; ++k) J[k] = 0.0;
   State<NS> L;
   load_state<NS>(q, i, L, m);
   bool lchanged = false;
-  const double hh = 1.0e-8;
   for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
     const int h = he_id[p], tag = he_tag[h];
     const int ty = bc_type[tag];
@@ -413,18 +419,59 @@ __global__ void __launch_bounds__(kFB) k_fr_jac_bnd(const Model* __restrict__ mp
     }
     double fS[NE];
     fr::hllc<NS>(m, L, R, a, beta, fS);
-    for (int c = 0; c < NE; ++c) {
-      State<NS> PL = L, PR = R;
-      PL.q[c] += hh;
-      fr::aux(m, PL);
-      fr::bc_state<NS>(m, ty, PL, PR, Qr, a, beta);
-      double fL[NE];
-      fr::hllc<NS>(m, PL, PR, a, beta, fL);
+    double* rec = bhe + (long long)p * 3 * NE;
 #pragma unroll
-      for (int j = 0; j < NE; ++j) J[j * NE + c] += (fL[j] - fS[j]) / hh;
+    for (int k = 0; k < NE; ++k) {
+      rec[k] = L.q[k];
+      rec[NE + k] = R.q[k];
+      rec[2 * NE + k] = fS[k];
     }
   }
   if (lchanged) store_state<NS>(q, i, L);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kFB) k_fr_jac_bnd_cols(const Model* __restrict__ mp, int nbnd,
+                                                        const int* __restrict__ bnd, const int* __restrict__ he_ptr,
+                                                        const int* __restrict__ he_id, const int* __restrict__ he_tag,
+                                                        const double* __restrict__ he_a,
+                                                        const int* __restrict__ bc_type,
+                                                        const double* __restrict__ qref, double beta,
+                                                        const double* __restrict__ bhe, double* __restrict__ jdb) {
+  constexpr int NE = NS + 4, NE2 = NE * NE;
+  const long long t2 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t2 >= (long long)nbnd * NE) return;
+  const int t = (int)(t2 / NE), c = (int)(t2 % NE);
+  const Model& m = *mp;
+  const int i = bnd[t];
+  const double hh = 1.0e-8;
+  double acc[NE];
+#pragma unroll
+  for (int j = 0; j < NE; ++j) acc[j] = 0.0;
+  for (int p = he_ptr[i]; p < he_ptr[i + 1]; ++p) {
+    const int h = he_id[p], tag = he_tag[h];
+    const int ty = bc_type[tag];
+    const double* rec = bhe + (long long)p * 3 * NE;
+    State<NS> PL, PR, Qr;
+    load_state<NS>(rec, 0, PL, m);
+    load_state<NS>(rec + NE, 0, PR, m);
+    load_state<NS>(qref, tag, Qr, m);
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = he_a[4LL * h + k];
+    // PL.q[c] += hh without a dynamic register index
+#pragma unroll
+    for (int k = 0; k < NE; ++k) PL.q[k] = (k == c) ? PL.q[k] + hh : PL.q[k];
+    fr::aux(m, PL);
+    fr::bc_state<NS>(m, ty, PL, PR, Qr, a, beta);
+    double fL[NE];
+    fr::hllc<NS>(m, PL, PR, a, beta, fL);
+#pragma unroll
+    for (int j = 0; j < NE; ++j) acc[j] += (fL[j] - rec[2 * NE + j]) / hh;
+  }
+  double* J = jdb + (long long)t * NE2;
+#pragma unroll
+  for (int j = 0; j < NE; ++j) J[j * NE + c] = acc[j];
 }
 
 // Per node: the temporal block (ContributeTemporalTerms applied to a zero
@@ -779,7 +826,7 @@ struct pcfd_fr {
   long long bytes = 0;
   // device arrays
   double *q = nullptr, *qref = nullptr, *vol = nullptr, *edge_a = nullptr, *he_a = nullptr, *eflux = nullptr,
-         *elam = nullptr, *b = nullptr, *dt = nullptr, *jdb = nullptr, *red = nullptr, *aux = nullptr;
+         *elam = nullptr, *b = nullptr, *dt = nullptr, *jdb = nullptr, *bhe = nullptr, *red = nullptr, *aux = nullptr;
   int *edges = nullptr, *nptr = nullptr, *nslot_d = nullptr, *he_ptr = nullptr, *he_id = nullptr, *he_tag = nullptr,
       *bc_type = nullptr, *bnd = nullptr, *bidx = nullptr, *srow = nullptr, *swidth = nullptr, *sbase = nullptr,
       *scol = nullptr, *icount = nullptr;
@@ -819,7 +866,7 @@ void dfree(T*& p) {
 
 void free_all(pcfd_fr& f) {
   dfree(f.q); dfree(f.qref); dfree(f.vol); dfree(f.edge_a); dfree(f.he_a); dfree(f.eflux); dfree(f.elam);
-  dfree(f.b); dfree(f.dt); dfree(f.jdb); dfree(f.red); dfree(f.aux); dfree(f.edges); dfree(f.nptr);
+  dfree(f.b); dfree(f.dt); dfree(f.jdb); dfree(f.bhe); dfree(f.red); dfree(f.aux); dfree(f.edges); dfree(f.nptr);
   dfree(f.nslot_d); dfree(f.he_ptr); dfree(f.he_id); dfree(f.he_tag); dfree(f.bc_type); dfree(f.bnd);
   dfree(f.bidx); dfree(f.srow); dfree(f.swidth); dfree(f.sbase); dfree(f.scol); dfree(f.icount);
   dfree(f.piv); dfree(f.Md); dfree(f.Dd); dfree(f.xd); dfree(f.Mf); dfree(f.Df); dfree(f.xf); dfree(f.dmodel);
@@ -959,9 +1006,14 @@ void run_jacobian_t(pcfd_fr& f, TM* M, TM* D) {
     send(f, 6);
   }
   if (f.nbnd > 0) {
-    hipLaunchKernelGGL(k_fr_jac_bnd<NS5>, dim3(nblocks(f.nbnd)), dim3(kFB), 0, f.stream, f.dmodel, f.nbnd, f.bnd,
-                       f.he_ptr, f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref, f.nnode, f.prm.beta, f.q, f.jdb);
-    hcheck(hipGetLastError(), "k_fr_jac_bnd");
+    hipLaunchKernelGGL(k_fr_jac_bnd_base<NS5>, dim3(nblocks(f.nbnd)), dim3(kFB), 0, f.stream, f.dmodel, f.nbnd,
+                       f.bnd, f.he_ptr, f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref, f.nnode, f.prm.beta, f.q,
+                       f.bhe);
+    hcheck(hipGetLastError(), "k_fr_jac_bnd_base");
+    hipLaunchKernelGGL(k_fr_jac_bnd_cols<NS5>, dim3(nblocks((long long)f.nbnd * (NS5 + 4))), dim3(kFB), 0,
+                       f.stream, f.dmodel, f.nbnd, f.bnd, f.he_ptr, f.he_id, f.he_tag, f.he_a, f.bc_type, f.qref,
+                       f.prm.beta, f.bhe, f.jdb);
+    hcheck(hipGetLastError(), "k_fr_jac_bnd_cols");
   }
   hipLaunchKernelGGL(k_fr_node_terms<NS5>, dim3(nblocks(f.nnode)), dim3(kFB), 0, f.stream, f.dmodel,
                      chem_device_model(f.chem), f.nnode, f.q, f.vol, f.dt, f.prm.reactions, f.prm.beta, f.nterm);
@@ -1303,6 +1355,7 @@ int pcfd_fr_create(const pcfd_mesh_desc* m, pcfd_chem* chem, const double* qref,
     dalloc(*f, &f->b, (size_t)nn * NE);
     dalloc(*f, &f->dt, (size_t)nn);
     dalloc(*f, &f->jdb, (size_t)std::max(f->nbnd, 1) * NE2);
+    dalloc(*f, &f->bhe, (size_t)std::max(nb, 1) * 3 * NE);
     dalloc(*f, &f->red, (size_t)(NE + 1) * std::max(nblocks(nn), 256));
     dalloc(*f, &f->icount, 4);
     dalloc(*f, &f->piv, (size_t)f->nrow_pad);
