@@ -586,6 +586,52 @@ __global__ void __launch_bounds__(64 * (NS + 4)) k_fr_jac_diag(int nrow_pad, con
   for (int c = 0; c < NE; ++c) Dr[(long long)(rr * NE + c) * kLane] = (TM)A[c];
 }
 
+// fr::lu (LU with logical pivoting, ucs/matrix.h:184-235) with the rows
+// kept physically in pivot order: on return b[k * N + c] is lu's
+// a[p[k] * N + c].  Every element meets the same operations in the same
+// order, so the factor is the same bits, while all indices are static and
+// the block stays in registers (lu's a[p[j] * N + i] sends it to scratch);
+// a row swap is a select over the static row indices.  As in lu, the pivot
+// row index carries over from the previous column when a column holds no
+// nonzero, so the swap partner may be an earlier row.
+template <int N, typename T>
+__device__ __forceinline__ void lu_rows(T* b, int* p) {
+  int row = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) p[i] = i;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    T large = 0.0;
+#pragma unroll
+    for (int j = i; j < N; ++j)
+      if (fabs(b[j * N + i]) > fabs(large)) {
+        large = b[j * N + i];
+        row = j;
+      }
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      if (j == i) continue;
+      const bool s = (j == row);
+      const int pi = p[i], pj = p[j];
+      p[i] = s ? pj : pi;
+      p[j] = s ? pi : pj;
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        const T x = b[i * N + k], y = b[j * N + k];
+        b[i * N + k] = s ? y : x;
+        b[j * N + k] = s ? x : y;
+      }
+    }
+    large = (T)1.0 / large;
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) b[j * N + i] *= large;
+#pragma unroll
+    for (int j = i + 1; j < N; ++j)
+#pragma unroll
+      for (int k = i + 1; k < N; ++k) b[j * N + k] -= b[j * N + i] * b[i * N + k];
+  }
+}
+
 // PrepareSGS (ucs/crsmatrix.tcc:839-876): LU with logical pivoting of every
 // diagonal block, in place; the row order is packed 4 bits per entry.
 template <int NS, typename TM>
@@ -594,41 +640,20 @@ __global__ void __launch_bounds__(64) k_fr_prepare(int nrow_pad, TM* __restrict_
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrow_pad) return;
   TM* Dr = D + (long long)(r >> 6) * NE2 * kLane + (r & 63);
-  double A[NE2];
-  for (int k = 0; k < NE2; ++k) A[k] = (double)Dr[(long long)k * kLane];
+  // in the storage precision: the FP32 form factors in FP32 arithmetic
+  TM A[NE2];
+#pragma unroll
+  for (int k = 0; k < NE2; ++k) A[k] = Dr[(long long)k * kLane];
   int p[NE];
-  if constexpr (sizeof(TM) == 8) {
-    fr::lu<NE>(A, p);
-  } else {
-    // FP32 form: factor in FP32 arithmetic
-    float Af[NE2];
-    for (int k = 0; k < NE2; ++k) Af[k] = (float)A[k];
-    int row = 0;
-    for (int k = 0; k < NE; ++k) p[k] = k;
-    for (int c = 0; c < NE; ++c) {
-      float large = 0.0f;
-      for (int j = c; j < NE; ++j)
-        if (fabsf(Af[p[j] * NE + c]) > fabsf(large)) {
-          large = Af[p[j] * NE + c];
-          row = j;
-        }
-      const int tt = p[c];
-      p[c] = p[row];
-      p[row] = tt;
-      large = 1.0f / large;
-      for (int j = c + 1; j < NE; ++j) Af[p[j] * NE + c] *= large;
-      for (int j = c + 1; j < NE; ++j)
-        for (int k = c + 1; k < NE; ++k) Af[p[j] * NE + k] -= Af[p[j] * NE + c] * Af[p[c] * NE + k];
-    }
-    for (int k = 0; k < NE2; ++k) A[k] = Af[k];
-  }
+  lu_rows<NE, TM>(A, p);
   long long pk = 0;
+#pragma unroll
   for (int k = 0; k < NE; ++k) pk |= (long long)p[k] << (4 * k);
   piv[r] = pk;
   // stored with the rows in pivot order (row k holds factor row p[k]), so the
   // solve indexes the factor statically
-  for (int k = 0; k < NE; ++k)
-    for (int c = 0; c < NE; ++c) Dr[(long long)(k * NE + c) * kLane] = (TM)A[p[k] * NE + c];
+#pragma unroll
+  for (int k = 0; k < NE2; ++k) Dr[(long long)k * kLane] = A[k];
 }
 
 // LuSolve (ucs/matrix.h:237-264) on a factor stored with its rows in pivot
